@@ -1,0 +1,146 @@
+"""Benchmark of the FactSieve GPU sieve on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c1|c3]
+
+A step is one whole plain sieve job of the workload through the C ABI
+(fs_sieve: segmented candidate generation, nmin! setup, the n-iteration with
++-1 capture, the checksum fold, the result drain and CPU re-verification of
+every factor).  The default workload is the production-shape single job
+n in [1e5, 2e5), p in [1e12, 1e12 + 1e9) (36,191,004 candidates, 3.619e12
+p*n residue steps per job).  `value` is p*n residue steps per second of whole
+jobs.  One JSON line is printed.
+
+roofline: the dominant kernel is the residue iteration.  Its work is counted
+as 7 VALU lane-operations per p*n step (the FP64 formulation: 2 multiplies,
+2 FMAs, 1 add, 1 round, 1 compare), timed with HIP events on the launch
+stream; peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
+vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
+int32_mul_equiv: the same steps/s expressed in the 11 32-bit multiplies a
+64-bit Montgomery step needs, over the measured int32-multiply peak (full
+VALU rate on gfx950).
+cpu_baseline: the CPU oracle (a C/OpenMP port of the reference algorithm) on a
+bounded p-subrange of the same workload, on this host's cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WORKLOADS = {
+    # name: (pmin, pmax, nmin, nmax, description)
+    "c2": (10 ** 12, 10 ** 12 + 10 ** 9, 100000, 200000,
+           "plain single job at production shape: n in [1e5,2e5), p in [1e12,1e12+1e9)"),
+    "c1": (10 ** 8, 10 ** 8 + 2 * 10 ** 6, 1000, 10000,
+           "plain sieve at self-test scale: n in [1e3,1e4), p in [1e8,1e8+2e6)"),
+    "c3": (2 ** 62, 2 ** 62 + 2 ** 30, 500000, 510000,
+           "full-width modulus stress: n in [5e5,5.1e5), p in [2^62,2^62+2^30)"),
+}
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine clock
+OPS_PER_STEP = {1: 7, 2: 14, 3: 45}             # VALU lane-ops per p*n step by path
+INT32_MUL_PER_STEP = 11                         # 64-bit Montgomery step: 4 + 3 + 4 partial products
+INT32_MUL_PEAK = VALU_PEAK_TOPS * 1e12          # full-rate v_mad_u64_u32 / v_mul_*_u32 (measured)
+# HBM bytes per candidate per iterate launch, from rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE on the c2 workload, scaled by the copy-kernel calibration factor.
+TRAFFIC_BYTES_PER_CAND = None
+
+
+def cpu_baseline(pmin, nmin, nmax, target_s=15.0):
+    """Time the oracle on a p-subrange sized for ~target_s seconds of CPU work."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol  # the oracle is used here only as the CPU baseline
+
+    threads = min(os.cpu_count() or 1, 16)
+    span = 2 * 10 ** 5
+    t = time.time()
+    ol.sieve(pmin, pmin + span, nmin, nmax, threads=threads)
+    dt = time.time() - t
+    span = int(min(10 ** 9, max(span, span * target_s / max(dt, 1e-3))))
+    t = time.time()
+    r = ol.sieve(pmin, pmin + span, nmin, nmax, threads=threads)
+    dt = time.time() - t
+    steps = r["primecount"] * (nmax - nmin)
+    return {"value": steps / dt, "unit": "p*n steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle (C/OpenMP port of the reference algorithm) on p in [{pmin}, {pmin + span}), "
+                      f"n in [{nmin}, {nmax}): {r['primecount']} candidates, {steps:.3e} p*n steps in {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the sieve shards over p across GPUs with a host-side checksum sum, not run here)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import factsieve_amd as fs
+    fs.init(0)
+    pmin, pmax, nmin, nmax, desc = WORKLOADS[args.workload]
+    for _ in range(args.warmup):
+        fs.sieve(pmin, pmax, nmin, nmax, verify_threads=16)
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        results.append(fs.sieve(pmin, pmax, nmin, nmax, verify_threads=16))
+    t1 = time.perf_counter()
+    ref = results[0]
+    for r in results[1:]:
+        assert (r.checksum, r.factorcount, r.primecount) == (ref.checksum, ref.factorcount, ref.primecount)
+    elapsed = t1 - t0
+    steps_total = sum(r.steps for r in results)
+    value = steps_total / elapsed
+
+    it_ms = sum(r.kernel_ms_iterate for r in results)
+    launches = sum(r.iterate_launches for r in results)
+    avg_launch_ms = it_ms / max(launches, 1)
+    ops = OPS_PER_STEP[ref.path]
+    achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
+    traffic = None
+    if TRAFFIC_BYTES_PER_CAND is not None:
+        traffic = TRAFFIC_BYTES_PER_CAND * ref.primecount / max(ref.iterate_launches, 1)
+    line = {
+        "metric": "p*n residue steps/s (plain single job)",
+        "value": value,
+        "unit": "p*n steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {1: "f64 (exact integer residues)", 2: "u64+f64", 3: "u64"}[ref.path],
+        "data": "synthetic (the workload is a numeric range; no data set)",
+        "config": {"workload": desc, "pmin": pmin, "pmax": pmax, "nmin": nmin, "nmax": nmax,
+                   "candidates": ref.primecount, "pn_steps_per_job": ref.steps, "factors": ref.factorcount,
+                   "checksum": f"{ref.checksum:016X}", "path": {1: "fp64", 2: "hybrid", 3: "montgomery"}[ref.path],
+                   "segments": ref.segments, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "kernel": "k_iterate_" + {1: "fp64", 2: "hybrid", 3: "mont"}[ref.path],
+                     "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s (VALU lane-ops)",
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic,
+                     "ops_per_step": ops, "avg_launch_ms": avg_launch_ms,
+                     "kernel_steps_per_s": steps_total / (it_ms * 1e-3)},
+        "int32_mul_equiv": {"frac": steps_total / (it_ms * 1e-3) * INT32_MUL_PER_STEP / INT32_MUL_PEAK,
+                            "muls_per_step": INT32_MUL_PER_STEP, "peak_ops_per_s": INT32_MUL_PEAK},
+        "kernel_ms_per_job": {"iterate": it_ms / args.steps,
+                              "setup": sum(r.kernel_ms_setup for r in results) / args.steps,
+                              "primegen": sum(r.kernel_ms_primegen for r in results) / args.steps},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(pmin, nmin, nmax)
+        line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,78 @@
+"""Build the gfx950 library, the command-line executable and (as test
+infrastructure) the CPU oracle.
+
+    python -m factsieve_amd.build        # or __graft_entry__.build()
+
+Outputs, all in-tree so they travel with the repository to a GPU machine:
+    factsieve_amd/libfactsieve.so   C ABI of include/factsieve.h (HIP kernels + host driver)
+    factsieve_amd/factsieve         drop-in command line (-n -N -p -P -v -s)
+    oracle/liboracle.so             CPU oracle (tests and the CPU baseline only)
+    oracle/_ref/libfsref.so         reference verifyprime.c + putil.c (only where
+                                    the reference checkout is present)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libfactsieve.so")
+EXE = os.path.join(HERE, "factsieve")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("FS_OFFLOAD_ARCH", "gfx950")
+
+# -packed-fp32-ops: never emit v_pk_{add,mul,fma}_f32 (the host pass warns that
+# it does not know the feature; that warning is harmless).
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+SOURCES = ["fs_device.hip", "fs_host.cpp", "fs_hostemu.cpp", "fs_batch.cpp", "fs_server.cpp"]
+
+
+def _run(cmd, cwd=None):
+    res = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in res.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if res.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return out
+
+
+def build_library(verbose=False):
+    objs = []
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(HERE, "_build", os.path.splitext(src)[0] + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        out = _run([HIPCC] + HIP_FLAGS + lang + ["-c", path, "-o", obj])
+        if verbose and out.strip():
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    os.replace(tmp, LIB)
+    _run([HIPCC, "-O2", "-std=c++17", "-o", EXE, os.path.join(CSRC, "fs_main.cpp"), "-L", HERE,
+          "-lfactsieve", "-Wl,-rpath,$ORIGIN"])
+    return LIB
+
+
+def build_oracle():
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-s", "-C", odir, "all"])
+
+
+def build(verbose=False):
+    build_library(verbose)
+    build_oracle()
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    print("built", LIB)

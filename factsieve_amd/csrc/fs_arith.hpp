@@ -1,0 +1,180 @@
+// fs_arith.hpp -- exact modular arithmetic for the sieve kernels (gfx950).
+//
+// Three formulations of "r <- r * m mod p" are provided; all return exact
+// residues, so every kernel built on them produces the same factors and
+// checksum (the reference's results do not depend on how a residue is
+// computed, only on its value: cl_sieve.cpp / check.cl fold canonical values).
+//
+//  * Montgomery REDC, R = 2^64, q = p^-1 mod 2^64, subtractive form with a
+//    canonical result -- the reference formulation (iterate.cl:19-47).  Valid for
+//    every odd p < 2^64.  64x64->128 products are built from 32x32->64
+//    v_mad_u64_u32 partial products.
+//
+//  * FP64 error-free product with a rounded quotient (p < 2^52): the residue is
+//    a double s with |s| <= 0.75 p; for an integer multiplier m < 2^48,
+//      h = fl(s*m), l = fma(s, m, -h)        (h + l == s*m exactly)
+//      k = rint(fl(h * (1/p)))               (|k - s*m/p| <= 0.57)
+//      s' = fma(-k, p, h) + l                (exact: |fma result| < 2^53)
+//    gives s' == s*m - k*p exactly with |s'| <= 0.57 p.  Because |s| < p - 1,
+//    s == +-1 (mod p) iff |s| == 1.  Seven full-rate FP64 VALU operations.
+//
+//  * Hybrid (p < 2^63): lazily reduced s in [0, 2p), multiplier m < 2^31; the
+//    product s*m mod 2^64 comes from two v_mad_u64_u32, the quotient from an
+//    FP64 estimate biased low by 1/256 so k in {floor(X)-1, floor(X)}, and
+//    s' = s*m - k*p lands in [0, 2p) exactly (computed mod 2^64).
+//
+// The error bounds are proved in the design notes shipped with the library and
+// are exercised by the parity tests at the edges of each range.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define FS_HD __host__ __device__ __forceinline__
+
+// ------------------------------------------------------------------ Montgomery
+
+// p^-1 mod 2^64 by Newton iteration (each step doubles the correct low bits).
+FS_HD uint64_t fs_invert(uint64_t p) {
+  uint64_t x = p;                 // p*p == 1 mod 8 for odd p: 3 correct bits
+  for (int i = 0; i < 5; ++i) x *= 2 - p * x;  // 3 -> 6 -> 12 -> 24 -> 48 -> 96
+  return x;
+}
+
+// 64x64 -> 128 product from four 32x32->64 partial products.
+FS_HD void fs_mul_wide(uint64_t a, uint64_t b, uint64_t& lo, uint64_t& hi) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+  const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+  const uint64_t p00 = (uint64_t)a0 * b0;
+  const uint64_t p01 = (uint64_t)a0 * b1 + (p00 >> 32);
+  const uint64_t p10 = (uint64_t)a1 * b0 + (uint32_t)p01;
+  hi = (uint64_t)a1 * b1 + (p01 >> 32) + (p10 >> 32);
+  lo = (p10 << 32) | (uint32_t)p00;
+}
+
+// High 64 bits of a 64x64 product.
+FS_HD uint64_t fs_mul_hi(uint64_t a, uint64_t b) {
+  uint64_t lo, hi;
+  fs_mul_wide(a, b, lo, hi);
+  return hi;
+}
+
+// a*b*2^-64 mod p, canonical in [0, p) when a*b < p*2^64.
+FS_HD uint64_t fs_mont_mul(uint64_t a, uint64_t b, uint64_t p, uint64_t q) {
+  uint64_t lo, hi;
+  fs_mul_wide(a, b, lo, hi);
+  const uint64_t m = lo * q;
+  const uint64_t mp = fs_mul_hi(m, p);
+  const uint64_t r = hi - mp;
+  return (hi < mp) ? r + p : r;
+}
+
+// (a + b) mod p for a, b in [0, p), overflow-safe.
+FS_HD uint64_t fs_add_mod(uint64_t a, uint64_t b, uint64_t p) {
+  const uint64_t c = (a >= p - b) ? p : 0;
+  return a + b - c;
+}
+
+// Plain-domain product a*b mod p (a, b canonical) via two Montgomery products.
+FS_HD uint64_t fs_mul_mod(uint64_t a, uint64_t b, uint64_t p, uint64_t q, uint64_t r2) {
+  return fs_mont_mul(fs_mont_mul(a, b, p, q), r2, p, q);
+}
+
+// 2^128 mod p from one = 2^64 mod p: (4*one)^(2^5) in Montgomery form.
+FS_HD uint64_t fs_r2_from_one(uint64_t one, uint64_t p, uint64_t q) {
+  const uint64_t two = fs_add_mod(one, one, p);
+  uint64_t r2 = fs_add_mod(two, two, p);
+  for (int i = 0; i < 5; ++i) r2 = fs_mont_mul(r2, r2, p, q);
+  return r2;
+}
+
+// ------------------------------------------------------------------ FP64 path
+
+// s*m mod p for |s| <= 0.75p, integer 0 <= m < 2^48, p < 2^52 (see header).
+FS_HD double fs_fp_mulmod(double s, double m, double p, double pinv) {
+#pragma clang fp contract(off)
+  const double h = s * m;
+  const double l = __builtin_fma(s, m, -h);
+  const double k = __builtin_rint(h * pinv);
+  const double r = __builtin_fma(-k, p, h);
+  return r + l;
+}
+
+// Canonical [0, p) integer from a symmetric FP64 residue.
+FS_HD uint64_t fs_fp_to_canon(double s, uint64_t p) {
+  const int64_t v = (int64_t)s;
+  return v < 0 ? (uint64_t)(v + (int64_t)p) : (uint64_t)v;
+}
+
+// Symmetric FP64 residue (|s| <= p/2) from a canonical value.
+FS_HD double fs_canon_to_fp(uint64_t r, uint64_t p) {
+  return (r > (p >> 1)) ? -(double)(p - r) : (double)r;
+}
+
+// ----------------------------------------------------------------- hybrid path
+
+// s*m mod p with s in [0, 2p), 0 <= m < 2^31, p < 2^63; result in [0, 2p).
+// pinv = fl(1/p), pinv32 = 2^32 * pinv, md = (double)m.
+FS_HD uint64_t fs_hy_mulmod(uint64_t s, uint32_t m, double md, uint64_t p,
+                                                 double pinv, double pinv32) {
+#pragma clang fp contract(off)
+  const uint32_t s0 = (uint32_t)s, s1 = (uint32_t)(s >> 32);
+  // quotient estimate X ~ s*m/p, biased low so that k in {floor(X)-1, floor(X)}
+  const double y = __builtin_fma((double)s1, pinv32, (double)s0 * pinv);
+  double x = __builtin_fma(y, md, -0.00390625);
+  x = __builtin_fmax(x, 0.0);
+  const uint32_t k = (uint32_t)x;
+  // s*m - k*p mod 2^64 (exact value lies in [0, 2p))
+  const uint64_t t0 = (uint64_t)s0 * m;
+  const uint64_t t = ((uint64_t)((uint32_t)(t0 >> 32) + s1 * m) << 32) | (uint32_t)t0;
+  const uint64_t kp = k * p;
+  return t - kp;
+}
+
+// ------------------------------------------------------------ candidate tests
+
+// Base-2 strong probable-prime test for odd N > 2 (any N < 2^64), Montgomery
+// arithmetic with q = N^-1 mod 2^64 and one = 2^64 mod N (getsegprimes.cl:79-108,
+// left-to-right square-and-double over the odd part d of N-1).
+FS_HD bool fs_spsp2(uint64_t N, uint64_t q, uint64_t one) {
+  const uint64_t nmo = N - one;
+  uint64_t d = N - 1;
+  const int t = __builtin_ctzll(d);
+  d >>= t;
+  uint64_t a = fs_add_mod(one, one, N);  // 2 in Montgomery form
+  if (d > 1) {
+    const int top = 63 - __builtin_clzll(d);
+    for (int b = top - 1; b >= 0; --b) {
+      a = fs_mont_mul(a, a, N, q);
+      if ((d >> b) & 1) a = fs_add_mod(a, a, N);
+    }
+  }
+  if (a == one || a == nmo) return true;
+  for (int s = 1; s < t; ++s) {
+    a = fs_mont_mul(a, a, N, q);
+    if (a == nmo) return true;
+  }
+  return false;
+}
+
+// Odd primes 3..113 removed before the probable-prime test: 3 and 5 (the
+// reference's mod-30 wheel) and 7..113 (its bit tables, getsegprimes.cl:115-167).
+#define FS_NSMALL 29
+#define FS_NMASK 1591  // sum of the 29 primes: one 64-bit mask per residue class
+static constexpr uint32_t kSmallQ[FS_NSMALL] = {3,  5,  7,  11, 13, 17, 19, 23, 29, 31,
+                                                37, 41, 43, 47, 53, 59, 61, 67, 71, 73,
+                                                79, 83, 89, 97, 101, 103, 107, 109, 113};
+FS_HD constexpr uint32_t fs_small_off(int j) {
+  uint32_t o = 0;
+  for (int k = 0; k < j; ++k) o += kSmallQ[k];
+  return o;
+}
+static_assert(fs_small_off(FS_NSMALL) == FS_NMASK, "mask table size");
+
+// Mask of residue class r of prime q: bit i set iff q divides B + 2i for odd
+// B = r (mod q), i.e. i = i0 (mod q) with i0 = (q - r) * 2^-1 mod q.
+FS_HD uint64_t fs_small_mask(uint32_t q, uint32_t r) {
+  const uint32_t i0 = ((q - r) % q) * ((q + 1) >> 1) % q;
+  uint64_t m = 0;
+  for (uint32_t i = i0; i < 64; i += q) m |= (uint64_t)1 << i;
+  return m;
+}

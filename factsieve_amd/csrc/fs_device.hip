@@ -1,0 +1,799 @@
+// fs_device.hip -- gfx950 kernels of the FactSieve sieve and the C-ABI runtime
+// layer that replaces the reference's OpenCL stack (simpleCL.c + raw cl* calls).
+//
+// Kernels (reference counterparts in kernels/*.cl):
+//   k_clearn / k_clearresult     clearn.cl, clearresult.cl
+//   k_getsegprimes               getsegprimes.cl  (113-sieve + base-2 SPRP, bounded append)
+//   k_setup_{fp64,hybrid,mont}   setup.cl         (nmin! mod p)
+//   k_iterate_{fp64,hybrid,mont} iterate.cl       (n! mod p for n in [startN, endN), +-1 capture)
+//   k_check                      check.cl         (checksum fold, count, validation)
+//   k_verify{slow,pow,reduce,result} verify*.cl   (nmin! cross-check on one candidate)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <mutex>
+#include "fs_arith.hpp"
+#include "../../include/factsieve.h"
+
+// ===================================================================== errors
+
+static thread_local std::string g_last_error;
+static void set_error(const char* what, hipError_t e) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  g_last_error = buf;
+}
+extern "C" void fs_set_error_msg(const char* msg) { g_last_error = msg ? msg : ""; }
+extern "C" const char* fs_last_error(void) { return g_last_error.c_str(); }
+
+#define HIPCHK(call, what)              \
+  do {                                  \
+    hipError_t e__ = (call);            \
+    if (e__ != hipSuccess) {            \
+      set_error(what, e__);             \
+      return FS_E_HIP;                  \
+    }                                   \
+  } while (0)
+
+static hipStream_t S(fs_stream s) { return (hipStream_t)s; }
+
+// ================================================================== constants
+
+#define FS_BLOCK 256
+#define FS_VERIFY_BLOCKS 10000u
+#define FS_VERIFY_L2 40u
+
+
+// ============================================================ device helpers
+
+__device__ __forceinline__ uint32_t fs_live(const uint32_t* ctr, uint32_t cap) {
+  const uint32_t n = ctr[FS_CTR_COUNT];
+  return n < cap ? n : cap;
+}
+
+__device__ __forceinline__ void fs_record(uint32_t* ctr, fs_factor* fac, uint32_t cap, uint64_t p,
+                                          int32_t nc) {
+  const uint32_t idx = atomicAdd(&ctr[FS_CTR_NFACTORS], 1u);
+  if (idx < cap) {
+    fac[idx].p = p;
+    fac[idx].nc = nc;
+    fac[idx].pad = 0;
+  } else {
+    atomicOr(&ctr[FS_CTR_FACTOROVF], 1u);
+  }
+}
+
+// Wave64 sum of a 64-bit value (result valid in every lane).
+__device__ __forceinline__ uint64_t fs_wave_sum(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ================================================================== kernels
+
+__global__ void k_clearn(uint32_t* ctr) {
+  if (threadIdx.x == 0) ctr[FS_CTR_COUNT] = 0;
+}
+
+__global__ void k_clearresult(uint32_t* ctr, uint64_t* sum) {
+  const uint32_t t = threadIdx.x;
+  if (t < FS_SUM_WORDS) sum[t] = 0;
+  if (t >= 1 && t < FS_CTR_WORDS) ctr[t] = 0;
+}
+
+__global__ void k_set_flag(uint32_t* ctr, uint32_t idx) {
+  if (threadIdx.x == 0) atomicOr(&ctr[idx], 1u);
+}
+
+// Segmented candidate generation over [low, high).  Thread t owns the 64 odd
+// numbers base + 2i + 128t (i < 64) as one bit mask: the masks of the 29 small
+// primes are OR-ed from an LDS table indexed by B mod q.  Each wave then deals
+// its survivors out round-robin (exclusive prefix of popcounts + select-bit) so
+// that all 64 lanes run the probable-prime test on consecutive survivors.
+__global__ __launch_bounds__(FS_BLOCK) void k_getsegprimes(uint64_t base, uint64_t high, fs_cands c,
+                                                           uint32_t* ctr) {
+  __shared__ uint64_t s_tab[FS_NMASK];
+  __shared__ uint64_t s_mask[FS_BLOCK];
+  __shared__ uint32_t s_excl[FS_BLOCK];
+  for (uint32_t k = threadIdx.x; k < FS_NMASK; k += FS_BLOCK) {
+    int j = 0;
+    while (j + 1 < FS_NSMALL && fs_small_off(j + 1) <= k) ++j;
+    s_tab[k] = fs_small_mask(kSmallQ[j], k - fs_small_off(j));
+  }
+  __syncthreads();
+
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wbase = threadIdx.x & ~63u;
+  const uint64_t t = (uint64_t)blockIdx.x * FS_BLOCK + threadIdx.x;
+  const uint64_t span = high > base ? high - base : 0;  // base is odd, <= high
+  uint64_t surv = 0;
+  uint64_t B = 0;
+  if (t * 128 < span) {
+    B = base + t * 128;
+    uint64_t comp = 0;
+#pragma unroll
+    for (int j = 0; j < FS_NSMALL; ++j) comp |= s_tab[fs_small_off(j) + (uint32_t)(B % kSmallQ[j])];
+    surv = ~comp;
+    const uint64_t left = (high - B + 1) >> 1;  // odd numbers B + 2i < high
+    if (left < 64) surv &= (((uint64_t)1 << left) - 1);
+    if (B <= 113) {  // the small primes themselves are never candidates
+      for (uint32_t i = 0; i < 64; ++i)
+        if (B + 2 * i <= 113) surv &= ~((uint64_t)1 << i);
+    }
+  }
+  s_mask[threadIdx.x] = surv;
+  // wave-inclusive scan of popcounts
+  const uint32_t cnt = __popcll(surv);
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o, 64);
+    if (lane >= (uint32_t)o) incl += v;
+  }
+  s_excl[threadIdx.x] = incl - cnt;
+  const uint32_t total = __shfl(incl, 63, 64);
+  __syncthreads();
+
+  const uint64_t tw = (uint64_t)blockIdx.x * FS_BLOCK + wbase;
+  for (uint32_t k = lane; k < total; k += 64) {
+    // owner lane: last L with excl[L] <= k
+    uint32_t lo = 0, hi = 63;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (s_excl[wbase + mid] <= k) lo = mid; else hi = mid - 1;
+    }
+    uint64_t m = s_mask[wbase + lo];
+    for (uint32_t j = k - s_excl[wbase + lo]; j > 0; --j) m &= m - 1;
+    const uint32_t bit = __builtin_ctzll(m);
+    const uint64_t N = base + (tw + lo) * 128 + 2 * (uint64_t)bit;
+    const uint64_t q = fs_invert(N);
+    const uint64_t one = (0 - N) % N;
+    if (fs_spsp2(N, q, one)) {
+      const uint64_t r2 = fs_r2_from_one(one, N, q);
+      const uint32_t idx = atomicAdd(&ctr[FS_CTR_COUNT], 1u);
+      if (idx < c.cap) {
+        c.p[idx] = N;
+        c.q[idx] = q;
+        c.r2[idx] = r2;
+        c.res[idx] = 0;
+        c.nnext[idx] = 0;
+      } else {
+        atomicOr(&ctr[FS_CTR_PRIMEOVF], 1u);
+      }
+    }
+  }
+}
+
+// ---- setup: res <- res * prod(program[start..end)), program word 0 = square
+
+__global__ __launch_bounds__(FS_BLOCK) void k_setup_fp64(fs_cands c, const uint32_t* ctr,
+                                                         const double* prog, uint32_t start,
+                                                         uint32_t end, uint32_t nmin) {
+  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
+  if (i >= fs_live(ctr, c.cap)) return;
+  const uint64_t p = c.p[i];
+  const double pd = (double)p, pinv = 1.0 / pd;
+  double s = (start == 0) ? 1.0 : fs_canon_to_fp(c.res[i], p);
+  for (uint32_t k = start; k < end; ++k) {
+    const double m = prog[k];
+    if (m == 0.0) {
+      const uint64_t q = c.q[i], r2 = c.r2[i];
+      const uint64_t a = fs_fp_to_canon(s, p);
+      s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+    } else {
+      s = fs_fp_mulmod(s, m, pd, pinv);
+    }
+  }
+  c.res[i] = fs_fp_to_canon(s, p);
+  c.nnext[i] = nmin;
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_setup_hybrid(fs_cands c, const uint32_t* ctr,
+                                                           const uint64_t* prog, uint32_t start,
+                                                           uint32_t end, uint32_t nmin) {
+  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
+  if (i >= fs_live(ctr, c.cap)) return;
+  const uint64_t p = c.p[i];
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+  uint64_t s = (start == 0) ? 1 : c.res[i];
+  for (uint32_t k = start; k < end; ++k) {
+    const uint32_t m = (uint32_t)prog[k];
+    if (m == 0) {
+      const uint64_t q = c.q[i], r2 = c.r2[i];
+      const uint64_t a = s >= p ? s - p : s;
+      s = fs_mul_mod(a, a, p, q, r2);
+    } else {
+      s = fs_hy_mulmod(s, m, (double)m, p, pinv, pinv32);
+    }
+  }
+  c.res[i] = s >= p ? s - p : s;
+  c.nnext[i] = nmin;
+}
+
+// Reference power-table evaluation (setup.cl:49-103) on the Montgomery path.
+// prog = [bases[0..cnt) | (exp | curbit<<32)[0..cnt)].
+__global__ __launch_bounds__(FS_BLOCK) void k_setup_mont(fs_cands c, const uint32_t* ctr,
+                                                         const uint64_t* prog, uint32_t cnt,
+                                                         uint32_t start, uint32_t end, uint32_t nmin) {
+  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
+  if (i >= fs_live(ctr, c.cap)) return;
+  const uint64_t p = c.p[i], q = c.q[i], r2 = c.r2[i];
+  const uint64_t one = fs_mont_mul(1, r2, p, q);
+  uint64_t s6 = (start == 0) ? one : fs_mont_mul(c.res[i], r2, p, q);
+  uint32_t k = start;
+  if (k == 0 && end > 0) {  // base 2: square-and-double
+    const uint64_t w = prog[cnt];
+    const uint32_t e = (uint32_t)w;
+    uint32_t cb = (uint32_t)(w >> 32);
+    uint64_t a = fs_add_mod(one, one, p);
+    while (cb) {
+      a = fs_mont_mul(a, a, p, q);
+      if (e & cb) a = fs_add_mod(a, a, p);
+      cb >>= 1;
+    }
+    s6 = a;
+    k = 1;
+  }
+  for (; k < end; ++k) {
+    const uint64_t w = prog[cnt + k];
+    const uint32_t e = (uint32_t)w;
+    uint32_t cb = (uint32_t)(w >> 32);
+    const uint64_t b = fs_mont_mul(prog[k], r2, p, q);
+    uint64_t a = b;
+    if (e != 1) {
+      while (cb) {
+        a = fs_mont_mul(a, a, p, q);
+        if (e & cb) a = fs_mont_mul(a, b, p, q);
+        cb >>= 1;
+      }
+    }
+    s6 = fs_mont_mul(s6, a, p, q);
+  }
+  c.res[i] = fs_mont_mul(s6, 1, p, q);
+  c.nnext[i] = nmin;
+}
+
+// ---- iterate: for n in [n0, n1): capture n! = +-1, then multiply by n+1
+
+__global__ __launch_bounds__(FS_BLOCK) void k_iterate_fp64(fs_cands c, uint32_t* ctr, fs_factor* fac,
+                                                           uint32_t fcap, uint32_t n0, uint32_t n1) {
+  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
+  if (i >= fs_live(ctr, c.cap)) return;
+  const uint64_t p = c.p[i];
+  if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
+  const double pd = (double)p, pinv = 1.0 / pd;
+  double s = fs_canon_to_fp(c.res[i], p);
+  for (uint32_t n = n0; n < n1; ++n) {
+    if (__builtin_fabs(s) == 1.0) fs_record(ctr, fac, fcap, p, s > 0.0 ? -(int32_t)n : (int32_t)n);
+    s = fs_fp_mulmod(s, (double)(n + 1), pd, pinv);
+  }
+  c.res[i] = fs_fp_to_canon(s, p);
+  c.nnext[i] = n1;
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_iterate_hybrid(fs_cands c, uint32_t* ctr, fs_factor* fac,
+                                                             uint32_t fcap, uint32_t n0, uint32_t n1) {
+  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
+  if (i >= fs_live(ctr, c.cap)) return;
+  const uint64_t p = c.p[i];
+  if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+  const uint64_t pp1 = p + 1, pm1 = p - 1, p2m1 = 2 * p - 1;
+  uint64_t s = c.res[i];
+  for (uint32_t n = n0; n < n1; ++n) {
+    const bool plus1 = (s == 1) | (s == pp1);
+    const bool minus1 = (s == pm1) | (s == p2m1);
+    if (plus1 | minus1) fs_record(ctr, fac, fcap, p, plus1 ? -(int32_t)n : (int32_t)n);
+    const uint32_t m = n + 1;
+    s = fs_hy_mulmod(s, m, (double)m, p, pinv, pinv32);
+  }
+  c.res[i] = s >= p ? s - p : s;
+  c.nnext[i] = n1;
+}
+
+// Reference formulation (iterate.cl:54-80): Montgomery residue s6 and
+// Montgomery counter s7 advanced by one each step.
+__global__ __launch_bounds__(FS_BLOCK) void k_iterate_mont(fs_cands c, uint32_t* ctr, fs_factor* fac,
+                                                           uint32_t fcap, uint32_t n0, uint32_t n1) {
+  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
+  if (i >= fs_live(ctr, c.cap)) return;
+  const uint64_t p = c.p[i], q = c.q[i], r2 = c.r2[i];
+  if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
+  const uint64_t one = fs_mont_mul(1, r2, p, q), nmo = p - one;
+  uint64_t s6 = fs_mont_mul(c.res[i], r2, p, q);
+  uint64_t s7 = fs_mont_mul(n0 < p ? (uint64_t)n0 : (uint64_t)n0 % p, r2, p, q);
+  for (uint32_t n = n0; n < n1; ++n) {
+    s7 = fs_add_mod(s7, one, p);
+    if (s6 == one || s6 == nmo) fs_record(ctr, fac, fcap, p, (s6 == one) ? -(int32_t)n : (int32_t)n);
+    s6 = fs_mont_mul(s6, s7, p, q);
+  }
+  c.res[i] = fs_mont_mul(s6, 1, p, q);
+  c.nnext[i] = n1;
+}
+
+// ---- check: checksum fold + count + validation (check.cl:50-109)
+
+__global__ __launch_bounds__(FS_BLOCK) void k_check(fs_cands c, uint32_t* ctr, uint64_t* sum,
+                                                    uint32_t nmax) {
+  const uint32_t live = fs_live(ctr, c.cap);
+  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
+  uint64_t v = 0;
+  if (i < live) {
+    const uint64_t p = c.p[i], q = c.q[i], r2 = c.r2[i], s = c.res[i];
+    const uint64_t s6 = fs_mont_mul(s, r2, p, q);  // mont(nmax! mod p)
+    const uint64_t nm = (p > nmax) ? (uint64_t)nmax : (uint64_t)nmax % p;
+    const uint64_t s7 = fs_mont_mul(nm, r2, p, q);  // mont(nmax mod p)
+    v = s6 + s7;
+    const bool ok = (s < p) && (c.nnext[i] == nmax) && (fs_mont_mul(s6, 1, p, q) == s) &&
+                    (fs_mont_mul(s7, 1, p, q) == nm);
+    if (!ok) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
+  }
+  v = fs_wave_sum(v);
+  if ((threadIdx.x & 63) == 0 && v != 0) atomicAdd((unsigned long long*)&sum[1], (unsigned long long)v);
+  if (i == 0) {
+    atomicAdd((unsigned long long*)&sum[0], (unsigned long long)live);
+    atomicMax(&ctr[FS_CTR_MAXCOUNT], ctr[FS_CTR_COUNT]);
+  }
+}
+
+// ---- verification of nmin! on the segment's first candidate
+
+struct VerifyPair { uint64_t slow, pow; };
+
+// Block product (Montgomery form) of one value per thread.
+__device__ uint64_t fs_block_prod(uint64_t v, uint64_t p, uint64_t q, uint64_t* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t s = FS_BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = fs_mont_mul(sh[threadIdx.x], sh[threadIdx.x + s], p, q);
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_verifyslow(fs_cands c, const uint32_t* ctr,
+                                                         VerifyPair* ver, uint32_t nmin) {
+  __shared__ uint64_t sh[FS_BLOCK];
+  if (fs_live(ctr, c.cap) == 0) return;
+  const uint64_t p = c.p[0], q = c.q[0], r2 = c.r2[0];
+  uint64_t acc = fs_mont_mul(1, r2, p, q);
+  const uint32_t gs = gridDim.x * FS_BLOCK;
+  for (uint64_t n = 2 + blockIdx.x * FS_BLOCK + threadIdx.x; n <= nmin; n += gs)
+    acc = fs_mont_mul(acc, fs_mont_mul(n, r2, p, q), p, q);
+  const uint64_t b = fs_block_prod(acc, p, q, sh);
+  if (threadIdx.x == 0) ver[blockIdx.x].slow = b;
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_verifypow(fs_cands c, const uint32_t* ctr,
+                                                        const uint64_t* bases, const uint32_t* exps,
+                                                        uint32_t cnt, VerifyPair* ver) {
+  __shared__ uint64_t sh[FS_BLOCK];
+  if (fs_live(ctr, c.cap) == 0) return;
+  const uint64_t p = c.p[0], q = c.q[0], r2 = c.r2[0];
+  uint64_t acc = fs_mont_mul(1, r2, p, q);
+  const uint32_t gs = gridDim.x * FS_BLOCK;
+  for (uint32_t k = blockIdx.x * FS_BLOCK + threadIdx.x; k < cnt; k += gs) {
+    const uint64_t b = fs_mont_mul(bases[k], r2, p, q);
+    uint64_t a = fs_mont_mul(1, r2, p, q), x = b;
+    for (uint32_t e = exps[k]; e; e >>= 1) {  // right-to-left binary power
+      if (e & 1) a = fs_mont_mul(a, x, p, q);
+      x = fs_mont_mul(x, x, p, q);
+    }
+    acc = fs_mont_mul(acc, a, p, q);
+  }
+  const uint64_t b = fs_block_prod(acc, p, q, sh);
+  if (threadIdx.x == 0) ver[blockIdx.x].pow = b;
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_verifyreduce(fs_cands c, const uint32_t* ctr,
+                                                           VerifyPair* ver) {
+  __shared__ uint64_t sh[FS_BLOCK];
+  if (fs_live(ctr, c.cap) == 0) return;
+  const uint64_t p = c.p[0], q = c.q[0], r2 = c.r2[0];
+  const uint64_t one = fs_mont_mul(1, r2, p, q);
+  const uint32_t k = blockIdx.x * FS_BLOCK + threadIdx.x;
+  const bool in = k < FS_VERIFY_BLOCKS;
+  const uint64_t a = fs_block_prod(in ? ver[k].slow : one, p, q, sh);
+  __syncthreads();
+  const uint64_t b = fs_block_prod(in ? ver[k].pow : one, p, q, sh);
+  if (threadIdx.x == 0) {
+    ver[FS_VERIFY_BLOCKS + blockIdx.x].slow = a;
+    ver[FS_VERIFY_BLOCKS + blockIdx.x].pow = b;
+  }
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_verifyresult(fs_cands c, uint32_t* ctr, VerifyPair* ver) {
+  __shared__ uint64_t sh[FS_BLOCK];
+  if (fs_live(ctr, c.cap) == 0) return;
+  const uint64_t p = c.p[0], q = c.q[0], r2 = c.r2[0];
+  const uint64_t one = fs_mont_mul(1, r2, p, q);
+  const bool in = threadIdx.x < FS_VERIFY_L2;
+  const uint64_t a = fs_block_prod(in ? ver[FS_VERIFY_BLOCKS + threadIdx.x].slow : one, p, q, sh);
+  __syncthreads();
+  const uint64_t b = fs_block_prod(in ? ver[FS_VERIFY_BLOCKS + threadIdx.x].pow : one, p, q, sh);
+  if (threadIdx.x == 0) {
+    // slow product == power table product == the setup kernel's residue
+    const uint64_t plain = fs_mont_mul(a, 1, p, q);
+    if (a != b || plain != c.res[0]) atomicOr(&ctr[FS_CTR_POWERTABLE], 1u);
+  }
+}
+
+// ================================================================ C ABI: runtime
+
+static int g_device = -1;
+static std::mutex g_init_mu;
+static int g_inject = 0;
+
+extern "C" int fs_init(int device_index) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) {
+    g_last_error = "no HIP device available";
+    return FS_E_NODEV;
+  }
+  if (device_index < 0 || device_index >= n) {
+    g_last_error = "device index out of range";
+    return FS_E_NODEV;
+  }
+  HIPCHK(hipSetDevice(device_index), "hipSetDevice");
+  g_device = device_index;
+  return FS_OK;
+}
+
+static int ensure_init() {
+  if (g_device >= 0) {
+    HIPCHK(hipSetDevice(g_device), "hipSetDevice");
+    return FS_OK;
+  }
+  return fs_init(0);
+}
+
+extern "C" int fs_device_info(char* name, size_t name_len, uint32_t* cus, uint64_t* total,
+                              uint64_t* freeb) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipDeviceProp_t pr;
+  HIPCHK(hipGetDeviceProperties(&pr, g_device), "hipGetDeviceProperties");
+  if (name && name_len) {
+    snprintf(name, name_len, "%s (%s)", pr.name, pr.gcnArchName);
+  }
+  if (cus) *cus = (uint32_t)pr.multiProcessorCount;
+  size_t f = 0, t = 0;
+  HIPCHK(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+  if (total) *total = t;
+  if (freeb) *freeb = f;
+  return FS_OK;
+}
+
+extern "C" int fs_inject_fault(int idx) {
+  if (idx != 0 && idx != FS_CTR_POWERTABLE && idx != FS_CTR_VALIDATE && idx != FS_CTR_PRIMEOVF &&
+      idx != FS_CTR_FACTOROVF) {
+    g_last_error = "fs_inject_fault: unsupported counter";
+    return FS_E_ARG;
+  }
+  g_inject = idx;
+  return FS_OK;
+}
+
+extern "C" int fs_malloc(void** d, size_t bytes) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipError_t e = hipMalloc(d, bytes ? bytes : 1);
+  if (e != hipSuccess) {
+    set_error("hipMalloc", e);
+    return FS_E_NOMEM;
+  }
+  return FS_OK;
+}
+
+extern "C" int fs_free(void* d) {
+  if (!d) return FS_OK;
+  HIPCHK(hipFree(d), "hipFree");
+  return FS_OK;
+}
+
+extern "C" int fs_memcpy_htod(void* dst, const void* src, size_t bytes, fs_stream s, int blocking) {
+  if (!bytes) return FS_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(s)), "hipMemcpyAsync(HtoD)");
+  if (blocking) HIPCHK(hipStreamSynchronize(S(s)), "hipStreamSynchronize");
+  return FS_OK;
+}
+
+extern "C" int fs_memcpy_dtoh(void* dst, const void* src, size_t bytes, fs_stream s, int blocking) {
+  if (!bytes) return FS_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(s)), "hipMemcpyAsync(DtoH)");
+  if (blocking) HIPCHK(hipStreamSynchronize(S(s)), "hipStreamSynchronize");
+  return FS_OK;
+}
+
+extern "C" int fs_memset(void* d, int v, size_t bytes, fs_stream s) {
+  if (!bytes) return FS_OK;
+  HIPCHK(hipMemsetAsync(d, v, bytes, S(s)), "hipMemsetAsync");
+  return FS_OK;
+}
+
+extern "C" int fs_cands_alloc(fs_cands* c, uint32_t cap) {
+  memset(c, 0, sizeof *c);
+  if (cap == 0) {
+    g_last_error = "fs_cands_alloc: cap must be > 0";
+    return FS_E_ARG;
+  }
+  int rc;
+  if ((rc = fs_malloc((void**)&c->p, (size_t)cap * 8)) || (rc = fs_malloc((void**)&c->q, (size_t)cap * 8)) ||
+      (rc = fs_malloc((void**)&c->r2, (size_t)cap * 8)) ||
+      (rc = fs_malloc((void**)&c->res, (size_t)cap * 8)) ||
+      (rc = fs_malloc((void**)&c->nnext, (size_t)cap * 4))) {
+    fs_cands_free(c);
+    return rc;
+  }
+  c->cap = cap;
+  return FS_OK;
+}
+
+extern "C" int fs_cands_free(fs_cands* c) {
+  fs_free(c->p);
+  fs_free(c->q);
+  fs_free(c->r2);
+  fs_free(c->res);
+  fs_free(c->nnext);
+  memset(c, 0, sizeof *c);
+  return FS_OK;
+}
+
+extern "C" int fs_stream_create(fs_stream* s) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipStream_t h;
+  HIPCHK(hipStreamCreateWithFlags(&h, hipStreamNonBlocking), "hipStreamCreate");
+  *s = (fs_stream)h;
+  return FS_OK;
+}
+extern "C" int fs_stream_destroy(fs_stream s) {
+  if (s) HIPCHK(hipStreamDestroy(S(s)), "hipStreamDestroy");
+  return FS_OK;
+}
+extern "C" int fs_stream_sync(fs_stream s) {
+  HIPCHK(hipStreamSynchronize(S(s)), "hipStreamSynchronize");
+  return FS_OK;
+}
+extern "C" int fs_event_create(fs_event* e) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  hipEvent_t h;
+  HIPCHK(hipEventCreate(&h), "hipEventCreate");
+  *e = (fs_event)h;
+  return FS_OK;
+}
+extern "C" int fs_event_destroy(fs_event e) {
+  if (e) HIPCHK(hipEventDestroy((hipEvent_t)e), "hipEventDestroy");
+  return FS_OK;
+}
+extern "C" int fs_event_record(fs_event e, fs_stream s) {
+  HIPCHK(hipEventRecord((hipEvent_t)e, S(s)), "hipEventRecord");
+  return FS_OK;
+}
+extern "C" int fs_event_query(fs_event e) {
+  hipError_t r = hipEventQuery((hipEvent_t)e);
+  if (r == hipSuccess) return 1;
+  if (r == hipErrorNotReady) return 0;
+  set_error("hipEventQuery", r);
+  return FS_E_HIP;
+}
+extern "C" int fs_event_sync(fs_event e) {
+  HIPCHK(hipEventSynchronize((hipEvent_t)e), "hipEventSynchronize");
+  return FS_OK;
+}
+extern "C" int fs_event_elapsed_ms(fs_event a, fs_event b, float* ms) {
+  HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b), "hipEventElapsedTime");
+  return FS_OK;
+}
+extern "C" int fs_graph_begin(fs_stream s) {
+  HIPCHK(hipStreamBeginCapture(S(s), hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  return FS_OK;
+}
+extern "C" int fs_graph_end(fs_stream s, fs_graph* g) {
+  hipGraph_t graph;
+  HIPCHK(hipStreamEndCapture(S(s), &graph), "hipStreamEndCapture");
+  hipGraphExec_t ex;
+  hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    set_error("hipGraphInstantiate", e);
+    return FS_E_HIP;
+  }
+  *g = (fs_graph)ex;
+  return FS_OK;
+}
+extern "C" int fs_graph_launch(fs_graph g, fs_stream s) {
+  HIPCHK(hipGraphLaunch((hipGraphExec_t)g, S(s)), "hipGraphLaunch");
+  return FS_OK;
+}
+extern "C" int fs_graph_destroy(fs_graph g) {
+  if (g) HIPCHK(hipGraphExecDestroy((hipGraphExec_t)g), "hipGraphExecDestroy");
+  return FS_OK;
+}
+
+// ================================================================ C ABI: kernels
+
+static inline uint32_t grid_for(uint32_t n) { return (n + FS_BLOCK - 1) / FS_BLOCK; }
+// Workgroups for a per-candidate kernel: the live count when the host knows it.
+static inline uint32_t cand_grid(const fs_cands* c) {
+  const uint32_t n = (c->grid_hint && c->grid_hint < c->cap) ? c->grid_hint : c->cap;
+  return n ? grid_for(n) : 1;
+}
+
+#define LAUNCHCHK(what)                              \
+  do {                                               \
+    hipError_t e__ = hipGetLastError();              \
+    if (e__ != hipSuccess) {                         \
+      set_error(what, e__);                          \
+      return FS_E_HIP;                               \
+    }                                                \
+  } while (0)
+
+static int check_cands(const fs_cands* c, const char* who) {
+  if (!c || !c->p || !c->q || !c->r2 || !c->res || !c->nnext || c->cap == 0) {
+    g_last_error = std::string(who) + ": invalid candidate buffers";
+    return FS_E_ARG;
+  }
+  return FS_OK;
+}
+
+extern "C" int fs_clearn(uint32_t* d_ctr, fs_stream s) {
+  if (!d_ctr) return FS_E_ARG;
+  k_clearn<<<1, 64, 0, S(s)>>>(d_ctr);
+  LAUNCHCHK("clearn");
+  return FS_OK;
+}
+
+extern "C" int fs_clearresult(uint32_t* d_ctr, uint64_t* d_sum, fs_stream s) {
+  if (!d_ctr || !d_sum) return FS_E_ARG;
+  k_clearresult<<<1, 64, 0, S(s)>>>(d_ctr, d_sum);
+  LAUNCHCHK("clearresult");
+  return FS_OK;
+}
+
+extern "C" int fs_getsegprimes(uint64_t low, uint64_t high, const fs_cands* c, uint32_t* d_ctr,
+                               fs_stream s) {
+  int rc = check_cands(c, "getsegprimes");
+  if (rc) return rc;
+  if (low > high) {
+    g_last_error = "getsegprimes: low > high";
+    return FS_E_ARG;
+  }
+  if (low == high || low == UINT64_MAX) return FS_OK;
+  const uint64_t base = low | 1;  // first odd number >= low
+  if (base >= high) return FS_OK;
+  const uint64_t threads = (high - base + 127) / 128;
+  const uint64_t blocks = (threads + FS_BLOCK - 1) / FS_BLOCK;
+  if (blocks > 0x7fffffffull) {
+    g_last_error = "getsegprimes: range too large for one launch";
+    return FS_E_ARG;
+  }
+  k_getsegprimes<<<(uint32_t)blocks, FS_BLOCK, 0, S(s)>>>(base, high, *c, d_ctr);
+  LAUNCHCHK("getsegprimes");
+  if (g_inject == FS_CTR_PRIMEOVF) {
+    k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_PRIMEOVF);
+    g_inject = 0;
+  }
+  return FS_OK;
+}
+
+extern "C" int fs_setup(const fs_cands* c, const uint32_t* d_ctr, const void* d_prog,
+                        uint32_t prog_terms, uint32_t start, uint32_t end, uint32_t nmin, int path,
+                        fs_stream s) {
+  int rc = check_cands(c, "setup");
+  if (rc) return rc;
+  if (end > prog_terms) end = prog_terms;
+  if (start > end) start = end;
+  if (!d_prog && end > start) {
+    g_last_error = "setup: null program";
+    return FS_E_ARG;
+  }
+  const uint32_t g = cand_grid(c);
+  switch (path) {
+    case FS_PATH_FP64:
+      k_setup_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (const double*)d_prog, start, end, nmin);
+      break;
+    case FS_PATH_HYBRID:
+      k_setup_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (const uint64_t*)d_prog, start, end, nmin);
+      break;
+    case FS_PATH_MONT:  // d_prog: bases[prog_terms] then (exp | curbit << 32)[prog_terms]
+      k_setup_mont<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (const uint64_t*)d_prog, prog_terms, start,
+                                            end, nmin);
+      break;
+    default:
+      g_last_error = "setup: unknown path";
+      return FS_E_ARG;
+  }
+  LAUNCHCHK("setup");
+  return FS_OK;
+}
+
+extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t fcap,
+                          uint32_t n0, uint32_t n1, int path, fs_stream s) {
+  int rc = check_cands(c, "iterate");
+  if (rc) return rc;
+  if (n0 > n1 || n1 > 0x7fffffffu) {
+    g_last_error = "iterate: need startN <= endN < 2^31";
+    return FS_E_ARG;
+  }
+  const uint32_t g = cand_grid(c);
+  switch (path) {
+    case FS_PATH_FP64:
+      k_iterate_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1);
+      break;
+    case FS_PATH_HYBRID:
+      k_iterate_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1);
+      break;
+    case FS_PATH_MONT:
+      k_iterate_mont<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1);
+      break;
+    default:
+      g_last_error = "iterate: unknown path";
+      return FS_E_ARG;
+  }
+  LAUNCHCHK("iterate");
+  if (g_inject == FS_CTR_FACTOROVF) {
+    k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_FACTOROVF);
+    g_inject = 0;
+  }
+  return FS_OK;
+}
+
+extern "C" int fs_check(const fs_cands* c, uint32_t* d_ctr, uint64_t* d_sum, uint32_t nmax, fs_stream s) {
+  int rc = check_cands(c, "check");
+  if (rc) return rc;
+  k_check<<<cand_grid(c), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_sum, nmax);
+  LAUNCHCHK("check");
+  if (g_inject == FS_CTR_VALIDATE) {
+    k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_VALIDATE);
+    g_inject = 0;
+  }
+  return FS_OK;
+}
+
+extern "C" size_t fs_verify_bytes(void) { return (FS_VERIFY_BLOCKS + 64) * sizeof(VerifyPair); }
+
+extern "C" int fs_verifyslow(const fs_cands* c, const uint32_t* d_ctr, void* d_verify, uint32_t nmin,
+                             fs_stream s) {
+  int rc = check_cands(c, "verifyslow");
+  if (rc) return rc;
+  k_verifyslow<<<FS_VERIFY_BLOCKS, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (VerifyPair*)d_verify, nmin);
+  LAUNCHCHK("verifyslow");
+  return FS_OK;
+}
+
+extern "C" int fs_verifypow(const fs_cands* c, const uint32_t* d_ctr, const uint64_t* d_bases,
+                            const uint32_t* d_exps, uint32_t count, void* d_verify, fs_stream s) {
+  int rc = check_cands(c, "verifypow");
+  if (rc) return rc;
+  k_verifypow<<<FS_VERIFY_BLOCKS, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_bases, d_exps, count,
+                                                       (VerifyPair*)d_verify);
+  LAUNCHCHK("verifypow");
+  return FS_OK;
+}
+
+extern "C" int fs_verifyreduce(const fs_cands* c, const uint32_t* d_ctr, void* d_verify, fs_stream s) {
+  int rc = check_cands(c, "verifyreduce");
+  if (rc) return rc;
+  k_verifyreduce<<<FS_VERIFY_L2, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (VerifyPair*)d_verify);
+  LAUNCHCHK("verifyreduce");
+  return FS_OK;
+}
+
+extern "C" int fs_verifyresult(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_stream s) {
+  int rc = check_cands(c, "verifyresult");
+  if (rc) return rc;
+  k_verifyresult<<<1, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (VerifyPair*)d_verify);
+  LAUNCHCHK("verifyresult");
+  if (g_inject == FS_CTR_POWERTABLE) {
+    k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_POWERTABLE);
+    g_inject = 0;
+  }
+  return FS_OK;
+}

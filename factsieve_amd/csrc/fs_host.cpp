@@ -1,0 +1,932 @@
+// fs_host.cpp -- host side of the FactSieve sieve on the C ABI: setup programs,
+// CPU factor verifier, number parsing, the segment driver (cl_sieve), result
+// drain, factors.txt, checkpoint/resume and the self-test.
+//
+// Reference counterparts: cl_sieve.cpp (driver, power table, drain, checkpoint,
+// self-test), verifyprime.c (CPU verifier), putil.c (number parsing).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <errno.h>
+#include <math.h>
+#include <time.h>
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <string>
+#include <mutex>
+#include <thread>
+#include <vector>
+#include "fs_arith.hpp"
+#include "fs_internal.hpp"
+#include "../../include/factsieve.h"
+
+typedef unsigned __int128 u128;
+
+// ============================================================ small primes
+
+// Primes <= limit (odd-only bit sieve of Eratosthenes).  Replaces the
+// reference's primesieve_generate_primes(2, nmin) call (cl_sieve.cpp:797).
+static std::vector<uint32_t> primes_upto_uncached(uint32_t limit) {
+  std::vector<uint32_t> out;
+  if (limit < 2) return out;
+  out.push_back(2);
+  const uint64_t nodd = ((uint64_t)limit - 1) / 2;  // odd numbers 3..limit -> index (x-3)/2
+  std::vector<uint64_t> comp((nodd + 64) / 64, 0);
+  for (uint64_t i = 0; i < nodd; ++i) {
+    if (comp[i >> 6] >> (i & 63) & 1) continue;
+    const uint64_t x = 2 * i + 3;
+    out.push_back((uint32_t)x);
+    for (uint64_t y = x * x; y <= limit; y += 2 * x) {
+      const uint64_t j = (y - 3) / 2;
+      comp[j >> 6] |= (uint64_t)1 << (j & 63);
+    }
+  }
+  return out;
+}
+
+std::vector<uint32_t> fs_primes_upto(uint32_t limit) {
+  static std::mutex mu;
+  static uint32_t last_limit = 0;
+  static std::vector<uint32_t> last;
+  std::lock_guard<std::mutex> lk(mu);
+  if (last.empty() || last_limit != limit) {
+    last = primes_upto_uncached(limit);
+    last_limit = limit;
+  }
+  return last;
+}
+
+// Legendre exponent of prime q in n! (getPower, cl_sieve.cpp:773-782).
+static uint32_t legendre(uint32_t q, uint32_t n) {
+  uint32_t e = 0;
+  for (uint64_t qk = q; qk <= n; qk *= q) e += (uint32_t)(n / qk);
+  return e;
+}
+
+// ============================================================ setup programs
+
+extern "C" int64_t fs_power_table(uint32_t nmin, uint64_t* bases, uint32_t* exps, uint32_t* curbits) {
+  const std::vector<uint32_t> pr = fs_primes_upto(nmin);
+  int64_t m = 0;
+  size_t i = 0;
+  while (i < pr.size()) {
+    const uint32_t e = legendre(pr[i], nmin);
+    uint64_t b = pr[i];
+    ++i;
+    // entry 0 (the prime 2) stays alone; later runs of equal exponent are
+    // merged while the product stays below 2^64 - 1 (cl_sieve.cpp:818-835)
+    if (m > 0) {
+      while (i < pr.size() && legendre(pr[i], nmin) == e) {
+        const u128 pp = (u128)b * pr[i];
+        if (pp >= (u128)UINT64_MAX) break;
+        b = (uint64_t)pp;
+        ++i;
+      }
+    }
+    if (bases) bases[m] = b;
+    if (exps) exps[m] = e;
+    if (curbits) {
+      uint32_t cb = 0x80000000u;
+      if (e > 1) cb >>= (__builtin_clz(e) + 1);
+      curbits[m] = cb;
+    }
+    ++m;
+  }
+  return m;
+}
+
+// Horner program over the binary digits of the Legendre exponents:
+// nmin! = prod_j C_j^(2^j), C_j = product of primes q whose exponent has bit j
+// set; evaluated as (((C_J)^2 C_{J-1})^2 ...) C_0 with each C_j split into
+// multipliers below `bound`.  Returns the program as multipliers, 0 = square.
+std::vector<uint64_t> fs_horner_program(uint32_t nmin, uint64_t bound) {
+  std::vector<uint64_t> prog;
+  const std::vector<uint32_t> pr = fs_primes_upto(nmin);
+  if (pr.empty()) return prog;
+  std::vector<uint32_t> ex(pr.size());
+  uint32_t emax = 0;
+  for (size_t i = 0; i < pr.size(); ++i) {
+    ex[i] = legendre(pr[i], nmin);
+    emax = std::max(emax, ex[i]);
+  }
+  const int J = 31 - __builtin_clz(emax);
+  for (int j = J; j >= 0; --j) {
+    if (j != J) prog.push_back(0);
+    uint64_t cur = 1;
+    for (size_t i = 0; i < pr.size(); ++i) {
+      if (!((ex[i] >> j) & 1)) continue;
+      if ((u128)cur * pr[i] >= bound) {
+        prog.push_back(cur);
+        cur = pr[i];
+      } else {
+        cur *= pr[i];
+      }
+    }
+    if (cur > 1) prog.push_back(cur);
+  }
+  return prog;
+}
+
+extern "C" int fs_select_path(uint64_t pmax) {
+  if (pmax <= ((uint64_t)1 << 52)) return FS_PATH_FP64;
+  if (pmax <= ((uint64_t)1 << 63)) return FS_PATH_HYBRID;
+  return FS_PATH_MONT;
+}
+
+extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
+  if (path == FS_PATH_MONT) {
+    const int64_t m = fs_power_table(nmin, nullptr, nullptr, nullptr);
+    if (words) {
+      std::vector<uint64_t> b(m);
+      std::vector<uint32_t> e(m), c(m);
+      fs_power_table(nmin, b.data(), e.data(), c.data());
+      for (int64_t i = 0; i < m; ++i) {
+        words[i] = b[i];
+        words[m + i] = (uint64_t)e[i] | ((uint64_t)c[i] << 32);
+      }
+    }
+    return m;
+  }
+  if (path != FS_PATH_FP64 && path != FS_PATH_HYBRID) {
+    fs_set_error_msg("fs_setup_program: unknown path");
+    return FS_E_ARG;
+  }
+  const uint64_t bound = (path == FS_PATH_FP64) ? ((uint64_t)1 << 48) : ((uint64_t)1 << 31);
+  const std::vector<uint64_t> prog = fs_horner_program(nmin, bound);
+  if (words) {
+    for (size_t i = 0; i < prog.size(); ++i) {
+      if (path == FS_PATH_FP64) {
+        const double d = (double)prog[i];  // exact: < 2^48
+        memcpy(&words[i], &d, 8);
+      } else {
+        words[i] = prog[i];
+      }
+    }
+  }
+  return (int64_t)prog.size();
+}
+
+// ============================================================ CPU verifier
+
+// Deterministic primality for odd 64-bit p: strong probable-prime tests to the
+// seven bases {2, 325, 9375, 28178, 450775, 9780504, 1795265022} (Jim Sinclair's
+// set, as used by isPrime, verifyprime.c:96-132), bases reduced mod p.
+extern "C" int fs_is_prime(uint64_t p) {
+  static const uint32_t bases[7] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+  if (p < 2) return 0;
+  if (p < 4) return 1;
+  if ((p & 1) == 0) return 0;
+  const uint64_t q = fs_invert(p);
+  const uint64_t one = (0 - p) % p;
+  const uint64_t r2 = fs_r2_from_one(one, p, q);
+  const uint64_t nmo = p - one;
+  uint64_t d = p - 1;
+  const int t = __builtin_ctzll(d);
+  d >>= t;
+  for (int i = 0; i < 7; ++i) {
+    uint64_t b = bases[i];
+    if (b >= p) {
+      b %= p;
+      if (b == 0) continue;
+    }
+    const uint64_t bm = fs_mont_mul(b, r2, p, q);
+    uint64_t a = one, x = bm;
+    for (uint64_t e = d; e; e >>= 1) {
+      if (e & 1) a = fs_mont_mul(a, x, p, q);
+      x = fs_mont_mul(x, x, p, q);
+    }
+    if (a == one || a == nmo) continue;
+    bool ok = false;
+    for (int s = 1; s < t && !ok; ++s) {
+      a = fs_mont_mul(a, a, p, q);
+      ok = (a == nmo);
+    }
+    if (!ok) return 0;
+  }
+  return 1;
+}
+
+// O(n) recomputation of n! mod p; true iff n! = +1 (c = -1) or -1 (c = +1)
+// (verify, verifyprime.c:137-164).
+extern "C" int fs_verify_factor(uint64_t p, uint32_t n, int32_t c) {
+  if (p < 3 || (c != 1 && c != -1)) return 0;
+  uint64_t r;
+  if (n >= p) {
+    r = 0;  // p | n!
+  } else if (p < 0xFFFFFFFFull) {
+    r = 1;
+    for (uint64_t i = 2; i <= n; ++i) r = r * i % p;
+  } else {
+    const uint64_t q = fs_invert(p), one = (0 - p) % p;
+    uint64_t rm = one, im = one;  // Montgomery forms of r = 1 and i = 1
+    for (uint32_t i = 2; i <= n; ++i) {
+      im = fs_add_mod(im, one, p);
+      rm = fs_mont_mul(rm, im, p, q);
+    }
+    r = fs_mont_mul(rm, 1, p, q);
+  }
+  if (c == -1) return r == 1;
+  return r == p - 1;
+}
+
+// Number parsing of the -p/-P/-n/-N/-v options with the suffixes of
+// parse_uint64 (putil.c:33-115): K/M/G/T/P = x1000^k, e<k> = x10^k,
+// k/m/g/t/p = x1024^k, b<k> = x2^k.  0 ok, -1 cannot parse, -2 out of range.
+extern "C" int fs_parse_uint64(uint64_t* result, const char* str, uint64_t lo, uint64_t hi) {
+  char* tail = nullptr;
+  errno = 0;
+  uint64_t v = strtoull(str, &tail, 0);
+  if (errno != 0 || v > hi) return -2;
+  const char suf = *tail;
+  auto scale = [&](uint64_t f, uint32_t times) -> int {
+    for (uint32_t k = 0; k < times; ++k) {
+      if (v > hi / f) return -2;
+      v *= f;
+    }
+    return 0;
+  };
+  switch (suf) {
+    case '\0':
+      break;
+    case 'K': case 'M': case 'G': case 'T': case 'P': {
+      if (tail[1] != '\0') return -1;
+      const char* order = "KMGTP";
+      const uint32_t k = (uint32_t)(strchr(order, suf) - order) + 1;
+      if (scale(1000, k)) return -2;
+      break;
+    }
+    case 'k': case 'm': case 'g': case 't': case 'p': {
+      if (tail[1] != '\0') return -1;
+      const char* order = "kmgtp";
+      const uint32_t sh = 10 * ((uint32_t)(strchr(order, suf) - order) + 1);
+      if (v > (hi >> sh)) return -2;
+      v <<= sh;
+      break;
+    }
+    case 'e': case 'E': case 'b': case 'B': {
+      char* t2 = nullptr;
+      const unsigned long k = strtoul(tail + 1, &t2, 0);
+      if (errno != 0) return -2;
+      if (*t2 != '\0') return -1;
+      if (suf == 'e' || suf == 'E') {
+        if (scale(10, (uint32_t)k)) return -2;
+      } else {
+        for (unsigned long j = 0; j < k; ++j) {
+          if (v > (hi >> 1)) return -2;
+          v <<= 1;
+        }
+      }
+      break;
+    }
+    default:
+      return -1;
+  }
+  if (v < lo) return -2;
+  *result = v;
+  return 0;
+}
+
+// ============================================================ driver
+
+extern "C" void fs_default_options(fs_options* o) {
+  memset(o, 0, sizeof *o);
+  o->path = FS_PATH_AUTO;
+  o->verify_threads = 2;
+  o->write_files = 0;
+  o->resume = 0;
+  o->quiet = 1;
+  o->checkpoint_secs = 60;
+}
+
+// Checkpoint record, byte-identical to workStatus (cl_sieve.h:9-12): 8 x u64 then
+// 2 x u32, 72 bytes, state_sum = wrapping sum of the other fields.
+struct WorkStatus {
+  uint64_t pmin, pmax, p, checksum, primecount, factorcount, last_trickle, state_sum;
+  uint32_t nmin, nmax;
+};
+static_assert(sizeof(WorkStatus) == 72, "workStatus layout");
+
+static uint64_t state_sum(const WorkStatus& s) {
+  return s.pmin + s.pmax + s.p + s.checksum + s.primecount + s.factorcount + s.last_trickle + s.nmin +
+         s.nmax;
+}
+
+static std::string in_dir(const char* dir, const char* name) {
+  if (!dir || !*dir) return name;
+  std::string s = dir;
+  if (s.back() != '/') s += '/';
+  return s + name;
+}
+
+// write_state (cl_sieve.cpp:100-129): alternate stateA.ckp / stateB.ckp.
+static void write_state(WorkStatus& st, bool& write_a_next, const char* dir) {
+  st.state_sum = state_sum(st);
+  const std::string fn = in_dir(dir, write_a_next ? "stateA.ckp" : "stateB.ckp");
+  FILE* f = fopen(fn.c_str(), "wb");
+  if (!f) {
+    fprintf(stderr, "Cannot open %s !!!\n", fn.c_str());
+    return;
+  }
+  if (fwrite(&st, sizeof st, 1, f) != 1) {
+    fprintf(stderr, "Cannot write checkpoint to file. Continuing...\n");
+    fclose(f);
+    return;
+  }
+  if (fclose(f) == 0) write_a_next = !write_a_next;
+}
+
+// read_state (cl_sieve.cpp:132-222): a file is good when it parses, matches the
+// ranges and its state_sum checks; with two good files the larger p wins.
+static int read_state(WorkStatus& st, bool& write_a_next, const char* dir, bool verbose) {
+  WorkStatus s[2];
+  bool good[2] = {false, false};
+  const char* names[2] = {"stateA.ckp", "stateB.ckp"};
+  for (int k = 0; k < 2; ++k) {
+    const std::string fn = in_dir(dir, names[k]);
+    FILE* f = fopen(fn.c_str(), "rb");
+    if (!f) continue;
+    if (fread(&s[k], sizeof(WorkStatus), 1, f) != 1) {
+      fprintf(stderr, "Cannot parse %s !!!\n", names[k]);
+      if (verbose) printf("Cannot parse %s !!!\n", names[k]);
+    } else if (s[k].pmin != st.pmin || s[k].pmax != st.pmax || s[k].nmin != st.nmin ||
+               s[k].nmax != st.nmax) {
+      fprintf(stderr, "Invalid checkpoint file %s !!!\n", names[k]);
+      if (verbose) printf("Invalid checkpoint file %s !!!\n", names[k]);
+    } else if (state_sum(s[k]) != s[k].state_sum) {
+      fprintf(stderr, "Checksum error in %s !!!\n", names[k]);
+      if (verbose) printf("Checksum error in %s !!!\n", names[k]);
+    } else {
+      good[k] = true;
+    }
+    fclose(f);
+  }
+  if (good[0] && good[1]) {
+    if (s[0].p > s[1].p) good[1] = false;
+    else good[0] = false;
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (good[k]) {
+      st = s[k];
+      write_a_next = (k == 1);
+      if (verbose) printf("Resuming from checkpoint in %s\n", names[k]);
+      return 1;
+    }
+  }
+  return 0;
+}
+
+// Sort key of the drained factor list: p, then |n| (factorcompare, cl_sieve.cpp:422-436).
+static bool factor_less(const fs_factor& a, const fs_factor& b) {
+  if (a.p != b.p) return a.p < b.p;
+  const int32_t na = a.nc < 0 ? -a.nc : a.nc, nb = b.nc < 0 ? -b.nc : b.nc;
+  return na < nb;
+}
+
+void fs_sort_factors(fs_factor* f, size_t n) { std::stable_sort(f, f + n, factor_less); }
+
+// Verify every factor on `threads` CPU threads; returns the index of the first
+// failure or -1 (the OpenMP loop of cl_sieve.cpp:497-517).
+int64_t fs_verify_all(const fs_factor* f, size_t n, int threads) {
+  if (n == 0) return -1;
+  if (threads < 1) threads = 1;
+  if ((size_t)threads > n) threads = (int)n;
+  std::atomic<int64_t> bad(-1);
+  std::atomic<size_t> next(0);
+  auto work = [&]() {
+    for (;;) {
+      const size_t i = next.fetch_add(1);
+      if (i >= n) break;
+      const uint32_t fn = (uint32_t)(f[i].nc < 0 ? -f[i].nc : f[i].nc);
+      const int32_t fc = f[i].nc < 0 ? -1 : 1;
+      if (!fs_verify_factor(f[i].p, fn, fc)) {
+        int64_t cur = bad.load();
+        while ((cur < 0 || (int64_t)i < cur) && !bad.compare_exchange_weak(cur, (int64_t)i)) {
+        }
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  return bad.load();
+}
+
+// Filter composites (2-PRP p) and fold kept factors into the checksum
+// (cl_sieve.cpp:531-550).  Compacts kept factors to the front; returns count.
+size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose) {
+  size_t kept = 0;
+  uint64_t lastgood = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t fp = f[i].p;
+    const uint32_t fn = (uint32_t)(f[i].nc < 0 ? -f[i].nc : f[i].nc);
+    const int32_t fc = f[i].nc < 0 ? -1 : 1;
+    if (fp == lastgood || fs_is_prime(fp)) {
+      lastgood = fp;
+      *checksum += (uint32_t)(fn + (uint32_t)fc);
+      f[kept++] = f[i];
+    } else {
+      fprintf(stderr, "discarded 2-PRP factor %llu\n", (unsigned long long)fp);
+      if (verbose) printf("discarded 2-PRP factor %llu\n", (unsigned long long)fp);
+    }
+  }
+  return kept;
+}
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Upper estimate of candidates in [lo, lo + len): the probable primes have
+// density ~1/ln p, bounded here by 1/(ln lo - 1.5) plus slack, and never more
+// than the survivors of the 113-sieve (density 0.1188).
+uint64_t cand_capacity(uint64_t lo, uint64_t len) {
+  const double l = (double)(lo < 3 ? 3 : lo);
+  const double sieve_bound = (double)len * 0.1190 + 4096.0;
+  const double dens = (log(l) > 4.0) ? 1.0 / (log(l) - 1.5) : 1.0;
+  const double est = (double)len * dens * 1.15 + 65536.0;
+  const double c = std::min(sieve_bound, est);
+  return (uint64_t)std::min(c, 4.0e9);
+}
+
+struct DevBufs {
+  fs_cands c{};
+  uint32_t* ctr = nullptr;
+  uint64_t* sum = nullptr;
+  fs_factor* fac = nullptr;
+  uint32_t fcap = 0;
+  void* prog = nullptr;
+  uint32_t prog_terms = 0;
+  uint64_t* pt_bases = nullptr;
+  uint32_t* pt_exps = nullptr;
+  uint32_t pt_count = 0;
+  void* verify = nullptr;
+  fs_stream s = nullptr;
+  fs_event ev[6] = {};
+  ~DevBufs() {
+    fs_cands_free(&c);
+    fs_free(ctr);
+    fs_free(sum);
+    fs_free(fac);
+    fs_free(prog);
+    fs_free(pt_bases);
+    fs_free(pt_exps);
+    fs_free(verify);
+    for (auto& e : ev) fs_event_destroy(e);
+    fs_stream_destroy(s);
+  }
+};
+
+struct Drain {
+  std::vector<fs_factor> kept;  // all kept factors, in drain order (globally sorted)
+};
+
+}  // namespace
+
+// getResults (cl_sieve.cpp:439-554): read counters and sums, check the flags,
+// read + sort + verify + filter the factors, append them to factors.txt.
+static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs_result* res) {
+  uint32_t ctr[FS_CTR_WORDS];
+  uint64_t sum[FS_SUM_WORDS];
+  int rc;
+  if ((rc = fs_memcpy_dtoh(sum, d.sum, sizeof sum, d.s, 0)) ||
+      (rc = fs_memcpy_dtoh(ctr, d.ctr, sizeof ctr, d.s, 1)))
+    return rc;
+  st.primecount += sum[0];
+  st.checksum += sum[1];
+  const bool verbose = !o.quiet;
+  if (ctr[FS_CTR_MAXCOUNT] > d.c.cap || ctr[FS_CTR_PRIMEOVF]) {
+    fs_set_error_msg("error: gpu prime array overflow");
+    return FS_E_OVERFLOW;
+  }
+  if (ctr[FS_CTR_VALIDATE]) {
+    fs_set_error_msg("error: gpu validation failure");
+    return FS_E_VALIDATE;
+  }
+  if (ctr[FS_CTR_POWERTABLE]) {
+    fs_set_error_msg("error: power table verification failed");
+    return FS_E_POWERTABLE;
+  }
+  const uint32_t nf = ctr[FS_CTR_NFACTORS];
+  if (nf > d.fcap || ctr[FS_CTR_FACTOROVF]) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "Error: number of results (%u) overflowed array.", nf);
+    fs_set_error_msg(buf);
+    return FS_E_OVERFLOW;
+  }
+  if (nf > 0) {
+    if (verbose) printf("processing %u factors on CPU\n", nf);
+    std::vector<fs_factor> h(nf);
+    if ((rc = fs_memcpy_dtoh(h.data(), d.fac, (size_t)nf * sizeof(fs_factor), d.s, 1))) return rc;
+    fs_sort_factors(h.data(), nf);
+    res->raw_factors += nf;
+    const int64_t bad = fs_verify_all(h.data(), nf, o.verify_threads);
+    if (bad >= 0) {
+      char buf[256];
+      const fs_factor& f = h[bad];
+      snprintf(buf, sizeof buf, "CPU factor verification failed!  %llu is not a factor of %u!%+d",
+               (unsigned long long)f.p, (unsigned)(f.nc < 0 ? -f.nc : f.nc), f.nc < 0 ? -1 : 1);
+      fs_set_error_msg(buf);
+      fprintf(stderr, "%s\n", buf);
+      return FS_E_VERIFY;
+    }
+    fprintf(stderr, "Verified %u factors.\n", nf);
+    if (verbose) printf("Verified %u factors.\n", nf);
+    uint64_t ck = 0;
+    const size_t kept = fs_filter_factors(h.data(), nf, &ck, verbose);
+    st.checksum += ck;
+    st.factorcount += kept;
+    if (o.write_files) {
+      const std::string fn = in_dir(o.workdir, "factors.txt");
+      FILE* f = fopen(fn.c_str(), "a");
+      if (!f) {
+        fs_set_error_msg("Cannot open factors.txt !!!");
+        return FS_E_IO;
+      }
+      for (size_t i = 0; i < kept; ++i) {
+        const uint32_t fn_ = (uint32_t)(h[i].nc < 0 ? -h[i].nc : h[i].nc);
+        if (fprintf(f, "%llu | %u!%+d\n", (unsigned long long)h[i].p, fn_, h[i].nc < 0 ? -1 : 1) < 0) {
+          fclose(f);
+          fs_set_error_msg("Cannot write to factors.txt !!!");
+          return FS_E_IO;
+        }
+      }
+      fclose(f);
+    }
+    out.kept.insert(out.kept.end(), h.begin(), h.begin() + kept);
+  }
+  return fs_clearresult(d.ctr, d.sum, d.s);
+}
+
+// finalizeResults (cl_sieve.cpp:721-771): line-count check, then the checksum trailer.
+static int finalize_file(const WorkStatus& st, const fs_options& o) {
+  const std::string fn = in_dir(o.workdir, "factors.txt");
+  if (st.factorcount) {
+    FILE* f = fopen(fn.c_str(), "r");
+    if (!f) {
+      fs_set_error_msg("Cannot open factors.txt !!!");
+      return FS_E_IO;
+    }
+    char line[256];
+    uint64_t lc = 0;
+    while (fgets(line, sizeof line, f)) ++lc;
+    fclose(f);
+    if (lc < st.factorcount) {
+      fs_set_error_msg("ERROR: Missing factors in factors.txt !!!");
+      return FS_E_IO;
+    }
+  }
+  FILE* f = fopen(fn.c_str(), "a");
+  if (!f) {
+    fs_set_error_msg("Cannot open factors.txt !!!");
+    return FS_E_IO;
+  }
+  int w;
+  if (st.factorcount) w = fprintf(f, "%016llX\n", (unsigned long long)st.checksum);
+  else w = fprintf(f, "no factors\n%016llX\n", (unsigned long long)st.checksum);
+  fclose(f);
+  if (w < 0) {
+    fs_set_error_msg("Cannot write to factors.txt !!!");
+    return FS_E_IO;
+  }
+  return FS_OK;
+}
+
+static int validate_ranges(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax) {
+  // setupSearch (cl_sieve.cpp:557-589)
+  if (pmin == 0 || pmax == 0) {
+    fs_set_error_msg("-p and -P arguments are required");
+    return FS_E_ARG;
+  }
+  if (nmin == 0 || nmax == 0) {
+    fs_set_error_msg("-n and -N arguments are required");
+    return FS_E_ARG;
+  }
+  if (nmin > nmax) {
+    fs_set_error_msg("nmin <= nmax is required");
+    return FS_E_ARG;
+  }
+  if (pmin > pmax) {
+    fs_set_error_msg("pmin <= pmax is required");
+    return FS_E_ARG;
+  }
+  if (pmin < nmin) {
+    fs_set_error_msg("pmin must be >= nmin, there are no factors when p <= nmin");
+    return FS_E_ARG;
+  }
+  if (nmax > 0x7fffffffu) {
+    fs_set_error_msg("nmax must be < 2^31");
+    return FS_E_ARG;
+  }
+  return FS_OK;
+}
+
+uint32_t fs_auto_segment_candidates() {
+  uint32_t cus = 0;
+  if (fs_device_info(nullptr, 0, &cus, nullptr, nullptr) != FS_OK || cus == 0) cus = 256;
+  // 8 waves of 64 lanes on each of the 4 SIMDs of every CU, times 8 rounds
+  return cus * 4u * 8u * 64u * 8u;
+}
+
+// Time the events of the last segment's kernels into the result counters.
+static void account(DevBufs& d, fs_result* r) {
+  float ms = 0;
+  if (fs_event_elapsed_ms(d.ev[0], d.ev[1], &ms) == FS_OK) r->kernel_ms_primegen += ms;
+  if (fs_event_elapsed_ms(d.ev[2], d.ev[3], &ms) == FS_OK) r->kernel_ms_setup += ms;
+  if (fs_event_elapsed_ms(d.ev[4], d.ev[5], &ms) == FS_OK) r->kernel_ms_iterate += ms;
+}
+
+extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax, const fs_options* opt,
+                        fs_result* res, fs_factor* factors, uint64_t cap, uint64_t* nfactors) {
+  fs_options o;
+  if (opt) o = *opt;
+  else fs_default_options(&o);
+  fs_result r;
+  memset(&r, 0, sizeof r);
+  if (nfactors) *nfactors = 0;
+  int rc = validate_ranges(pmin, pmax, nmin, nmax);
+  if (rc) return rc;
+  const bool verbose = !o.quiet;
+  const int path = (o.path == FS_PATH_AUTO) ? fs_select_path(pmax) : o.path;
+  if (path != FS_PATH_FP64 && path != FS_PATH_HYBRID && path != FS_PATH_MONT) {
+    fs_set_error_msg("unknown arithmetic path");
+    return FS_E_ARG;
+  }
+  if ((path == FS_PATH_FP64 && pmax > ((uint64_t)1 << 52)) ||
+      (path == FS_PATH_HYBRID && pmax > ((uint64_t)1 << 63))) {
+    fs_set_error_msg("forced arithmetic path does not cover pmax");
+    return FS_E_ARG;
+  }
+  r.path = path;
+
+  WorkStatus st;
+  memset(&st, 0, sizeof st);
+  st.pmin = pmin;
+  st.pmax = pmax;
+  st.nmin = nmin;
+  st.nmax = nmax;
+  st.p = pmin;
+  bool write_a_next = true;
+  const double t0 = now_s();
+  fprintf(stderr, "Starting sieve at p: %llu n: %u\nStopping sieve at P: %llu N: %u\n",
+          (unsigned long long)pmin, nmin, (unsigned long long)pmax, nmax);
+  if (verbose)
+    printf("Starting sieve at p: %llu n: %u\nStopping sieve at P: %llu N: %u\n", (unsigned long long)pmin,
+           nmin, (unsigned long long)pmax, nmax);
+
+  if (o.write_files) {
+    bool resumed = false;
+    if (!o.test_mode && o.resume && read_state(st, write_a_next, o.workdir, verbose)) {
+      resumed = true;
+      r.resumed = 1;
+      fprintf(stderr, "Resuming from checkpoint, current p: %llu\n", (unsigned long long)st.p);
+      if (verbose) printf("Current p: %llu\n", (unsigned long long)st.p);
+      if (st.p == st.pmax) {
+        if (verbose) printf("Workunit complete.\n");
+        fprintf(stderr, "Workunit complete.\n");
+        r.factorcount = st.factorcount;
+        r.primecount = st.primecount;
+        r.checksum = st.checksum;
+        if (res) *res = r;
+        return FS_OK;
+      }
+    }
+    if (!resumed) {
+      FILE* f = fopen(in_dir(o.workdir, "factors.txt").c_str(), "w");
+      if (!f) {
+        fs_set_error_msg("Cannot open factors.txt !!!");
+        return FS_E_IO;
+      }
+      fclose(f);
+      st.last_trickle = (uint64_t)time(nullptr);
+    }
+  }
+
+  DevBufs d;
+  Drain dr;
+  if ((rc = fs_stream_create(&d.s))) return rc;
+  for (auto& e : d.ev)
+    if ((rc = fs_event_create(&e))) return rc;
+  // factor buffer: 1e6 records, 3e7 below 2^32 (main.cpp:193, cl_sieve.cpp:593-595)
+  d.fcap = (pmin < 0xFFFFFFFFull) ? 30000000u : 1000000u;
+  if ((rc = fs_malloc((void**)&d.ctr, FS_CTR_WORDS * 4)) || (rc = fs_malloc((void**)&d.sum, FS_SUM_WORDS * 8)) ||
+      (rc = fs_malloc((void**)&d.fac, (size_t)d.fcap * sizeof(fs_factor))) ||
+      (rc = fs_malloc(&d.verify, fs_verify_bytes())))
+    return rc;
+  if ((rc = fs_memset(d.ctr, 0, FS_CTR_WORDS * 4, d.s)) || (rc = fs_memset(d.sum, 0, FS_SUM_WORDS * 8, d.s)))
+    return rc;
+
+  // setup program and the reference power table (verifypow, and setup on MONT)
+  {
+    const int64_t T = fs_setup_program(nmin, path, nullptr);
+    if (T < 0) return (int)T;
+    std::vector<uint64_t> w((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1);
+    fs_setup_program(nmin, path, w.data());
+    d.prog_terms = (uint32_t)T;
+    if ((rc = fs_malloc(&d.prog, w.size() * 8)) || (rc = fs_memcpy_htod(d.prog, w.data(), w.size() * 8, d.s, 1)))
+      return rc;
+    const int64_t m = fs_power_table(nmin, nullptr, nullptr, nullptr);
+    std::vector<uint64_t> b((size_t)m + 1);
+    std::vector<uint32_t> e((size_t)m + 1), c((size_t)m + 1);
+    fs_power_table(nmin, b.data(), e.data(), c.data());
+    d.pt_count = (uint32_t)m;
+    if ((rc = fs_malloc((void**)&d.pt_bases, b.size() * 8)) || (rc = fs_malloc((void**)&d.pt_exps, e.size() * 4)) ||
+        (rc = fs_memcpy_htod(d.pt_bases, b.data(), b.size() * 8, d.s, 0)) ||
+        (rc = fs_memcpy_htod(d.pt_exps, e.data(), e.size() * 4, d.s, 1)))
+      return rc;
+    fprintf(stderr, "Compressed %zu power table terms to %u\n", fs_primes_upto(nmin).size(), d.pt_count);
+    if (verbose) printf("Compressed %zu power table terms to %u\n", fs_primes_upto(nmin).size(), d.pt_count);
+  }
+
+  const uint32_t target = o.seg_candidates ? o.seg_candidates : fs_auto_segment_candidates();
+  const uint32_t nchunk_max = o.n_chunk ? o.n_chunk : (1u << 22);
+  bool verified = false;
+  double last_ckpt = now_s(), last_prog = now_s();
+  const double irsize = 1.0 / (double)(pmax - pmin ? pmax - pmin : 1);
+  uint32_t prog_chunk = 1u << 16;
+
+  while (st.p < st.pmax) {
+    // segment [st.p, stop)
+    const double lnp = log((double)(st.p < 3 ? 3 : st.p));
+    double len_d = (double)target * std::max(lnp - 1.0, 1.0);
+    len_d = std::max(len_d, 65536.0);
+    uint64_t len = (uint64_t)std::min(len_d, 4.0e12);
+    uint64_t stop = st.p + len;
+    if (stop > st.pmax || stop < st.p) stop = st.pmax;
+    len = stop - st.p;
+    const uint64_t need = cand_capacity(st.p, len);
+    if (need > d.c.cap) {
+      fs_cands_free(&d.c);
+      if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>(need + need / 4, 0xFFFFFF00ull)))) return rc;
+    }
+    // candidates of the segment; exact count read back to size the grids
+    uint32_t count = 0;
+    for (int attempt = 0;; ++attempt) {
+      d.c.grid_hint = 0;
+      if ((rc = fs_clearn(d.ctr, d.s)) || (rc = fs_event_record(d.ev[0], d.s)) ||
+          (rc = fs_getsegprimes(st.p, stop, &d.c, d.ctr, d.s)) || (rc = fs_event_record(d.ev[1], d.s)))
+        return rc;
+      uint32_t h[FS_CTR_WORDS];
+      if ((rc = fs_memcpy_dtoh(h, d.ctr, sizeof h, d.s, 1))) return rc;
+      count = h[FS_CTR_COUNT];
+      if (count <= d.c.cap && !h[FS_CTR_PRIMEOVF]) break;
+      if (attempt > 2) {
+        fs_set_error_msg("error: gpu prime array overflow");
+        return FS_E_OVERFLOW;
+      }
+      // grow and redo the segment (nothing of it was folded yet)
+      const uint64_t ncap = std::min<uint64_t>((uint64_t)count + count / 2 + 65536, 0xFFFFFF00ull);
+      fs_cands_free(&d.c);
+      if ((rc = fs_cands_alloc(&d.c, (uint32_t)ncap))) return rc;
+      uint32_t zero = 0;
+      if ((rc = fs_memcpy_htod(d.ctr + FS_CTR_PRIMEOVF, &zero, 4, d.s, 1))) return rc;
+    }
+    d.c.grid_hint = count ? count : 1;
+
+    // setup: nmin! mod p, in chunks of the program
+    if ((rc = fs_event_record(d.ev[2], d.s))) return rc;
+    if (d.prog_terms == 0) {
+      if ((rc = fs_setup(&d.c, d.ctr, d.prog, 0, 0, 0, nmin, path, d.s))) return rc;
+    }
+    for (uint32_t s0 = 0; s0 < d.prog_terms; s0 += prog_chunk) {
+      const uint32_t s1 = std::min(d.prog_terms, s0 + prog_chunk);
+      if ((rc = fs_setup(&d.c, d.ctr, d.prog, d.prog_terms, s0, s1, nmin, path, d.s))) return rc;
+    }
+    if ((rc = fs_event_record(d.ev[3], d.s))) return rc;
+
+    // power-table / setup cross-check on the first candidate of the first
+    // non-empty segment (verify kernels, cl_sieve.cpp:868-907)
+    if (!verified && count > 0) {
+      if ((rc = fs_verifyslow(&d.c, d.ctr, d.verify, nmin, d.s)) ||
+          (rc = fs_verifypow(&d.c, d.ctr, d.pt_bases, d.pt_exps, d.pt_count, d.verify, d.s)) ||
+          (rc = fs_verifyreduce(&d.c, d.ctr, d.verify, d.s)) || (rc = fs_verifyresult(&d.c, d.ctr, d.verify, d.s)))
+        return rc;
+      uint32_t h[FS_CTR_WORDS];
+      if ((rc = fs_memcpy_dtoh(h, d.ctr, sizeof h, d.s, 1))) return rc;
+      if (h[FS_CTR_POWERTABLE]) {
+        fs_set_error_msg("error: power table verification failed");
+        return FS_E_POWERTABLE;
+      }
+      verified = true;
+      fprintf(stderr, "Verified power table (%llu bytes) starting sieve...\n",
+              (unsigned long long)d.pt_count * 16);
+      if (verbose) printf("Verified power table (%llu bytes) starting sieve...\n", (unsigned long long)d.pt_count * 16);
+    }
+
+    // iterate over n in chunks, then fold the checksum
+    if ((rc = fs_event_record(d.ev[4], d.s))) return rc;
+    for (uint32_t n0 = nmin; n0 < nmax;) {
+      const uint32_t n1 = (nmax - n0 > nchunk_max) ? n0 + nchunk_max : nmax;
+      if ((rc = fs_iterate(&d.c, d.ctr, d.fac, d.fcap, n0, n1, path, d.s))) return rc;
+      ++r.iterate_launches;
+      n0 = n1;
+    }
+    if ((rc = fs_event_record(d.ev[5], d.s))) return rc;
+    if ((rc = fs_check(&d.c, d.ctr, d.sum, nmax, d.s))) return rc;
+    r.steps += (uint64_t)count * (nmax - nmin);
+    r.setup_mulmods += (uint64_t)count * d.prog_terms;
+    ++r.segments;
+
+    const uint64_t nextp = st.p + len;
+    const bool last = (nextp >= st.pmax || nextp < st.p);
+    st.p = last ? st.pmax : nextp;
+    if ((rc = fs_stream_sync(d.s))) return rc;
+    account(d, &r);
+
+    const double tn = now_s();
+    if (verbose && tn - last_prog > 2.0) {
+      printf("Sieve Progress: %.1f%%\n", (double)(st.p - pmin) * irsize * 100.0);
+      last_prog = tn;
+    }
+    const bool stop_now = o.stop_after_segments && r.segments >= o.stop_after_segments && !last;
+    if (!last && (tn - last_ckpt > (double)o.checkpoint_secs || stop_now)) {
+      if ((rc = drain(d, st, o, dr, &r))) return rc;
+      if (o.write_files && !o.test_mode) {
+        write_state(st, write_a_next, o.workdir);
+        if (verbose) printf("Checkpoint, current p: %llu\n", (unsigned long long)st.p);
+      }
+      last_ckpt = tn;
+    }
+    if (stop_now) {
+      r.factorcount = st.factorcount;
+      r.primecount = st.primecount;
+      r.checksum = st.checksum;
+      r.seconds = now_s() - t0;
+      if (res) *res = r;
+      return FS_OK;
+    }
+  }
+
+  st.p = st.pmax;
+  if (verbose) printf("Sieve Progress: %.1f%%\n", 100.0);
+  if ((rc = drain(d, st, o, dr, &r))) return rc;
+  if (o.write_files) {
+    if (!o.test_mode) write_state(st, write_a_next, o.workdir);
+    if ((rc = finalize_file(st, o))) return rc;
+  }
+  r.seconds = now_s() - t0;
+  r.factorcount = st.factorcount;
+  r.primecount = st.primecount;
+  r.checksum = st.checksum;
+  fprintf(stderr, "Sieve complete.\nfactors %llu, prime count %llu\n", (unsigned long long)st.factorcount,
+          (unsigned long long)st.primecount);
+  if (verbose) {
+    printf("Sieve finished in %d sec.\n", (int)r.seconds);
+    printf("factors %llu, prime count %llu, checksum %016llX\n", (unsigned long long)st.factorcount,
+           (unsigned long long)st.primecount, (unsigned long long)st.checksum);
+  }
+  if (factors && cap) memcpy(factors, dr.kept.data(), std::min<uint64_t>(cap, dr.kept.size()) * sizeof(fs_factor));
+  if (nfactors) *nfactors = dr.kept.size();
+  if (res) *res = r;
+  return FS_OK;
+}
+
+// run_test (cl_sieve.cpp:1273-1391): the five reference ranges and their
+// expected factor count, candidate count and checksum.
+extern "C" int fs_self_test(const fs_options* opt, int* passed_mask) {
+  struct Case {
+    uint64_t pmin, pmax;
+    uint32_t nmin, nmax;
+    uint64_t factors, primes, checksum;
+  };
+  static const Case cases[5] = {
+      {100000000ull, 101000000ull, 1000000u, 2000000u, 1071, 54211, 0x000004F4F744CA97ull},
+      {1000000000000ull, 1000010000000ull, 10000u, 2000000u, 3, 361727, 0x05052A2D65F3D735ull},
+      {127ull, 100000ull, 127u, 1000000u, 42770, 9566, 0x0000000065C074F0ull},
+      {2000000000000ull, 2000010000000ull, 2000000u, 6000000u, 3, 352866, 0x09C9742A908451EBull},
+      {1000000000000ull, 1000001000000ull, 100000000u, 110000000u, 3, 36249, 0x0081056DC98DA092ull},
+  };
+  fs_options o;
+  if (opt) o = *opt;
+  else fs_default_options(&o);
+  o.test_mode = 1;
+  o.resume = 0;
+  const bool verbose = !o.quiet;
+  if (verbose) printf("Beginning self test of 5 ranges.\n");
+  const double t0 = now_s();
+  int good = 0, mask = 0;
+  for (int k = 0; k < 5; ++k) {
+    fs_result r;
+    const int rc = fs_sieve(cases[k].pmin, cases[k].pmax, cases[k].nmin, cases[k].nmax, &o, &r, nullptr, 0, nullptr);
+    const bool ok = rc == FS_OK && r.factorcount == cases[k].factors && r.primecount == cases[k].primes &&
+                    r.checksum == cases[k].checksum;
+    if (ok) {
+      ++good;
+      mask |= 1 << k;
+    }
+    fprintf(stderr, "test case %d %s.\n", k + 1, ok ? "passed" : "failed");
+    if (verbose) printf("test case %d %s.\n\n", k + 1, ok ? "passed" : "failed");
+  }
+  if (verbose) {
+    printf(good == 5 ? "All test cases completed successfully!\n" : "Self test FAILED!\n");
+    printf("Elapsed time: %d sec.\n", (int)(now_s() - t0));
+  }
+  fprintf(stderr, good == 5 ? "All test cases completed successfully!\n" : "Self test FAILED!\n");
+  if (passed_mask) *passed_mask = mask;
+  return good;
+}

@@ -1,0 +1,163 @@
+// fs_hostemu.cpp -- the per-candidate loops of the sieve kernels executed on the
+// host with the same inline arithmetic (fs_arith.hpp), one candidate at a time.
+//
+// These entry points exist so that the CPU test suite can check each arithmetic
+// formulation (FP64 / hybrid / Montgomery) against the oracle at sizes and
+// moduli where a GPU run would be needed otherwise.  The sieve driver never
+// calls them: fs_sieve, fs_batch and the server run only the gfx950 kernels.
+#include <stdint.h>
+#include <string.h>
+#include <vector>
+#include "fs_arith.hpp"
+#include "fs_internal.hpp"
+#include "../../include/factsieve.h"
+
+extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* res) {
+  if ((p & 1) == 0 || p < 3) return FS_E_ARG;
+  const uint64_t q = fs_invert(p), one = (0 - p) % p, r2 = fs_r2_from_one(one, p, q);
+  const int64_t T = fs_setup_program(nmin, path, nullptr);
+  if (T < 0) return (int)T;
+  std::vector<uint64_t> w((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1);
+  fs_setup_program(nmin, path, w.data());
+  if (path == FS_PATH_FP64) {
+    if (p >= ((uint64_t)1 << 52)) return FS_E_ARG;
+    const double pd = (double)p, pinv = 1.0 / pd;
+    double s = 1.0;
+    for (int64_t k = 0; k < T; ++k) {
+      double m;
+      memcpy(&m, &w[k], 8);
+      if (m == 0.0) {
+        const uint64_t a = fs_fp_to_canon(s, p);
+        s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+      } else {
+        s = fs_fp_mulmod(s, m, pd, pinv);
+      }
+    }
+    *res = fs_fp_to_canon(s, p);
+  } else if (path == FS_PATH_HYBRID) {
+    if (p >= ((uint64_t)1 << 63)) return FS_E_ARG;
+    const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+    uint64_t s = 1;
+    for (int64_t k = 0; k < T; ++k) {
+      const uint32_t m = (uint32_t)w[k];
+      if (m == 0) {
+        const uint64_t a = s >= p ? s - p : s;
+        s = fs_mul_mod(a, a, p, q, r2);
+      } else {
+        s = fs_hy_mulmod(s, m, (double)m, p, pinv, pinv32);
+      }
+    }
+    *res = s >= p ? s - p : s;
+  } else if (path == FS_PATH_MONT) {
+    uint64_t s6 = one;
+    for (int64_t k = 0; k < T; ++k) {
+      const uint32_t e = (uint32_t)w[T + k];
+      uint32_t cb = (uint32_t)(w[T + k] >> 32);
+      if (k == 0) {
+        uint64_t a = fs_add_mod(one, one, p);
+        while (cb) {
+          a = fs_mont_mul(a, a, p, q);
+          if (e & cb) a = fs_add_mod(a, a, p);
+          cb >>= 1;
+        }
+        s6 = a;
+        continue;
+      }
+      const uint64_t b = fs_mont_mul(w[k], r2, p, q);
+      uint64_t a = b;
+      if (e != 1) {
+        while (cb) {
+          a = fs_mont_mul(a, a, p, q);
+          if (e & cb) a = fs_mont_mul(a, b, p, q);
+          cb >>= 1;
+        }
+      }
+      s6 = fs_mont_mul(s6, a, p, q);
+    }
+    *res = fs_mont_mul(s6, 1, p, q);
+  } else {
+    return FS_E_ARG;
+  }
+  return FS_OK;
+}
+
+extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1, uint64_t* res,
+                                  fs_factor* fac, uint32_t cap, uint32_t* nf) {
+  if ((p & 1) == 0 || p < 3 || n0 > n1 || *res >= p) return FS_E_ARG;
+  const uint64_t q = fs_invert(p), one = (0 - p) % p, r2 = fs_r2_from_one(one, p, q);
+  uint32_t cnt = 0;
+  auto rec = [&](int32_t nc) {
+    if (cnt < cap) {
+      fac[cnt].p = p;
+      fac[cnt].nc = nc;
+      fac[cnt].pad = 0;
+    }
+    ++cnt;
+  };
+  if (path == FS_PATH_FP64) {
+    if (p >= ((uint64_t)1 << 52)) return FS_E_ARG;
+    const double pd = (double)p, pinv = 1.0 / pd;
+    double s = fs_canon_to_fp(*res, p);
+    for (uint32_t n = n0; n < n1; ++n) {
+      if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
+      s = fs_fp_mulmod(s, (double)(n + 1), pd, pinv);
+    }
+    *res = fs_fp_to_canon(s, p);
+  } else if (path == FS_PATH_HYBRID) {
+    if (p >= ((uint64_t)1 << 63)) return FS_E_ARG;
+    const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+    const uint64_t pp1 = p + 1, pm1 = p - 1, p2m1 = 2 * p - 1;
+    uint64_t s = *res;
+    for (uint32_t n = n0; n < n1; ++n) {
+      const bool plus1 = (s == 1) | (s == pp1);
+      const bool minus1 = (s == pm1) | (s == p2m1);
+      if (plus1 | minus1) rec(plus1 ? -(int32_t)n : (int32_t)n);
+      const uint32_t m = n + 1;
+      s = fs_hy_mulmod(s, m, (double)m, p, pinv, pinv32);
+    }
+    *res = s >= p ? s - p : s;
+  } else if (path == FS_PATH_MONT) {
+    const uint64_t nmo = p - one;
+    uint64_t s6 = fs_mont_mul(*res, r2, p, q);
+    uint64_t s7 = fs_mont_mul(n0 < p ? (uint64_t)n0 : (uint64_t)n0 % p, r2, p, q);
+    for (uint32_t n = n0; n < n1; ++n) {
+      s7 = fs_add_mod(s7, one, p);
+      if (s6 == one || s6 == nmo) rec((s6 == one) ? -(int32_t)n : (int32_t)n);
+      s6 = fs_mont_mul(s6, s7, p, q);
+    }
+    *res = fs_mont_mul(s6, 1, p, q);
+  } else {
+    return FS_E_ARG;
+  }
+  *nf = cnt;
+  return FS_OK;
+}
+
+// The candidate set of [low, high) as the getsegprimes kernel forms it: the
+// 64-odd-number masks of the 29 small primes, then the base-2 SPRP test.
+extern "C" int64_t fs_hostemu_candidates(uint64_t low, uint64_t high, uint64_t* out, uint64_t cap) {
+  if (low >= high) return 0;
+  const uint64_t base = low | 1;
+  if (base >= high) return 0;
+  int64_t n = 0;
+  for (uint64_t B = base; B < high; B += 128) {
+    uint64_t comp = 0;
+    for (int j = 0; j < FS_NSMALL; ++j) comp |= fs_small_mask(kSmallQ[j], (uint32_t)(B % kSmallQ[j]));
+    uint64_t surv = ~comp;
+    const uint64_t left = (high - B + 1) >> 1;
+    if (left < 64) surv &= (((uint64_t)1 << left) - 1);
+    while (surv) {
+      const int bit = __builtin_ctzll(surv);
+      surv &= surv - 1;
+      const uint64_t N = B + 2 * (uint64_t)bit;
+      if (N <= 113) continue;
+      const uint64_t q = fs_invert(N), one = (0 - N) % N;
+      if (fs_spsp2(N, q, one)) {
+        if (out && (uint64_t)n < cap) out[n] = N;
+        ++n;
+      }
+    }
+    if (B > UINT64_MAX - 128) break;
+  }
+  return n;
+}

@@ -1,0 +1,20 @@
+// fs_internal.hpp -- declarations shared by the library's translation units
+// (not part of the C ABI).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+#include <vector>
+#include "../../include/factsieve.h"
+
+extern "C" void fs_set_error_msg(const char* msg);
+
+// Primes <= limit (cached for the last limit asked).
+std::vector<uint32_t> fs_primes_upto(uint32_t limit);
+// Horner program of nmin!: multipliers < bound, 0 = square.
+std::vector<uint64_t> fs_horner_program(uint32_t nmin, uint64_t bound);
+// Sort by (p, |n|); verify on CPU threads (index of first failure or -1);
+// drop composite p and fold kept factors into *checksum (returns kept count).
+void fs_sort_factors(fs_factor* f, size_t n);
+int64_t fs_verify_all(const fs_factor* f, size_t n, int threads);
+size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose);
+uint32_t fs_auto_segment_candidates();

@@ -1,0 +1,237 @@
+/*
+ * factsieve.h -- C ABI of the MI355X-native FactSieve GPU sieve (n!+-1 factors).
+ *
+ * This is the drop-in boundary that replaces the reference's OpenCL layer:
+ *   - simpleCL.h:67-139 (sclGetCLSoftware / sclSetKernelArg / sclEnqueueKernel /
+ *     sclRead / sclWrite / release / sclPrintErrorFlags) and the raw cl* calls in
+ *     main.cpp:235-303 and cl_sieve.cpp:250-342, 654-1063;
+ *   - the ten OpenCL kernels (the kernels/<name>.cl files), which become entry points that
+ *     enqueue precompiled gfx950 code objects on a HIP stream;
+ *   - the host driver cl_sieve() / run_test() (cl_sieve.cpp:918-1391), kept with
+ *     the same inputs, outputs, factors.txt format and BOINC quorum checksum.
+ *
+ * Conventions: plain C types, device pointers are opaque `void*`-compatible
+ * pointers returned by fs_malloc, streams are `fs_stream` handles (0 = the
+ * library's default stream).  Every entry point returns FS_OK (0) or a negative
+ * FS_E* code and records a message for fs_last_error(); none of them exits the
+ * process (the reference prints and calls exit(EXIT_FAILURE), simpleCL.c:347 --
+ * the CLI executable restores that behaviour on top of these codes).
+ */
+#ifndef FACTSIEVE_H
+#define FACTSIEVE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define FS_OK 0
+#define FS_E_ARG -1          /* invalid argument / range (setupSearch, cl_sieve.cpp:557-589) */
+#define FS_E_HIP -2          /* HIP runtime failure (sclPrintErrorFlags, simpleCL.c:30-347) */
+#define FS_E_NOMEM -3        /* allocation failure (clCreateBuffer failure, cl_sieve.cpp:929-1068) */
+#define FS_E_OVERFLOW -4     /* candidate or factor buffer overflow (cl_sieve.cpp:451-476) */
+#define FS_E_VALIDATE -5     /* device validation flag (check.cl:66-78 -> cl_sieve.cpp:463-467) */
+#define FS_E_POWERTABLE -6   /* nmin! cross-check failed (verifyresult.cl -> cl_sieve.cpp:903-907) */
+#define FS_E_VERIFY -7       /* CPU re-verification of a factor failed (cl_sieve.cpp:502-506) */
+#define FS_E_IO -8           /* factors.txt / checkpoint I/O (cl_sieve.cpp:523-527, 742-765) */
+#define FS_E_NODEV -9        /* no usable gfx950 device */
+#define FS_E_BUSY -10        /* serving: no slot or memory free for the request */
+
+/* Device-side counter block, same index meaning as the reference's
+ * g_primecount[6] (clearresult.cl:15-23), plus two bounded-append flags. */
+#define FS_CTR_COUNT 0       /* candidates in the current segment */
+#define FS_CTR_MAXCOUNT 1    /* largest segment count since the last drain */
+#define FS_CTR_NFACTORS 2    /* factors appended since the last drain */
+#define FS_CTR_POWERTABLE 3  /* power-table / setup cross-check failed */
+#define FS_CTR_PRIMEOVF 4    /* candidate buffer overflow in getsegprimes */
+#define FS_CTR_VALIDATE 5    /* residue validation failed in check */
+#define FS_CTR_FACTOROVF 6   /* factor buffer overflow in iterate */
+#define FS_CTR_WORDS 8
+
+/* Device-side sum block: [0] candidates since last drain, [1] checksum fold of
+ * mont(nmax!) + mont(nmax) over those candidates (g_sum, check.cl:88-104). */
+#define FS_SUM_WORDS 2
+
+/* Arithmetic formulation of the residue kernels.  All produce bit-identical
+ * residues; the driver picks by pmax unless a run forces one. */
+#define FS_PATH_AUTO 0
+#define FS_PATH_FP64 1       /* p < 2^52: exact FP64 error-free products + rounded quotient */
+#define FS_PATH_HYBRID 2     /* p < 2^63: exact 64x32 integer product, FP64 quotient */
+#define FS_PATH_MONT 3       /* any odd p < 2^64: 64-bit Montgomery REDC (reference formulation) */
+
+/* 16-byte factor record, identical layout to `factor` (cl_sieve.h:4-7,
+ * iterate.cl:13-16): nc = -n for p | n!-1, +n for p | n!+1. */
+typedef struct {
+  uint64_t p;
+  int32_t nc;
+  int32_t pad;
+} fs_factor;
+
+/* Candidate set of one segment, structure-of-arrays in HBM (the reference packs
+ * a 64-byte ulong8 record per candidate, getsegprimes.cl:227-228).  All arrays
+ * have `cap` elements; the live count is counter FS_CTR_COUNT. */
+typedef struct {
+  uint64_t* p;      /* candidate modulus (base-2 strong probable prime) */
+  uint64_t* q;      /* p^-1 mod 2^64 (invert, getsegprimes.cl:73-77) */
+  uint64_t* r2;     /* 2^128 mod p */
+  uint64_t* res;    /* canonical residue n! mod p in [0, p) between launches */
+  uint32_t* nnext;  /* next n the residue belongs to (validated by check) */
+  uint32_t cap;
+  uint32_t grid_hint; /* 0, or the live count when the host knows it: kernels then
+                         launch only enough workgroups for min(grid_hint, cap) */
+} fs_cands;
+
+typedef void* fs_stream;
+typedef void* fs_event;
+typedef void* fs_graph;
+
+/* ------------------------------------------------ device / error handling */
+/* Open device `index` (HIP ordinal; main.cpp:230-275 picks platform 0 / GPU 0).
+ * Idempotent.  Replaces clGetPlatformIDs/clGetDeviceIDs/clCreateContext/
+ * clCreateCommandQueue (main.cpp:235-270). */
+int fs_init(int device_index);
+/* Name, compute units, total and free bytes (clGetDeviceInfo, main.cpp:283-308). */
+int fs_device_info(char* name, size_t name_len, uint32_t* compute_units, uint64_t* total_mem,
+                   uint64_t* free_mem);
+/* Message of the last failing call on this thread (sclPrintErrorFlags). */
+const char* fs_last_error(void);
+/* Fault injection for tests: the next check/verify launch raises the given
+ * counter flag (FS_CTR_POWERTABLE / _VALIDATE / _PRIMEOVF / _FACTOROVF); 0 clears. */
+int fs_inject_fault(int counter_index);
+
+/* -------------------------------------------------- memory (simpleCL.c I/O) */
+int fs_malloc(void** dptr, size_t bytes);                 /* clCreateBuffer */
+int fs_free(void* dptr);                                  /* sclReleaseMemObject */
+int fs_memcpy_htod(void* dst, const void* src, size_t bytes, fs_stream s, int blocking); /* sclWrite[NB] */
+int fs_memcpy_dtoh(void* dst, const void* src, size_t bytes, fs_stream s, int blocking); /* sclRead[NB] */
+int fs_memset(void* dptr, int value, size_t bytes, fs_stream s);
+int fs_cands_alloc(fs_cands* c, uint32_t cap);
+int fs_cands_free(fs_cands* c);
+
+/* -------------------------------------------- streams, events, graphs */
+int fs_stream_create(fs_stream* s);
+int fs_stream_destroy(fs_stream s);
+int fs_stream_sync(fs_stream s);                          /* sclFinish / sleepCPU (cl_sieve.cpp:289-354) */
+int fs_event_create(fs_event* e);
+int fs_event_destroy(fs_event e);
+int fs_event_record(fs_event e, fs_stream s);             /* clEnqueueMarker */
+int fs_event_query(fs_event e);                           /* 1 done, 0 pending (clGetEventInfo) */
+int fs_event_sync(fs_event e);                            /* waitOnEvent (cl_sieve.cpp:237-285) */
+int fs_event_elapsed_ms(fs_event start, fs_event stop, float* ms); /* ProfilesclEnqueueKernel */
+int fs_graph_begin(fs_stream s);                          /* start capturing launches on s */
+int fs_graph_end(fs_stream s, fs_graph* g);               /* instantiate the captured graph */
+int fs_graph_launch(fs_graph g, fs_stream s);
+int fs_graph_destroy(fs_graph g);
+
+/* ------------------------------------ kernels (reference kernels/<name>.cl) */
+/* clearn.cl:10-19: counter[FS_CTR_COUNT] = 0. */
+int fs_clearn(uint32_t* d_ctr, fs_stream s);
+/* clearresult.cl:10-25: zero the sum block and counters 1..7. */
+int fs_clearresult(uint32_t* d_ctr, uint64_t* d_sum, fs_stream s);
+/* getsegprimes.cl:170-239: append every base-2 strong probable prime in
+ * [low, high) that has no prime factor <= 113 to `c` (with q and r2);
+ * bounded append, overflow raises FS_CTR_PRIMEOVF.  low, high: any values with
+ * low <= high (no wheel alignment needed; findWheelOffset is internal). */
+int fs_getsegprimes(uint64_t low, uint64_t high, const fs_cands* c, uint32_t* d_ctr, fs_stream s);
+/* setup.cl:49-103: res = (res * prod of program[start..end)) per the setup
+ * program of `path` (built by fs_setup_program, prog_terms = its term count,
+ * see fs_setup_program); start == 0 resets res to 1 (FP64/HYBRID) or starts the
+ * power table at the prime 2 (MONT).  Sets nnext = nmin.  d_prog is the device
+ * copy of the program words. */
+int fs_setup(const fs_cands* c, const uint32_t* d_ctr, const void* d_prog, uint32_t prog_terms,
+             uint32_t start, uint32_t end, uint32_t nmin, int path, fs_stream s);
+/* iterate.cl:54-80: for n in [startN, endN): record (p, -n) if n! = 1 and
+ * (p, +n) if n! = -1 (mod p), then res <- res * (n+1).  Bounded factor append. */
+int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t factor_cap,
+               uint32_t startN, uint32_t endN, int path, fs_stream s);
+/* check.cl:50-109: fold mont(nmax!) + mont(nmax) into d_sum[1], count into
+ * d_sum[0], track the max count, validate every residue reached nmax. */
+int fs_check(const fs_cands* c, uint32_t* d_ctr, uint64_t* d_sum, uint32_t nmax, fs_stream s);
+/* verifyslow.cl / verifypow.cl / verifyreduce.cl / verifyresult.cl: for the
+ * first candidate of the segment, compute nmin! two ways (direct product and
+ * the reference power table) and compare with the setup kernel's residue;
+ * mismatch raises FS_CTR_POWERTABLE.  d_verify: scratch of fs_verify_bytes(). */
+size_t fs_verify_bytes(void);
+int fs_verifyslow(const fs_cands* c, const uint32_t* d_ctr, void* d_verify, uint32_t nmin, fs_stream s);
+int fs_verifypow(const fs_cands* c, const uint32_t* d_ctr, const uint64_t* d_bases,
+                 const uint32_t* d_exps, uint32_t count, void* d_verify, fs_stream s);
+int fs_verifyreduce(const fs_cands* c, const uint32_t* d_ctr, void* d_verify, fs_stream s);
+int fs_verifyresult(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_stream s);
+
+/* --------------------------------------------------- setup programs (host) */
+/* Reference power table (getPower + merge, cl_sieve.cpp:773-838): primes <= nmin,
+ * Legendre exponents, runs of equal exponent merged below 2^64-1.  Pass NULL
+ * arrays to get the term count only. */
+int64_t fs_power_table(uint32_t nmin, uint64_t* bases, uint32_t* exps, uint32_t* curbits);
+/* Setup program for `path` in words of 8 bytes; returns the term count T
+ * (pass NULL words to get it).  FP64: T doubles, HYBRID: T integers; each term
+ * is a multiplier (< 2^48 resp. < 2^31) or 0 = square the running product, and
+ * the ordered evaluation from 1 yields nmin!.  MONT: the reference power table,
+ * 2*T words: bases[T] then (exp | curbit << 32)[T]. */
+int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words);
+/* Path the driver uses for p < pmax. */
+int fs_select_path(uint64_t pmax);
+
+/* ------------------------------------------------ CPU verifier (verifyprime.c) */
+int fs_is_prime(uint64_t p);                              /* isPrime, verifyprime.c:96-132 */
+int fs_verify_factor(uint64_t p, uint32_t n, int32_t c);  /* verify, verifyprime.c:137-164 */
+int fs_parse_uint64(uint64_t* result, const char* str, uint64_t lo, uint64_t hi); /* putil.c:33-115 */
+
+/* ----------------------------- host execution of the kernel arithmetic */
+/* The per-candidate loops of the setup / iterate / getsegprimes kernels run on
+ * the CPU with the same inline arithmetic, for testing the three formulations
+ * without a GPU.  No reference counterpart; the sieve driver never calls them. */
+int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* res);
+int fs_hostemu_iterate(int path, uint64_t p, uint32_t startN, uint32_t endN, uint64_t* res,
+                       fs_factor* factors, uint32_t cap, uint32_t* nfactors);
+int64_t fs_hostemu_candidates(uint64_t low, uint64_t high, uint64_t* out, uint64_t cap);
+
+/* ------------------------------------------------------- whole sieve */
+typedef struct {
+  int path;                 /* FS_PATH_* (AUTO = by pmax) */
+  int verify_threads;       /* -v: OpenMP threads for CPU verification (default 2) */
+  int write_files;          /* 1: factors.txt + stateA/B.ckp in `workdir` (reference behaviour) */
+  int resume;               /* 1: resume from checkpoint files when present (non-test runs) */
+  int quiet;                /* 1: no stdout progress lines */
+  int test_mode;            /* 1: self-test semantics (truncate factors.txt, no resume) */
+  uint32_t seg_candidates;  /* target candidates per segment (0 = autotuned) */
+  uint32_t n_chunk;         /* n values per iterate launch (0 = whole range) */
+  uint32_t checkpoint_secs; /* drain + checkpoint period (reference: 60) */
+  const char* workdir;      /* directory for factors.txt / checkpoints (NULL = cwd) */
+  uint64_t stop_after_segments; /* test hook: stop (as if killed) after k segments (0 = off) */
+} fs_options;
+
+typedef struct {
+  uint64_t factorcount;     /* kept factors (prime p) */
+  uint64_t primecount;      /* candidates sieved */
+  uint64_t checksum;        /* BOINC quorum checksum */
+  uint64_t raw_factors;     /* factors found on the device before the isPrime filter */
+  uint64_t steps;           /* p*n residue steps executed (candidates x (nmax-nmin)) */
+  uint64_t setup_mulmods;   /* setup program steps x candidates */
+  double seconds;           /* wall time of the device work + drains */
+  double kernel_ms_iterate; /* summed iterate kernel time (HIP events) */
+  double kernel_ms_setup;
+  double kernel_ms_primegen;
+  uint32_t iterate_launches;
+  int path;                 /* path used */
+  uint32_t segments;
+  uint32_t resumed;         /* 1 if resumed from a checkpoint */
+} fs_result;
+
+void fs_default_options(fs_options* o);
+/* cl_sieve (cl_sieve.cpp:918-1270): sieve p in [pmin, pmax), n in [nmin, nmax).
+ * If factors/cap given, the kept factors (sorted by p, |n|) are copied there
+ * (*nfactors = total kept, may exceed cap). */
+int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax, const fs_options* opt,
+             fs_result* res, fs_factor* factors, uint64_t cap, uint64_t* nfactors);
+/* run_test (cl_sieve.cpp:1273-1391): the five reference ranges; returns the
+ * number of passing cases (5 = all). */
+int fs_self_test(const fs_options* opt, int* passed_mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FACTSIEVE_H */

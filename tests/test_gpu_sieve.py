@@ -1,0 +1,209 @@
+"""GPU parity: the HIP kernels and the sieve driver against the reference's
+self-test vectors, the oracle's golden fixtures and full-size invariants.
+Every call goes through the C ABI of libfactsieve.so."""
+import json
+import os
+import subprocess
+
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KATS = json.load(open(os.path.join(GOLDEN, "kats.json")))["cases"]
+TINY = json.load(open(os.path.join(GOLDEN, "tiny.json")))["cases"]
+C1 = json.load(open(os.path.join(GOLDEN, "c1.json")))
+
+
+def paths_for(fs, pmax):
+    if pmax <= 2 ** 52:
+        return [fs.PATH_FP64, fs.PATH_HYBRID, fs.PATH_MONT]
+    if pmax <= 2 ** 63:
+        return [fs.PATH_HYBRID, fs.PATH_MONT]
+    return [fs.PATH_MONT]
+
+
+@pytest.mark.parametrize("k", range(5))
+def test_reference_self_test_case(gpu, k):
+    pmin, pmax, nmin, nmax, nf, pc, ck = KATS[k]
+    print(f"KAT {k + 1}: p=[{pmin},{pmax}) n=[{nmin},{nmax})", flush=True)
+    r = gpu.sieve(pmin, pmax, nmin, nmax, verify_threads=16)
+    assert (r.factorcount, r.primecount, f"{r.checksum:016X}") == (nf, pc, ck)
+
+
+def test_c1_golden_factor_list(gpu):
+    r = gpu.sieve(*C1["range"])
+    assert r.factorcount == 25 and r.primecount == 108529 and f"{r.checksum:016X}" == C1["checksum"]
+    assert [list(f) for f in r.factors] == C1["factors"]
+
+
+@pytest.mark.parametrize("ci", range(len(TINY)))
+def test_tiny_cases_every_path(gpu, ci):
+    case = TINY[ci]
+    pmin, pmax, nmin, nmax = case["range"]
+    for path in paths_for(gpu, pmax):
+        print(f"tiny {ci} path {path}", flush=True)
+        r = gpu.sieve(pmin, pmax, nmin, nmax, path=path)
+        assert r.primecount == case["primecount"]
+        assert f"{r.checksum:016X}" == case["checksum"]
+        assert r.raw_factors == len(case["raw"])
+        assert [list(f) for f in r.factors] == case["factors"]
+
+
+@pytest.mark.parametrize("ci", range(len(TINY)))
+def test_kernels_per_candidate(gpu, ci):
+    """Each kernel on its own against the oracle's per-candidate residues."""
+    case = TINY[ci]
+    pmin, pmax, nmin, nmax = case["range"]
+    cands = case["cands"]
+    O = ol.oracle()
+    for path in paths_for(gpu, pmax):
+        print(f"kernels {ci} path {path}", flush=True)
+        seg = gpu.Segment(cap=len(cands) + 4096)
+        try:
+            seg.clearresult()
+            seg.clearn()
+            seg.getsegprimes(pmin, pmax)
+            n = seg.counters()[gpu.CTR_COUNT]
+            assert n == len(cands)
+            ps = seg.read_u64("p", n)
+            assert sorted(ps) == [c[0] for c in cands]
+            qs, r2s = seg.read_u64("q", n), seg.read_u64("r2", n)
+            for p, q, r2 in zip(ps, qs, r2s):
+                assert (p * q) % 2 ** 64 == 1 and r2 == 2 ** 128 % p
+            seg.load_program(nmin, path)
+            seg.setup(nmin, path)
+            res = seg.read_u64("res", n)
+            for p, r in zip(ps, res):
+                assert r == O.ora_factorial_mod(nmin, p), (path, p)
+            seg.verify(nmin)
+            assert seg.counters()[gpu.CTR_POWERTABLE] == 0
+            mid = (nmin + nmax) // 2
+            seg.iterate(nmin, mid, path)
+            seg.iterate(mid, nmax, path)
+            res = seg.read_u64("res", n)
+            want = {c[0]: c[1] * pow(2, -64, c[0]) % c[0] for c in cands}
+            for p, r in zip(ps, res):
+                assert r == want[p], (path, p)
+            assert sorted(seg.factors()) == sorted(tuple(f) for f in case["raw"])
+            seg.check(nmax)
+            ctr, sums = seg.counters(), seg.sums()
+            assert ctr[gpu.CTR_VALIDATE] == 0 and ctr[gpu.CTR_MAXCOUNT] == n
+            assert sums[0] == n and f"{sums[1]:016X}" == case["device_checksum"]
+        finally:
+            seg.close()
+
+
+def test_validation_flags_iterate_out_of_order(gpu):
+    """Launching iterate on a chunk that does not continue the residue's n is
+    caught by the residue-position check (check.cl:66-78 validation)."""
+    seg = gpu.Segment(cap=8192)
+    try:
+        seg.clearresult()
+        seg.clearn()
+        seg.getsegprimes(10 ** 8, 10 ** 8 + 20000)
+        seg.load_program(1000, gpu.PATH_FP64)
+        seg.setup(1000, gpu.PATH_FP64)
+        seg.iterate(1001, 1100, gpu.PATH_FP64)   # skips n = 1000
+        seg.check(1100)
+        assert seg.counters()[gpu.CTR_VALIDATE] == 1
+    finally:
+        seg.close()
+
+
+def test_paths_agree_at_full_size_c2(gpu):
+    """Full C2 (3.6e12 p*n steps): the FP64 and hybrid formulations are
+    independent arithmetic; their checksums and factor lists must agree."""
+    a = gpu.sieve(10 ** 12, 10 ** 12 + 10 ** 9, 10 ** 5, 2 * 10 ** 5, verify_threads=16, path=gpu.PATH_FP64)
+    b = gpu.sieve(10 ** 12, 10 ** 12 + 10 ** 9, 10 ** 5, 2 * 10 ** 5, verify_threads=16, path=gpu.PATH_HYBRID)
+    assert a.primecount == b.primecount == 36191004
+    assert (a.checksum, a.factors) == (b.checksum, b.factors)
+
+
+def test_c3_subrange_hybrid_vs_montgomery(gpu):
+    """C3's modulus regime (p ~ 2^62, nmin = 5e5) on a 1/64 p-subrange: the
+    hybrid path against the reference's Montgomery formulation."""
+    lo = 2 ** 62
+    args = (lo, lo + 2 ** 24, 500000, 510000)
+    a = gpu.sieve(*args, path=gpu.PATH_HYBRID)
+    b = gpu.sieve(*args, path=gpu.PATH_MONT)
+    assert a.primecount == b.primecount == ol.oracle().ora_candidates(lo, lo + 2 ** 24, None, 0)
+    assert (a.checksum, a.factors) == (b.checksum, b.factors)
+
+
+def test_segmentation_and_chunking_invariance(gpu):
+    g = C1["range"]
+    r = gpu.sieve(*g, seg_candidates=5000, n_chunk=777)
+    assert r.segments > 10 and r.iterate_launches > 100
+    assert (r.factorcount, r.primecount, f"{r.checksum:016X}") == (25, 108529, C1["checksum"])
+    assert [list(f) for f in r.factors] == C1["factors"]
+
+
+@pytest.mark.parametrize("args", [
+    (10 ** 8, 10 ** 8, 1000, 2000),          # empty p-range
+    (10 ** 8, 10 ** 8 + 100000, 5000, 5000),  # empty n-range
+    (114, 126, 114, 200),                     # no candidates
+    (2 ** 64 - 200000, 2 ** 64 - 1, 1000, 1500),
+    (2 ** 52 - 100000, 2 ** 52 + 100000, 3000, 3500),
+    (2 ** 63 - 100000, 2 ** 63 + 100000, 3000, 3500),
+])
+def test_edge_ranges_match_oracle(gpu, args):
+    print("edge", args, flush=True)
+    r = gpu.sieve(*args)
+    o = ol.sieve(*args, threads=8)
+    assert (r.factorcount, r.primecount, r.checksum) == (o["factorcount"], o["primecount"], o["checksum"])
+    assert r.factors == o["factors"]
+
+
+@pytest.mark.parametrize("flag,code", [(3, -6), (5, -5), (6, -4)])
+def test_fault_injection_is_reported(gpu, flag, code):
+    gpu.check(gpu.lib().fs_inject_fault(flag))
+    with pytest.raises(gpu.FactSieveError) as e:
+        gpu.sieve(10 ** 8, 10 ** 8 + 200000, 1000, 2000)
+    assert e.value.code == code
+    gpu.lib().fs_inject_fault(0)
+
+
+def test_candidate_overflow_recovers(gpu):
+    """A candidate-buffer overflow flag makes the driver grow the buffer and
+    redo the segment; the result is unchanged."""
+    gpu.check(gpu.lib().fs_inject_fault(gpu.CTR_PRIMEOVF))
+    r = gpu.sieve(*C1["range"])
+    assert (r.factorcount, r.primecount, f"{r.checksum:016X}") == (25, 108529, C1["checksum"])
+
+
+def test_invalid_ranges_rejected(gpu):
+    for args in [(10 ** 8, 10 ** 7, 1000, 2000), (1000, 2000, 1500, 1000), (200, 300, 500, 600)]:
+        with pytest.raises(gpu.FactSieveError) as e:
+            gpu.sieve(*args)
+        assert e.value.code == gpu.FS_E_ARG
+
+
+def test_cli_writes_reference_factors_file(gpu, tmp_path):
+    exe = gpu.EXE_PATH
+    r = subprocess.run([exe, "-n", "1000", "-N", "1e4", "-p", "1e8", "-P", "102e6", "-v", "4"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "factors 25, prime count 108529, checksum 000009F5FD6D6CFD" in r.stdout
+    lines = open(tmp_path / "factors.txt").read().splitlines()
+    want = [gpu.format_factor(p, nc) for p, nc in C1["factors"]] + [C1["checksum"]]
+    assert lines == want
+    # a finished workunit resumes straight to "Workunit complete."
+    r2 = subprocess.run([exe, "-n", "1000", "-N", "1e4", "-p", "1e8", "-P", "102e6"], cwd=tmp_path,
+                        capture_output=True, text=True, timeout=120)
+    assert r2.returncode == 0 and "Workunit complete." in r2.stdout
+
+
+def test_checkpoint_resume_matches_uninterrupted(gpu, tmp_path):
+    g = C1["range"]
+    opts = dict(write_files=1, resume=1, workdir=str(tmp_path), checkpoint_secs=0, seg_candidates=8000)
+    part = gpu.sieve(*g, stop_after_segments=4, **opts)
+    assert part.segments == 4
+    assert os.path.exists(tmp_path / "stateA.ckp")
+    full = gpu.sieve(*g, **opts)
+    assert full.resumed == 1
+    assert (full.factorcount, full.primecount, f"{full.checksum:016X}") == (25, 108529, C1["checksum"])
+    lines = open(tmp_path / "factors.txt").read().splitlines()
+    assert lines == [gpu.format_factor(p, nc) for p, nc in C1["factors"]] + [C1["checksum"]]

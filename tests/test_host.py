@@ -1,0 +1,185 @@
+"""Host logic of the product library on the CPU: exports of the C ABI, the
+setup programs, the CPU verifier, number parsing, and the three residue
+arithmetic formulations executed on the host, all against the oracle."""
+import ctypes
+import json
+import math
+import os
+import random
+import re
+import subprocess
+
+import pytest
+
+import oracle_lib as ol
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names += re.findall(r"\b(fs_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol(fs):
+    L = fs.lib()
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts(fs):
+    assert ctypes.sizeof(fs.Factor) == 16
+    assert ctypes.sizeof(fs.Cands) == 48
+
+
+def test_no_device_gives_clean_error(fs):
+    # this container has no GPU: init must fail with an error code, not crash
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU may be present")
+    rc = fs.lib().fs_init(0)
+    assert rc == fs.FS_E_NODEV and "device" in fs.last_error()
+
+
+def test_select_path(fs):
+    assert fs.select_path(2 ** 52) == fs.PATH_FP64
+    assert fs.select_path(2 ** 52 + 1) == fs.PATH_HYBRID
+    assert fs.select_path(2 ** 63) == fs.PATH_HYBRID
+    assert fs.select_path(2 ** 63 + 1) == fs.PATH_MONT
+
+
+@pytest.mark.parametrize("nmin", [127, 1000, 10000, 100000, 500000])
+def test_power_table_matches_oracle(fs, nmin):
+    O = ol.oracle()
+    B = ctypes.POINTER(ctypes.c_uint64)()
+    E = ctypes.POINTER(ctypes.c_uint32)()
+    C = ctypes.POINTER(ctypes.c_uint32)()
+    O.ora_power_table.restype = ctypes.c_uint64
+    m = O.ora_power_table(nmin, ctypes.byref(B), ctypes.byref(E), ctypes.byref(C))
+    b, e, c = fs.power_table(nmin)
+    assert len(b) == m
+    assert b == list(B[:m]) and e == list(E[:m]) and c == list(C[:m])
+
+
+def test_power_table_term_counts(fs):
+    # merged term counts (run-length merge of equal exponents below 2^64-1)
+    want = {127: 14, 1000: 46, 10000: 313, 100000: 2770, 500000: 13586, 1000000: 26032, 2000000: 49682}
+    for nmin, terms in want.items():
+        assert len(fs.power_table(nmin)[0]) == terms
+
+
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("nmin", [0, 1, 2, 3, 127, 1000, 4096, 20000])
+def test_setup_program_evaluates_to_factorial(fs, path, nmin):
+    import struct
+    n, words = fs.setup_program(nmin, path)
+    bound = 2 ** 48 if path == 1 else 2 ** 31
+    acc = 1
+    for w in words:
+        m = int(struct.unpack("<d", struct.pack("<Q", w))[0]) if path == 1 else w
+        assert 0 <= m < bound
+        acc = acc * acc if m == 0 else acc * m
+    assert acc == math.factorial(nmin)
+
+
+def test_is_prime_and_verify_match_oracle(fs):
+    O = ol.oracle()
+    rng = random.Random(2)
+    vals = [rng.getrandbits(64) | 1 for _ in range(3000)] + ol.candidates(10 ** 12, 10 ** 12 + 5000)
+    vals += json.load(open(os.path.join(GOLDEN, "spsp2_survivors_1e6.json")))["values"]
+    for v in vals:
+        assert fs.is_prime(v) == bool(O.ora_is_prime(v))
+    assert [v for v in range(0, 200) if fs.is_prime(v)] == [v for v in range(2, 200)
+                                                             if all(v % d for d in range(2, v))]
+    for p, nc in json.load(open(os.path.join(GOLDEN, "c1.json")))["factors"]:
+        n, c = abs(nc), (-1 if nc < 0 else 1)
+        assert fs.verify_factor(p, n, c) and not fs.verify_factor(p, n, -c) and not fs.verify_factor(p, n + 1, c)
+    for p in [2 ** 62 + 135, 2 ** 64 - 59]:
+        for n in [34, 35, 200]:
+            for c in (-1, 1):
+                assert fs.verify_factor(p, n, c) == bool(O.ora_verify(p, n, c))
+
+
+def test_parse_matches_oracle(fs):
+    O = ol.oracle()
+    strs = ["127", "1e8", "4K", "4M", "1k", "1g", "3b62", "0x7f", "12x", "1e99", "99999999999999999999",
+            "18446744073709551615", "1b64", "abc", "", "1E3", "1B4", "2T", "1P", "1p", "5Kx"]
+    for s in strs:
+        for lo, hi in [(127, 2 ** 64 - 2), (128, 2 ** 64 - 1), (127, 2 ** 31 - 2)]:
+            a = ol.u64(0)
+            ra = O.ora_parse_uint64(ctypes.byref(a), s.encode(), lo, hi)
+            rb, vb = fs.parse_uint64(s, lo, hi)
+            assert (ra, a.value if ra == 0 else None) == (rb, vb if rb == 0 else None), (s, lo, hi)
+
+
+EDGE_RANGES = [
+    (10 ** 8, 10 ** 8 + 4000, [1, 2, 3]),
+    ((1 << 52) - 3000, 1 << 52, [1, 2, 3]),
+    ((1 << 52) + 1, (1 << 52) + 3000, [2, 3]),
+    ((1 << 62) - 3000, 1 << 62, [2, 3]),
+    ((1 << 63) - 3000, 1 << 63, [2, 3]),
+    ((1 << 64) - 3000, (1 << 64) - 1, [3]),
+]
+
+
+@pytest.mark.parametrize("lo,hi,paths", EDGE_RANGES)
+def test_host_arithmetic_matches_oracle(fs, lo, hi, paths):
+    """The kernels' inline arithmetic, executed on the host, against the
+    oracle's Montgomery residues, at the edges of each formulation's range and
+    for n up to 2^31 - 1."""
+    O = ol.oracle()
+    rng = random.Random(lo)
+    ps = ol.candidates(lo, hi)
+    assert ps
+    for p in rng.sample(ps, min(6, len(ps))):
+        for nmin, span in [(127, 2000), (5000, 1500), (2 ** 31 - 1500, 1499)]:
+            want = O.ora_factorial_mod(nmin, p) if nmin < 10 ** 6 else None
+            if want is None:
+                # start from a random residue instead of nmin!
+                want = rng.randrange(p)
+                start = want
+            for path in paths:
+                if nmin < 10 ** 6:
+                    start = fs.hostemu_setup(path, p, nmin)
+                    assert start == want, (path, p, nmin)
+                r, facs, nf = fs.hostemu_iterate(path, p, nmin, nmin + span, start)
+                M = ol.mont(p)
+                s6 = ol.u64(O.ora_mmul(start, M.r2, p, M.q))
+                s7 = ol.u64(O.ora_mmul(nmin % p, M.r2, p, M.q))
+                buf = (ol.OFactor * 64)()
+                onf = O.ora_iterate(ctypes.byref(M), nmin, nmin + span, ctypes.byref(s6), ctypes.byref(s7), buf, 64)
+                assert r == O.ora_mmul(s6.value, 1, p, M.q), (path, p, nmin)
+                assert facs == [(buf[i].p, buf[i].nc) for i in range(onf)]
+
+
+def test_host_arithmetic_planted_factor(fs):
+    """Start residues at +1 and -1: every formulation must report the factor."""
+    for p, paths in [(1000003, [1, 2, 3]), (2 ** 61 - 1, [2, 3]), (2 ** 64 - 59, [3])]:
+        for start, sign in [(1, -1), (p - 1, 1)]:
+            for path in paths:
+                _, facs, _ = fs.hostemu_iterate(path, p, 500, 502, start)
+                assert facs[0] == (p, sign * 500), (p, path)
+
+
+@pytest.mark.parametrize("lo,hi", [(127, 100000), (10 ** 8, 10 ** 8 + 200000), (2 ** 52 - 100000, 2 ** 52 + 100000),
+                                   (2 ** 62, 2 ** 62 + 100000), (2 ** 64 - 300000, 2 ** 64 - 1), (1000, 1001),
+                                   (127, 128), (2 ** 64 - 2, 2 ** 64 - 1)])
+def test_candidate_masks_match_oracle(fs, lo, hi):
+    assert fs.hostemu_candidates(lo, hi) == ol.candidates(lo, hi)
+
+
+def test_cli_rejects_bad_arguments(fs):
+    exe = fs.EXE_PATH
+    r = subprocess.run([exe, "-p", "12x"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "invalid argument -p 12x" in r.stdout
+    r = subprocess.run([exe, "-n", "99"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "out of range argument -n 99" in r.stdout
+    r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "-N #" in r.stdout
