@@ -220,10 +220,14 @@ def power_table(nmin):
 
 
 def setup_program(nmin, path):
+    """(terms, words): the device words of the setup program -- FP64: terms
+    doubles; HYBRID: terms multipliers then the power-of-two exponent word;
+    MONT: the power table as bases[terms] then (exp | curbit << 32)[terms]."""
     n = check(lib().fs_setup_program(nmin, path, None))
     words = (u64 * (max(2 * n, 1) + 1))()
     lib().fs_setup_program(nmin, path, words)
-    return n, list(words[: (2 * n if path == PATH_MONT else n)])
+    nw = {PATH_MONT: 2 * n, PATH_HYBRID: n + 1}.get(path, n)
+    return n, list(words[:nw])
 
 
 def hostemu_setup(path, p, nmin):

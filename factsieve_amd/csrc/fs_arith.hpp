@@ -123,11 +123,41 @@ FS_HD uint64_t fs_hy_mulmod(uint64_t s, uint32_t m, double md, uint64_t p,
   double x = __builtin_fma(y, md, -0.00390625);
   x = __builtin_fmax(x, 0.0);
   const uint32_t k = (uint32_t)x;
-  // s*m - k*p mod 2^64 (exact value lies in [0, 2p))
+  // s*m - k*p mod 2^64 (exact value lies in [0, 2p)); each 64-bit product is
+  // one full 32x32->64 partial product plus the low word of a second one
   const uint64_t t0 = (uint64_t)s0 * m;
-  const uint64_t t = ((uint64_t)((uint32_t)(t0 >> 32) + s1 * m) << 32) | (uint32_t)t0;
-  const uint64_t kp = k * p;
+  const uint32_t th = (uint32_t)(t0 >> 32) + s1 * m;
+  const uint64_t k0 = (uint64_t)k * (uint32_t)p;
+  const uint32_t kh = (uint32_t)(k0 >> 32) + k * (uint32_t)(p >> 32);
+  const uint64_t t = ((uint64_t)th << 32) | (uint32_t)t0;
+  const uint64_t kp = ((uint64_t)kh << 32) | (uint32_t)k0;
   return t - kp;
+}
+
+// --------------------------------------------- 64x32 Montgomery (REDC by 2^32)
+
+// s*m*2^-32 mod p for s < 2p, m < 2^31, odd p < 2^63, pn32 = -p^-1 mod 2^32;
+// result < 2p.  Five 32x32 partial products (four of them full 64-bit) and
+// one 64-bit add: the setup program's multiply on the hybrid path.
+FS_HD uint64_t fs_redc32_mul(uint64_t s, uint32_t m, uint64_t p, uint32_t pn32) {
+  const uint64_t a = (uint64_t)(uint32_t)s * m;                       // t mod 2^32 + carry
+  const uint64_t b = (uint64_t)(uint32_t)(s >> 32) * m + (a >> 32);  // t >> 32
+  const uint32_t u = (uint32_t)a * pn32;                              // t + u*p = 0 mod 2^32
+  const uint64_t c = (uint64_t)u * (uint32_t)p + (uint32_t)a;         // low word is zero
+  const uint64_t e = (uint64_t)u * (uint32_t)(p >> 32) + b;
+  return e + (c >> 32);
+}
+
+// 2^e * 2^64 mod p (Montgomery form of 2^e) for e >= 1: left-to-right
+// square-and-double starting from 2, one = 2^64 mod p.
+FS_HD uint64_t fs_mont_pow2(uint64_t e, uint64_t p, uint64_t q, uint64_t one) {
+  uint64_t a = fs_add_mod(one, one, p);
+  const int top = 63 - __builtin_clzll(e);
+  for (int b = top - 1; b >= 0; --b) {
+    a = fs_mont_mul(a, a, p, q);
+    if ((e >> b) & 1) a = fs_add_mod(a, a, p);
+  }
+  return a;
 }
 
 // ------------------------------------------------------------ candidate tests

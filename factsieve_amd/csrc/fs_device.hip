@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <string>
+#include <map>
 #include <mutex>
 #include "fs_arith.hpp"
 #include "../../include/factsieve.h"
@@ -191,25 +192,32 @@ __global__ __launch_bounds__(FS_BLOCK) void k_setup_fp64(fs_cands c, const uint3
   c.nnext[i] = nmin;
 }
 
+// Hybrid path: the running product is kept as x * 2^-E mod p (lazily below
+// 2p) because each multiply is a REDC by 2^32; squarings are Montgomery
+// products by 2^64.  prog[terms] holds the total E, removed after the last term.
 __global__ __launch_bounds__(FS_BLOCK) void k_setup_hybrid(fs_cands c, const uint32_t* ctr,
-                                                           const uint64_t* prog, uint32_t start,
-                                                           uint32_t end, uint32_t nmin) {
+                                                           const uint64_t* prog, uint32_t terms,
+                                                           uint32_t start, uint32_t end, uint32_t nmin) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
-  const uint64_t p = c.p[i];
-  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+  const uint64_t p = c.p[i], q = c.q[i];
+  const uint32_t pn32 = (uint32_t)(0 - q);
   uint64_t s = (start == 0) ? 1 : c.res[i];
   for (uint32_t k = start; k < end; ++k) {
     const uint32_t m = (uint32_t)prog[k];
     if (m == 0) {
-      const uint64_t q = c.q[i], r2 = c.r2[i];
       const uint64_t a = s >= p ? s - p : s;
-      s = fs_mul_mod(a, a, p, q, r2);
+      s = fs_mont_mul(a, a, p, q);
     } else {
-      s = fs_hy_mulmod(s, m, (double)m, p, pinv, pinv32);
+      s = fs_redc32_mul(s, m, p, pn32);
     }
   }
-  c.res[i] = s >= p ? s - p : s;
+  s = s >= p ? s - p : s;
+  if (end == terms && prog[terms] != 0) {
+    const uint64_t one = fs_mont_mul(1, c.r2[i], p, q);
+    s = fs_mont_mul(s, fs_mont_pow2(prog[terms], p, q, one), p, q);
+  }
+  c.res[i] = s;
   c.nnext[i] = nmin;
 }
 
@@ -258,24 +266,62 @@ __global__ __launch_bounds__(FS_BLOCK) void k_setup_mont(fs_cands c, const uint3
 
 // ---- iterate: for n in [n0, n1): capture n! = +-1, then multiply by n+1
 
+// FP64 path.  The multipliers n+1 are read as doubles from nt (nt[j] =
+// n0 + 1 + j) with scalar loads, so a step costs six FP64 VALU operations plus
+// one compare; four steps share one branch on the OR of their +-1 tests.
+__device__ __forceinline__ void fs_fp_record(uint32_t* ctr, fs_factor* fac, uint32_t fcap, uint64_t p,
+                                             double v, uint32_t n) {
+  if (__builtin_fabs(v) == 1.0) fs_record(ctr, fac, fcap, p, v > 0.0 ? -(int32_t)n : (int32_t)n);
+}
+
 __global__ __launch_bounds__(FS_BLOCK) void k_iterate_fp64(fs_cands c, uint32_t* ctr, fs_factor* fac,
-                                                           uint32_t fcap, uint32_t n0, uint32_t n1) {
+                                                           uint32_t fcap, uint32_t n0, uint32_t n1,
+                                                           const double* __restrict__ nt) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
   const uint64_t p = c.p[i];
   if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = fs_canon_to_fp(c.res[i], p);
-  for (uint32_t n = n0; n < n1; ++n) {
-    if (__builtin_fabs(s) == 1.0) fs_record(ctr, fac, fcap, p, s > 0.0 ? -(int32_t)n : (int32_t)n);
-    s = fs_fp_mulmod(s, (double)(n + 1), pd, pinv);
+  uint32_t n = n0;
+  const uint32_t n4 = n0 + ((n1 - n0) & ~3u);
+  // nt holds at least 4 entries past n1 - n0, so the next block's multipliers
+  // are loaded one iteration ahead
+  double m0 = nt[0], m1 = nt[1], m2 = nt[2], m3 = nt[3];
+  for (; n < n4; n += 4) {
+    const double* mn = nt + (n - n0) + 4;
+    const double x0 = mn[0], x1 = mn[1], x2 = mn[2], x3 = mn[3];
+    const double a = s;
+    const double b = fs_fp_mulmod(a, m0, pd, pinv);
+    const double d = fs_fp_mulmod(b, m1, pd, pinv);
+    const double e = fs_fp_mulmod(d, m2, pd, pinv);
+    s = fs_fp_mulmod(e, m3, pd, pinv);
+    m0 = x0, m1 = x1, m2 = x2, m3 = x3;
+    const bool hit = (__builtin_fabs(a) == 1.0) | (__builtin_fabs(b) == 1.0) | (__builtin_fabs(d) == 1.0) |
+                     (__builtin_fabs(e) == 1.0);
+    if (hit) {
+      fs_fp_record(ctr, fac, fcap, p, a, n);
+      fs_fp_record(ctr, fac, fcap, p, b, n + 1);
+      fs_fp_record(ctr, fac, fcap, p, d, n + 2);
+      fs_fp_record(ctr, fac, fcap, p, e, n + 3);
+    }
+  }
+  for (; n < n1; ++n) {
+    fs_fp_record(ctr, fac, fcap, p, s, n);
+    s = fs_fp_mulmod(s, nt[n - n0], pd, pinv);
   }
   c.res[i] = fs_fp_to_canon(s, p);
   c.nnext[i] = n1;
 }
 
+__global__ void k_fill_ntab(double* t, uint32_t first, uint32_t count) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < count) t[j] = (double)(first + j);
+}
+
 __global__ __launch_bounds__(FS_BLOCK) void k_iterate_hybrid(fs_cands c, uint32_t* ctr, fs_factor* fac,
-                                                             uint32_t fcap, uint32_t n0, uint32_t n1) {
+                                                             uint32_t fcap, uint32_t n0, uint32_t n1,
+                                                             const double* __restrict__ nt) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
   const uint64_t p = c.p[i];
@@ -287,8 +333,9 @@ __global__ __launch_bounds__(FS_BLOCK) void k_iterate_hybrid(fs_cands c, uint32_
     const bool plus1 = (s == 1) | (s == pp1);
     const bool minus1 = (s == pm1) | (s == p2m1);
     if (plus1 | minus1) fs_record(ctr, fac, fcap, p, plus1 ? -(int32_t)n : (int32_t)n);
-    const uint32_t m = n + 1;
-    s = fs_hy_mulmod(s, m, (double)m, p, pinv, pinv32);
+    uint32_t m = n + 1;
+    asm volatile("" : "+s"(m));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
+    s = fs_hy_mulmod(s, m, nt[n - n0], p, pinv, pinv32);
   }
   c.res[i] = s >= p ? s - p : s;
   c.nnext[i] = n1;
@@ -699,8 +746,9 @@ extern "C" int fs_setup(const fs_cands* c, const uint32_t* d_ctr, const void* d_
     case FS_PATH_FP64:
       k_setup_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (const double*)d_prog, start, end, nmin);
       break;
-    case FS_PATH_HYBRID:
-      k_setup_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (const uint64_t*)d_prog, start, end, nmin);
+    case FS_PATH_HYBRID:  // d_prog: prog_terms multipliers, then the 2-exponent word
+      k_setup_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (const uint64_t*)d_prog, prog_terms, start, end,
+                                              nmin);
       break;
     case FS_PATH_MONT:  // d_prog: bases[prog_terms] then (exp | curbit << 32)[prog_terms]
       k_setup_mont<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (const uint64_t*)d_prog, prog_terms, start,
@@ -714,6 +762,44 @@ extern "C" int fs_setup(const fs_cands* c, const uint32_t* d_ctr, const void* d_
   return FS_OK;
 }
 
+// Multiplier tables of the FP64 iterate kernel, one per stream: nt[j] = base + j
+// as doubles, refilled on the stream (stream order protects earlier launches).
+struct NTab {
+  double* d = nullptr;
+  uint32_t base = 0, len = 0, cap = 0;
+};
+static std::mutex g_ntab_mu;
+static std::map<hipStream_t, NTab> g_ntab;
+
+static int ntab_for(hipStream_t st, uint32_t first, uint32_t count, const double** out) {
+  std::lock_guard<std::mutex> lk(g_ntab_mu);
+  NTab& t = g_ntab[st];
+  if (t.d && first >= t.base && (uint64_t)first + count + 8 <= (uint64_t)t.base + t.len) {
+    *out = t.d + (first - t.base);
+    return FS_OK;
+  }
+  // 8 entries of padding: the kernel reads one unrolled block ahead
+  const uint32_t want = (count < 65536 ? 65536 : count) + 8;
+  if (want > t.cap) {
+    if (t.d) HIPCHK(hipFree(t.d), "hipFree(ntab)");
+    t.d = nullptr;
+    hipError_t e = hipMalloc(&t.d, (size_t)want * sizeof(double));
+    if (e != hipSuccess) {
+      set_error("hipMalloc(ntab)", e);
+      t.cap = t.len = 0;
+      return FS_E_NOMEM;
+    }
+    t.cap = want;
+  }
+  const uint32_t len = want;  // values past 2^31 are never used as multipliers
+  k_fill_ntab<<<(len + 255) / 256, 256, 0, st>>>(t.d, first, len);
+  HIPCHK(hipGetLastError(), "fill ntab");
+  t.base = first;
+  t.len = len;
+  *out = t.d;
+  return FS_OK;
+}
+
 extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t fcap,
                           uint32_t n0, uint32_t n1, int path, fs_stream s) {
   int rc = check_cands(c, "iterate");
@@ -724,12 +810,18 @@ extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_facto
   }
   const uint32_t g = cand_grid(c);
   switch (path) {
-    case FS_PATH_FP64:
-      k_iterate_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1);
+    case FS_PATH_FP64: {
+      const double* nt = nullptr;
+      if ((rc = ntab_for(S(s), n0 + 1, n1 - n0, &nt))) return rc;
+      k_iterate_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, nt);
       break;
-    case FS_PATH_HYBRID:
-      k_iterate_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1);
+    }
+    case FS_PATH_HYBRID: {
+      const double* nt = nullptr;
+      if ((rc = ntab_for(S(s), n0 + 1, n1 - n0, &nt))) return rc;
+      k_iterate_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, nt);
       break;
+    }
     case FS_PATH_MONT:
       k_iterate_mont<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1);
       break;

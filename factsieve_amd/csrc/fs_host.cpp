@@ -157,14 +157,17 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
   const uint64_t bound = (path == FS_PATH_FP64) ? ((uint64_t)1 << 48) : ((uint64_t)1 << 31);
   const std::vector<uint64_t> prog = fs_horner_program(nmin, bound);
   if (words) {
+    uint64_t E = 0;  // HYBRID: power of 2^-1 accumulated by REDC-by-2^32 / Montgomery squarings
     for (size_t i = 0; i < prog.size(); ++i) {
       if (path == FS_PATH_FP64) {
         const double d = (double)prog[i];  // exact: < 2^48
         memcpy(&words[i], &d, 8);
       } else {
         words[i] = prog[i];
+        E = (prog[i] == 0) ? 2 * E + 64 : E + 32;
       }
     }
+    if (path == FS_PATH_HYBRID) words[prog.size()] = E;
   }
   return (int64_t)prog.size();
 }

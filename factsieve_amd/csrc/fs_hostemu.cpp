@@ -36,18 +36,20 @@ extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* r
     *res = fs_fp_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {
     if (p >= ((uint64_t)1 << 63)) return FS_E_ARG;
-    const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+    const uint32_t pn32 = (uint32_t)(0 - q);
     uint64_t s = 1;
     for (int64_t k = 0; k < T; ++k) {
       const uint32_t m = (uint32_t)w[k];
       if (m == 0) {
         const uint64_t a = s >= p ? s - p : s;
-        s = fs_mul_mod(a, a, p, q, r2);
+        s = fs_mont_mul(a, a, p, q);
       } else {
-        s = fs_hy_mulmod(s, m, (double)m, p, pinv, pinv32);
+        s = fs_redc32_mul(s, m, p, pn32);
       }
     }
-    *res = s >= p ? s - p : s;
+    s = s >= p ? s - p : s;
+    if (w[T] != 0) s = fs_mont_mul(s, fs_mont_pow2(w[T], p, q, one), p, q);
+    *res = s;
   } else if (path == FS_PATH_MONT) {
     uint64_t s6 = one;
     for (int64_t k = 0; k < T; ++k) {
