@@ -69,6 +69,78 @@ def cpu_baseline(pmin, nmin, nmax, target_s=15.0):
                       f"n in [{nmin}, {nmax}): {r['primecount']} candidates, {steps:.3e} p*n steps in {dt:.1f} s"}
 
 
+def bench_batched(fs, njobs=64, seed=None):
+    """Mixed-job config: the same 64 jobs as one batched persistent run and as
+    64 plain single-job runs; results must agree job by job."""
+    from factsieve_amd import workloads
+    jobs = workloads.mixed_jobs(njobs, seed=workloads.DEFAULT_SEED if seed is None else seed)
+    opts = fs.batch_autotune(jobs)
+    fs.batch(jobs, **opts)  # warm (programs cached, buffers sized)
+    t = time.perf_counter()
+    res, _ = fs.batch(jobs, **opts)
+    t_batch = time.perf_counter() - t
+    t = time.perf_counter()
+    plain = [fs.sieve(*j, verify_threads=16) for j in jobs]
+    t_plain = time.perf_counter() - t
+    same = all((r.factorcount, r.primecount, r.checksum, r.factors) ==
+               (p.factorcount, p.primecount, p.checksum, p.factors) and r.status == 0 for r, p in zip(res, plain))
+    steps = sum(r.steps for r in res)
+    return {"workload": f"{njobs} mixed jobs (seed {workloads.DEFAULT_SEED if seed is None else seed}): n_start "
+                        "U[1e3,5e5], n_len lognormal(median 2e4, sigma 1), p_start 2^U[32,63], p_len "
+                        "lognormal(median 2^24, sigma 1)",
+            "pn_steps": steps, "candidates": sum(r.primecount for r in res),
+            "batched_steps_per_s": steps / t_batch, "plain_steps_per_s": steps / t_plain,
+            "batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
+            "bit_exact_vs_plain": same, "autotuned": opts,
+            "paths": {k: sum(1 for r in res if r.path == v) for k, v in (("fp64", 1), ("hybrid", 2), ("mont", 3))}}, \
+        {tuple(j): r for j, r in zip(jobs, res)}
+
+
+def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192):
+    """Poisson arrivals of mixed workunits into the continuous-batching server;
+    p50/p99 arrival-to-result latency and aggregate steps/s."""
+    from factsieve_amd import workloads
+    jobs = workloads.mixed_jobs(nwu)
+    mean_steps = sum(r.steps for r in reference.values()) / max(len(reference), 1)
+    cap = bench_serving.capacity_steps_per_s
+    rate = rate_frac * cap / mean_steps
+    arrivals = workloads.poisson_arrivals(rate, nwu)
+    srv = fs.Server(n_chunk=n_chunk)
+    ids, lat, results = {}, {}, {}
+    t0 = time.perf_counter()
+    nxt = 0
+    try:
+        while len(results) < nwu:
+            now = time.perf_counter() - t0
+            while nxt < nwu and arrivals[nxt] <= now:
+                ids[srv.submit(jobs[nxt])] = nxt
+                nxt += 1
+            st = srv.stats()
+            if st["active"] == 0 and st["queued"] == 0 and nxt < nwu:
+                time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
+                continue
+            srv.step()
+            now = time.perf_counter() - t0
+            for wid, k in ids.items():
+                if k in results:
+                    continue
+                r = srv.poll(wid)
+                if r is not None:
+                    results[k] = r
+                    lat[k] = now - arrivals[k]
+        makespan = time.perf_counter() - t0
+    finally:
+        srv.close()
+    steps = sum(r.steps for r in results.values())
+    same = all((results[k].factorcount, results[k].primecount, results[k].checksum, results[k].factors) ==
+               (reference[tuple(jobs[k])].factorcount, reference[tuple(jobs[k])].primecount,
+                reference[tuple(jobs[k])].checksum, reference[tuple(jobs[k])].factors) for k in results)
+    ls = sorted(lat.values())
+    return {"workunits": nwu, "arrival_rate_per_s": rate, "target_utilisation": rate_frac, "n_chunk": n_chunk,
+            "p50_latency_s": ls[len(ls) // 2], "p99_latency_s": ls[min(len(ls) - 1, int(0.99 * len(ls)))],
+            "makespan_s": makespan, "steps_per_s": steps / makespan, "bit_exact_vs_batched_alone": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +148,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched and serving legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -136,6 +209,11 @@ def main():
                               "setup": sum(r.kernel_ms_setup for r in results) / args.steps,
                               "primegen": sum(r.kernel_ms_primegen for r in results) / args.steps},
     }
+    if not args.no_batch:
+        bat, ref = bench_batched(fs)
+        line["batched"] = bat
+        bench_serving.capacity_steps_per_s = bat["batched_steps_per_s"]
+        line["serving"] = bench_serving(fs, ref)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(pmin, nmin, nmax)
         line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

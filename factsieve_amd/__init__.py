@@ -67,8 +67,8 @@ class JobResult(ctypes.Structure):
 
 
 class BatchOptions(ctypes.Structure):
-    _fields_ = [("tile_candidates", u32), ("n_chunk", u32), ("unroll", u32), ("use_graph", u32),
-                ("persistent_blocks", u32), ("verify_threads", ctypes.c_int)]
+    _fields_ = [("tile_candidates", u32), ("persistent_blocks", u32), ("use_graph", u32), ("lpt_order", u32),
+                ("verify_threads", ctypes.c_int)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -416,3 +416,87 @@ class Segment:
         h = (Factor * max(n, 1))()
         check(lib().fs_memcpy_dtoh(h, self.fac, n * 16, self.stream, 1))
         return [(h[i].p, h[i].nc) for i in range(n)]
+
+
+# ------------------------------------------------------------------ batched form and serving
+
+JobOutcome = namedtuple("JobOutcome", "factorcount primecount checksum raw_factors steps status path factors")
+
+
+def default_batch_options(**kw):
+    o = BatchOptions()
+    lib().fs_default_batch_options(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _jobs_array(jobs):
+    arr = (Job * max(len(jobs), 1))()
+    for i, (pmin, pmax, nmin, nmax) in enumerate(jobs):
+        arr[i].pmin, arr[i].pmax, arr[i].nmin, arr[i].nmax = pmin, pmax, nmin, nmax
+    return arr
+
+
+def batch(jobs, factor_cap=1 << 20, **opts):
+    """Run many (pmin, pmax, nmin, nmax) jobs in one batched persistent launch.
+    Returns (list of JobOutcome, wall seconds)."""
+    o = default_batch_options(**opts)
+    arr = _jobs_array(jobs)
+    out = (JobResult * max(len(jobs), 1))()
+    fac = (Factor * factor_cap)()
+    per = (u64 * max(len(jobs), 1))()
+    sec = ctypes.c_double()
+    check(lib().fs_batch(arr, len(jobs), ctypes.byref(o), out, fac, factor_cap, per, ctypes.byref(sec)))
+    res, k = [], 0
+    for i in range(len(jobs)):
+        r = out[i]
+        fl = [(fac[k + t].p, fac[k + t].nc) for t in range(per[i]) if k + t < factor_cap]
+        k += per[i]
+        res.append(JobOutcome(r.factorcount, r.primecount, r.checksum, r.raw_factors, r.steps, r.status, r.path, fl))
+    return res, sec.value
+
+
+def batch_autotune(jobs, **opts):
+    o = default_batch_options(**opts)
+    arr = _jobs_array(jobs)
+    check(lib().fs_batch_autotune(arr, len(jobs), ctypes.byref(o)))
+    return {f: getattr(o, f) for f, _ in BatchOptions._fields_}
+
+
+class Server:
+    """Continuous-batching serving loop over the C ABI (fs_server_*)."""
+
+    def __init__(self, mem_budget=0, max_slots=256, n_chunk=8192, verify_threads=16):
+        o = ServerOptions()
+        lib().fs_default_server_options(ctypes.byref(o))
+        o.mem_budget, o.max_slots, o.n_chunk, o.verify_threads = mem_budget, max_slots, n_chunk, verify_threads
+        self.h = vp()
+        check(lib().fs_server_create(ctypes.byref(o), ctypes.byref(self.h)))
+
+    def submit(self, job):
+        j = _jobs_array([job])
+        return check(lib().fs_server_submit(self.h, ctypes.byref(j[0])))
+
+    def step(self):
+        return check(lib().fs_server_step(self.h))
+
+    def poll(self, wid, factor_cap=1 << 16):
+        r = JobResult()
+        fac = (Factor * factor_cap)()
+        nf = u64()
+        done = check(lib().fs_server_poll(self.h, wid, ctypes.byref(r), fac, factor_cap, ctypes.byref(nf)))
+        if not done:
+            return None
+        fl = [(fac[i].p, fac[i].nc) for i in range(min(nf.value, factor_cap))]
+        return JobOutcome(r.factorcount, r.primecount, r.checksum, r.raw_factors, r.steps, r.status, r.path, fl)
+
+    def stats(self):
+        q, a, d, b = u32(), u32(), u32(), u64()
+        check(lib().fs_server_stats(self.h, ctypes.byref(q), ctypes.byref(a), ctypes.byref(d), ctypes.byref(b)))
+        return {"queued": q.value, "active": a.value, "done": d.value, "bytes_in_use": b.value}
+
+    def close(self):
+        if self.h:
+            lib().fs_server_destroy(self.h)
+            self.h = vp()

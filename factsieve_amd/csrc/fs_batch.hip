@@ -1,0 +1,208 @@
+// fs_batch.hip -- many sieve jobs in one persistent launch.
+//
+// A batch is run as four device launches on one stream (optionally captured
+// into a hipGraph):
+//   k_bclear       zero the per-job counters and the tile queue
+//   k_bprimegen    candidate generation for every job at once: workgroup b
+//                  belongs to the job whose block range holds b and appends
+//                  into that job's candidate region (bounded)
+//   k_bmake_tiles  one workgroup turns the per-job candidate counts into a tile
+//                  list (job, first candidate, n0, n1) in cost order
+//   k_bsieve       persistent workgroups pull tiles from an atomic queue; for
+//                  each candidate of a tile: setup (nmin!), iterate over the
+//                  tile's n-range with +-1 capture, and the checksum fold, all
+//                  in registers -- no residue goes back to memory in between
+// The serving loop (fs_server.cpp) drives the same kernels with n-chunked tiles
+// and keeps residues in the candidate regions between chunks.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <vector>
+#include "fs_arith.hpp"
+#include "fs_kern.hpp"
+#include "fs_internal.hpp"
+#include "fs_batch_internal.hpp"
+#include "../../include/factsieve.h"
+
+#define BCHK(call, what)                          \
+  do {                                            \
+    hipError_t e__ = (call);                      \
+    if (e__ != hipSuccess) {                      \
+      fs_set_hip_error(what, (int)e__);           \
+      return FS_E_HIP;                            \
+    }                                             \
+  } while (0)
+
+// ================================================================== kernels
+
+// Clears the counters of the jobs that start in this launch and the global words.
+__global__ void k_bclear(const fs_bjob* __restrict__ jobs, uint32_t* jcnt, uint32_t* jflag, uint64_t* jsum,
+                         uint32_t nj, uint32_t* glob) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nj && (jobs[t].flags & FS_BJ_NEW)) {
+    jcnt[t] = 0;
+    jflag[t] = 0;
+    jsum[t] = 0;
+  }
+  if (t < FS_BG_WORDS) glob[t] = 0;
+}
+
+// Job of primegen workgroup b: last job with pg_first <= b.
+__device__ __forceinline__ uint32_t job_of_block(const fs_bjob* jobs, uint32_t nj, uint32_t b) {
+  uint32_t lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].pg_first <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_bprimegen(const fs_bjob* __restrict__ jobs, uint32_t nj, fs_cands c,
+                                                        uint32_t* jcnt, uint32_t* jflag) {
+  __shared__ uint64_t s_tab[FS_NMASK];
+  __shared__ uint64_t s_mask[FS_BLOCK];
+  __shared__ uint32_t s_excl[FS_BLOCK];
+  fs_sieve_table_init(s_tab);
+  const uint32_t j = job_of_block(jobs, nj, blockIdx.x);
+  const fs_bjob J = jobs[j];
+  if (!(J.flags & FS_BJ_NEW)) return;  // uniform per workgroup
+  fs_primegen_block(J.pmin | 1, J.pmax, blockIdx.x - J.pg_first, s_tab, s_mask, s_excl,
+                    [&](uint64_t N, uint64_t q, uint64_t r2) {
+                      const uint32_t idx = atomicAdd(&jcnt[j], 1u);
+                      if (idx < J.cand_cap) {
+                        const uint32_t g = J.cand_off + idx;
+                        c.p[g] = N;
+                        c.q[g] = q;
+                        c.r2[g] = r2;
+                        c.res[g] = 0;
+                        c.nnext[g] = J.nmin;
+                      } else {
+                        atomicOr(&jflag[j], FS_BJF_CANDOVF);
+                      }
+                    });
+}
+
+// Tile list for the jobs of `order` (cost order), covering n in [n0, n1) of
+// each job as given by chunk[] (n0 = chunk[2k], n1 = chunk[2k+1] for job
+// order[k]); pref: nj+1 words of scratch.
+__global__ __launch_bounds__(FS_BLOCK) void k_bmake_tiles(const fs_bjob* __restrict__ jobs,
+                                                          const uint32_t* __restrict__ order,
+                                                          const uint32_t* __restrict__ chunk, uint32_t nj,
+                                                          const uint32_t* jcnt, uint32_t tile, uint4* tiles,
+                                                          uint32_t tiles_cap, uint32_t* pref, uint32_t* glob) {
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < nj; ++k) {
+      const fs_bjob& J = jobs[order[k]];
+      const uint32_t live = min(jcnt[order[k]], J.cand_cap);
+      pref[k] = acc;
+      acc += (live + tile - 1) / tile;
+    }
+    pref[nj] = acc;
+    glob[FS_BG_NTILES] = min(acc, tiles_cap);
+    if (acc > tiles_cap) atomicOr(&glob[FS_BG_TILEOVF], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = 0; k < nj; ++k) {
+    const uint32_t j = order[k], b = pref[k], e = min(pref[k + 1], tiles_cap);
+    for (uint32_t t = b + threadIdx.x; t < e; t += FS_BLOCK)
+      tiles[t] = make_uint4(j, (t - b) * tile, chunk[2 * k], chunk[2 * k + 1]);
+  }
+}
+
+// Persistent sieve: each workgroup repeatedly takes the next tile; every queue
+// index is taken once, and a workgroup leaves when the index passes the tile
+// count, so all workgroups exit.
+__global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__ jobs,
+                                                     const uint4* __restrict__ tiles, fs_cands c,
+                                                     const uint64_t* __restrict__ prog,
+                                                     const double* __restrict__ ntab, fs_factor* fac, uint32_t fcap,
+                                                     const uint32_t* jcnt, uint32_t* jflag, uint64_t* jsum,
+                                                     uint32_t* glob, uint32_t tile) {
+  __shared__ uint32_t s_t;
+  const uint32_t ntiles = glob[FS_BG_NTILES];
+  for (;;) {
+    if (threadIdx.x == 0) s_t = atomicAdd(&glob[FS_BG_HEAD], 1u);
+    __syncthreads();
+    const uint32_t t = s_t;
+    __syncthreads();
+    if (t >= ntiles) break;
+    const uint4 T = tiles[t];
+    const uint32_t j = T.x, begin = T.y, n0 = T.z, n1 = T.w;
+    const fs_bjob J = jobs[j];
+    const uint32_t live = min(jcnt[j], J.cand_cap);
+    const bool first = (n0 == J.nmin), last = (n1 == J.nmax);
+    const uint64_t* jp = prog + J.prog_off;
+    const double* nt = ntab + n0 + 1;
+    for (uint32_t k = threadIdx.x; k < tile; k += FS_BLOCK) {
+      const uint32_t i = begin + k;
+      uint64_t v = 0;
+      if (i < live) {
+        const uint32_t g = J.cand_off + i;
+        const uint64_t p = c.p[g], q = c.q[g], r2 = c.r2[g];
+        auto rec = [&](int32_t nc) {
+          fs_append_factor(&glob[FS_BG_NF], &glob[FS_BG_FOVF], fac, fcap, p, nc, (int32_t)j);
+        };
+        uint64_t s;
+        if (!first && c.nnext[g] != n0) atomicOr(&jflag[j], FS_BJF_VALIDATE);
+        if (J.path == FS_PATH_FP64) {
+          s = first ? fs_cand_setup_fp64(p, q, r2, 0, (const double*)jp, 0, J.prog_terms) : c.res[g];
+          s = fs_cand_iter_fp64(p, s, n0, n1, nt, rec);
+        } else if (J.path == FS_PATH_HYBRID) {
+          s = first ? fs_cand_setup_hybrid(p, q, r2, 0, jp, J.prog_terms, 0, J.prog_terms) : c.res[g];
+          s = fs_cand_iter_hybrid(p, s, n0, n1, nt, rec);
+        } else {
+          s = first ? fs_cand_setup_mont(p, q, r2, 0, jp, J.prog_terms, 0, J.prog_terms) : c.res[g];
+          s = fs_cand_iter_mont(p, q, r2, s, n0, n1, rec);
+        }
+        if (last) {
+          bool ok;
+          v = fs_cand_fold(p, q, r2, s, J.nmax, ok);
+          if (!ok) atomicOr(&jflag[j], FS_BJF_VALIDATE);
+        } else {
+          c.res[g] = s;
+          c.nnext[g] = n1;
+        }
+      }
+      if (last) {
+        v = fs_wave_sum(v);
+        if ((threadIdx.x & 63) == 0 && v != 0) atomicAdd((unsigned long long*)&jsum[j], (unsigned long long)v);
+      }
+    }
+  }
+}
+
+__global__ void k_bfill_ntab(double* t, uint32_t count) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < count) t[j] = (double)j;
+}
+
+// ============================================================ launch helpers
+
+int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, uint32_t norder, uint32_t pg_blocks, uint32_t tile,
+                    uint32_t blocks, hipStream_t st) {
+  k_bclear<<<(nj + 255) / 256 + 1, 256, 0, st>>>(d.jobs, d.jcnt, d.jflag, d.jsum, nj, d.glob);
+  BCHK(hipGetLastError(), "batch clear");
+  if (pg_blocks) {
+    k_bprimegen<<<pg_blocks, FS_BLOCK, 0, st>>>(d.jobs, nj, d.c, d.jcnt, d.jflag);
+    BCHK(hipGetLastError(), "batch primegen");
+  }
+  k_bmake_tiles<<<1, FS_BLOCK, 0, st>>>(d.jobs, d.order, d.chunk, norder, d.jcnt, tile, d.tiles, d.tiles_cap,
+                                        d.pref, d.glob);
+  BCHK(hipGetLastError(), "batch tiles");
+  k_bsieve<<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.fac, d.fcap, d.jcnt, d.jflag,
+                                        d.jsum, d.glob, tile);
+  BCHK(hipGetLastError(), "batch sieve");
+  return FS_OK;
+}
+
+int fs_batch_fill_ntab(double* t, uint32_t count, hipStream_t st) {
+  k_bfill_ntab<<<(count + 255) / 256, 256, 0, st>>>(t, count);
+  BCHK(hipGetLastError(), "batch ntab");
+  return FS_OK;
+}

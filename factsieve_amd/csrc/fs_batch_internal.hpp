@@ -1,0 +1,86 @@
+// fs_batch_internal.hpp -- device records and launch helpers shared by the
+// batched form (fs_batch_host.cpp) and the serving loop (fs_server.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <vector>
+#include "../../include/factsieve.h"
+
+#ifndef FS_BLOCK
+#define FS_BLOCK 256
+#endif
+
+// One job of a batch as the kernels see it (64 bytes).
+struct fs_bjob {
+  uint64_t pmin, pmax;     // p in [pmin, pmax)
+  uint32_t nmin, nmax;     // n in [nmin, nmax)
+  uint32_t path;           // FS_PATH_FP64 / _HYBRID / _MONT
+  uint32_t prog_off;       // setup program offset in 8-byte words
+  uint32_t prog_terms;     // setup program term count
+  uint32_t cand_off;       // candidate region [cand_off, cand_off + cand_cap)
+  uint32_t cand_cap;
+  uint32_t pg_first;       // first candidate-generation workgroup of this job
+  uint32_t flags;          // FS_BJ_*
+  uint32_t pad;
+};
+#define FS_BJ_NEW 1u       // counters cleared and candidates generated in this launch
+
+// per-job flag bits
+#define FS_BJF_CANDOVF 1u
+#define FS_BJF_VALIDATE 2u
+
+// global words
+#define FS_BG_HEAD 0       // tile queue head
+#define FS_BG_NTILES 1     // tiles in the list
+#define FS_BG_NF 2         // factors appended
+#define FS_BG_FOVF 3       // factor buffer overflow
+#define FS_BG_TILEOVF 4    // tile list overflow
+#define FS_BG_WORDS 8
+
+struct fs_batch_dev {
+  fs_bjob* jobs = nullptr;
+  uint32_t* order = nullptr;   // job visit order for the tile list
+  uint32_t* chunk = nullptr;   // (n0, n1) per order entry
+  uint32_t* jcnt = nullptr;
+  uint32_t* jflag = nullptr;
+  uint64_t* jsum = nullptr;
+  uint32_t* glob = nullptr;
+  uint32_t* pref = nullptr;
+  uint4* tiles = nullptr;
+  uint32_t tiles_cap = 0;
+  fs_cands c{};
+  uint64_t* prog = nullptr;
+  double* ntab = nullptr;      // ntab[k] = k for k < ntab_len
+  uint32_t ntab_len = 0;
+  fs_factor* fac = nullptr;
+  uint32_t fcap = 0;
+};
+
+// Enqueue clear -> candidate generation (jobs flagged FS_BJ_NEW) -> tile list
+// (the norder jobs of d.order with their d.chunk n-ranges) -> persistent sieve.
+int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, uint32_t norder, uint32_t pg_blocks, uint32_t tile,
+                    uint32_t blocks, hipStream_t st);
+int fs_batch_fill_ntab(double* t, uint32_t count, hipStream_t st);
+extern "C" void fs_set_hip_error(const char* what, int err);
+
+// Grow-only device buffers of one batched-form client (the batch entry point
+// or a server) with its own stream.
+struct fs_batch_bufs {
+  fs_batch_dev d;
+  uint32_t jobs_cap = 0, cands_cap = 0, prog_cap = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int ensure(uint32_t nj, uint32_t ncands, uint32_t nprog, uint32_t ntiles, uint32_t nmax_max, uint32_t fcap);
+  void release();
+};
+uint32_t fs_device_cus();
+// Setup program words of (nmin, path), cached; *terms = term count.
+const std::vector<uint64_t>& fs_program_cached(uint32_t nmin, int path, uint32_t* terms);
+// Upload of a job table / tile order / chunks / programs and the launch.
+int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const std::vector<uint32_t>& order,
+                        const std::vector<uint32_t>& chunk, const std::vector<uint64_t>& words, uint32_t pg_blocks,
+                        uint32_t ncands, uint32_t tile, uint32_t blocks, uint32_t fcap, bool graph, float* dev_ms);
+// Read back per-job counters, flags, sums and the factor records.
+int fs_batch_read(fs_batch_bufs& E, uint32_t nj, std::vector<uint32_t>& cnt, std::vector<uint32_t>& flag,
+                  std::vector<uint64_t>& sum, std::vector<fs_factor>& fac, uint32_t* glob_out);
+uint32_t fs_pg_blocks(uint64_t pmin, uint64_t pmax);

@@ -16,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include "fs_arith.hpp"
+#include "fs_kern.hpp"
 #include "../../include/factsieve.h"
 
 // ===================================================================== errors
@@ -27,6 +28,7 @@ static void set_error(const char* what, hipError_t e) {
   g_last_error = buf;
 }
 extern "C" void fs_set_error_msg(const char* msg) { g_last_error = msg ? msg : ""; }
+extern "C" void fs_set_hip_error(const char* what, int err) { set_error(what, (hipError_t)err); }
 extern "C" const char* fs_last_error(void) { return g_last_error.c_str(); }
 
 #define HIPCHK(call, what)              \
@@ -42,7 +44,7 @@ static hipStream_t S(fs_stream s) { return (hipStream_t)s; }
 
 // ================================================================== constants
 
-#define FS_BLOCK 256
+
 #define FS_VERIFY_BLOCKS 10000u
 #define FS_VERIFY_L2 40u
 
@@ -52,25 +54,6 @@ static hipStream_t S(fs_stream s) { return (hipStream_t)s; }
 __device__ __forceinline__ uint32_t fs_live(const uint32_t* ctr, uint32_t cap) {
   const uint32_t n = ctr[FS_CTR_COUNT];
   return n < cap ? n : cap;
-}
-
-__device__ __forceinline__ void fs_record(uint32_t* ctr, fs_factor* fac, uint32_t cap, uint64_t p,
-                                          int32_t nc) {
-  const uint32_t idx = atomicAdd(&ctr[FS_CTR_NFACTORS], 1u);
-  if (idx < cap) {
-    fac[idx].p = p;
-    fac[idx].nc = nc;
-    fac[idx].pad = 0;
-  } else {
-    atomicOr(&ctr[FS_CTR_FACTOROVF], 1u);
-  }
-}
-
-// Wave64 sum of a 64-bit value (result valid in every lane).
-__device__ __forceinline__ uint64_t fs_wave_sum(uint64_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
 }
 
 // ================================================================== kernels
@@ -89,190 +72,61 @@ __global__ void k_set_flag(uint32_t* ctr, uint32_t idx) {
   if (threadIdx.x == 0) atomicOr(&ctr[idx], 1u);
 }
 
-// Segmented candidate generation over [low, high).  Thread t owns the 64 odd
-// numbers base + 2i + 128t (i < 64) as one bit mask: the masks of the 29 small
-// primes are OR-ed from an LDS table indexed by B mod q.  Each wave then deals
-// its survivors out round-robin (exclusive prefix of popcounts + select-bit) so
-// that all 64 lanes run the probable-prime test on consecutive survivors.
+// Segmented candidate generation over [low, high) (getsegprimes.cl:170-239);
+// the block routine is fs_primegen_block.
 __global__ __launch_bounds__(FS_BLOCK) void k_getsegprimes(uint64_t base, uint64_t high, fs_cands c,
                                                            uint32_t* ctr) {
   __shared__ uint64_t s_tab[FS_NMASK];
   __shared__ uint64_t s_mask[FS_BLOCK];
   __shared__ uint32_t s_excl[FS_BLOCK];
-  for (uint32_t k = threadIdx.x; k < FS_NMASK; k += FS_BLOCK) {
-    int j = 0;
-    while (j + 1 < FS_NSMALL && fs_small_off(j + 1) <= k) ++j;
-    s_tab[k] = fs_small_mask(kSmallQ[j], k - fs_small_off(j));
-  }
-  __syncthreads();
-
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wbase = threadIdx.x & ~63u;
-  const uint64_t t = (uint64_t)blockIdx.x * FS_BLOCK + threadIdx.x;
-  const uint64_t span = high > base ? high - base : 0;  // base is odd, <= high
-  uint64_t surv = 0;
-  uint64_t B = 0;
-  if (t * 128 < span) {
-    B = base + t * 128;
-    uint64_t comp = 0;
-#pragma unroll
-    for (int j = 0; j < FS_NSMALL; ++j) comp |= s_tab[fs_small_off(j) + (uint32_t)(B % kSmallQ[j])];
-    surv = ~comp;
-    const uint64_t left = (high - B + 1) >> 1;  // odd numbers B + 2i < high
-    if (left < 64) surv &= (((uint64_t)1 << left) - 1);
-    if (B <= 113) {  // the small primes themselves are never candidates
-      for (uint32_t i = 0; i < 64; ++i)
-        if (B + 2 * i <= 113) surv &= ~((uint64_t)1 << i);
+  fs_sieve_table_init(s_tab);
+  fs_primegen_block(base, high, blockIdx.x, s_tab, s_mask, s_excl, [&](uint64_t N, uint64_t q, uint64_t r2) {
+    const uint32_t idx = atomicAdd(&ctr[FS_CTR_COUNT], 1u);
+    if (idx < c.cap) {
+      c.p[idx] = N;
+      c.q[idx] = q;
+      c.r2[idx] = r2;
+      c.res[idx] = 0;
+      c.nnext[idx] = 0;
+    } else {
+      atomicOr(&ctr[FS_CTR_PRIMEOVF], 1u);
     }
-  }
-  s_mask[threadIdx.x] = surv;
-  // wave-inclusive scan of popcounts
-  const uint32_t cnt = __popcll(surv);
-  uint32_t incl = cnt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t v = __shfl_up(incl, o, 64);
-    if (lane >= (uint32_t)o) incl += v;
-  }
-  s_excl[threadIdx.x] = incl - cnt;
-  const uint32_t total = __shfl(incl, 63, 64);
-  __syncthreads();
-
-  const uint64_t tw = (uint64_t)blockIdx.x * FS_BLOCK + wbase;
-  for (uint32_t k = lane; k < total; k += 64) {
-    // owner lane: last L with excl[L] <= k
-    uint32_t lo = 0, hi = 63;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (s_excl[wbase + mid] <= k) lo = mid; else hi = mid - 1;
-    }
-    uint64_t m = s_mask[wbase + lo];
-    for (uint32_t j = k - s_excl[wbase + lo]; j > 0; --j) m &= m - 1;
-    const uint32_t bit = __builtin_ctzll(m);
-    const uint64_t N = base + (tw + lo) * 128 + 2 * (uint64_t)bit;
-    const uint64_t q = fs_invert(N);
-    const uint64_t one = (0 - N) % N;
-    if (fs_spsp2(N, q, one)) {
-      const uint64_t r2 = fs_r2_from_one(one, N, q);
-      const uint32_t idx = atomicAdd(&ctr[FS_CTR_COUNT], 1u);
-      if (idx < c.cap) {
-        c.p[idx] = N;
-        c.q[idx] = q;
-        c.r2[idx] = r2;
-        c.res[idx] = 0;
-        c.nnext[idx] = 0;
-      } else {
-        atomicOr(&ctr[FS_CTR_PRIMEOVF], 1u);
-      }
-    }
-  }
+  });
 }
 
 // ---- setup: res <- res * prod(program[start..end)), program word 0 = square
 
 __global__ __launch_bounds__(FS_BLOCK) void k_setup_fp64(fs_cands c, const uint32_t* ctr,
-                                                         const double* prog, uint32_t start,
+                                                         const double* __restrict__ prog, uint32_t start,
                                                          uint32_t end, uint32_t nmin) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
-  const uint64_t p = c.p[i];
-  const double pd = (double)p, pinv = 1.0 / pd;
-  double s = (start == 0) ? 1.0 : fs_canon_to_fp(c.res[i], p);
-  for (uint32_t k = start; k < end; ++k) {
-    const double m = prog[k];
-    if (m == 0.0) {
-      const uint64_t q = c.q[i], r2 = c.r2[i];
-      const uint64_t a = fs_fp_to_canon(s, p);
-      s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
-    } else {
-      s = fs_fp_mulmod(s, m, pd, pinv);
-    }
-  }
-  c.res[i] = fs_fp_to_canon(s, p);
+  c.res[i] = fs_cand_setup_fp64(c.p[i], c.q[i], c.r2[i], c.res[i], prog, start, end);
   c.nnext[i] = nmin;
 }
 
-// Hybrid path: the running product is kept as x * 2^-E mod p (lazily below
-// 2p) because each multiply is a REDC by 2^32; squarings are Montgomery
-// products by 2^64.  prog[terms] holds the total E, removed after the last term.
 __global__ __launch_bounds__(FS_BLOCK) void k_setup_hybrid(fs_cands c, const uint32_t* ctr,
-                                                           const uint64_t* prog, uint32_t terms,
+                                                           const uint64_t* __restrict__ prog, uint32_t terms,
                                                            uint32_t start, uint32_t end, uint32_t nmin) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
-  const uint64_t p = c.p[i], q = c.q[i];
-  const uint32_t pn32 = (uint32_t)(0 - q);
-  uint64_t s = (start == 0) ? 1 : c.res[i];
-  for (uint32_t k = start; k < end; ++k) {
-    const uint32_t m = (uint32_t)prog[k];
-    if (m == 0) {
-      const uint64_t a = s >= p ? s - p : s;
-      s = fs_mont_mul(a, a, p, q);
-    } else {
-      s = fs_redc32_mul(s, m, p, pn32);
-    }
-  }
-  s = s >= p ? s - p : s;
-  if (end == terms && prog[terms] != 0) {
-    const uint64_t one = fs_mont_mul(1, c.r2[i], p, q);
-    s = fs_mont_mul(s, fs_mont_pow2(prog[terms], p, q, one), p, q);
-  }
-  c.res[i] = s;
+  c.res[i] = fs_cand_setup_hybrid(c.p[i], c.q[i], c.r2[i], c.res[i], prog, terms, start, end);
   c.nnext[i] = nmin;
 }
 
-// Reference power-table evaluation (setup.cl:49-103) on the Montgomery path.
-// prog = [bases[0..cnt) | (exp | curbit<<32)[0..cnt)].
 __global__ __launch_bounds__(FS_BLOCK) void k_setup_mont(fs_cands c, const uint32_t* ctr,
-                                                         const uint64_t* prog, uint32_t cnt,
+                                                         const uint64_t* __restrict__ prog, uint32_t cnt,
                                                          uint32_t start, uint32_t end, uint32_t nmin) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
-  const uint64_t p = c.p[i], q = c.q[i], r2 = c.r2[i];
-  const uint64_t one = fs_mont_mul(1, r2, p, q);
-  uint64_t s6 = (start == 0) ? one : fs_mont_mul(c.res[i], r2, p, q);
-  uint32_t k = start;
-  if (k == 0 && end > 0) {  // base 2: square-and-double
-    const uint64_t w = prog[cnt];
-    const uint32_t e = (uint32_t)w;
-    uint32_t cb = (uint32_t)(w >> 32);
-    uint64_t a = fs_add_mod(one, one, p);
-    while (cb) {
-      a = fs_mont_mul(a, a, p, q);
-      if (e & cb) a = fs_add_mod(a, a, p);
-      cb >>= 1;
-    }
-    s6 = a;
-    k = 1;
-  }
-  for (; k < end; ++k) {
-    const uint64_t w = prog[cnt + k];
-    const uint32_t e = (uint32_t)w;
-    uint32_t cb = (uint32_t)(w >> 32);
-    const uint64_t b = fs_mont_mul(prog[k], r2, p, q);
-    uint64_t a = b;
-    if (e != 1) {
-      while (cb) {
-        a = fs_mont_mul(a, a, p, q);
-        if (e & cb) a = fs_mont_mul(a, b, p, q);
-        cb >>= 1;
-      }
-    }
-    s6 = fs_mont_mul(s6, a, p, q);
-  }
-  c.res[i] = fs_mont_mul(s6, 1, p, q);
+  c.res[i] = fs_cand_setup_mont(c.p[i], c.q[i], c.r2[i], c.res[i], prog, cnt, start, end);
   c.nnext[i] = nmin;
 }
 
 // ---- iterate: for n in [n0, n1): capture n! = +-1, then multiply by n+1
 
-// FP64 path.  The multipliers n+1 are read as doubles from nt (nt[j] =
-// n0 + 1 + j) with scalar loads, so a step costs six FP64 VALU operations plus
-// one compare; four steps share one branch on the OR of their +-1 tests.
-__device__ __forceinline__ void fs_fp_record(uint32_t* ctr, fs_factor* fac, uint32_t fcap, uint64_t p,
-                                             double v, uint32_t n) {
-  if (__builtin_fabs(v) == 1.0) fs_record(ctr, fac, fcap, p, v > 0.0 ? -(int32_t)n : (int32_t)n);
-}
+#define FS_REC(p_)                                                                                   \
+  [&](int32_t nc) { fs_append_factor(&ctr[FS_CTR_NFACTORS], &ctr[FS_CTR_FACTOROVF], fac, fcap, p_, nc, 0); }
 
 __global__ __launch_bounds__(FS_BLOCK) void k_iterate_fp64(fs_cands c, uint32_t* ctr, fs_factor* fac,
                                                            uint32_t fcap, uint32_t n0, uint32_t n1,
@@ -281,42 +135,8 @@ __global__ __launch_bounds__(FS_BLOCK) void k_iterate_fp64(fs_cands c, uint32_t*
   if (i >= fs_live(ctr, c.cap)) return;
   const uint64_t p = c.p[i];
   if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
-  const double pd = (double)p, pinv = 1.0 / pd;
-  double s = fs_canon_to_fp(c.res[i], p);
-  uint32_t n = n0;
-  const uint32_t n4 = n0 + ((n1 - n0) & ~3u);
-  // nt holds at least 4 entries past n1 - n0, so the next block's multipliers
-  // are loaded one iteration ahead
-  double m0 = nt[0], m1 = nt[1], m2 = nt[2], m3 = nt[3];
-  for (; n < n4; n += 4) {
-    const double* mn = nt + (n - n0) + 4;
-    const double x0 = mn[0], x1 = mn[1], x2 = mn[2], x3 = mn[3];
-    const double a = s;
-    const double b = fs_fp_mulmod(a, m0, pd, pinv);
-    const double d = fs_fp_mulmod(b, m1, pd, pinv);
-    const double e = fs_fp_mulmod(d, m2, pd, pinv);
-    s = fs_fp_mulmod(e, m3, pd, pinv);
-    m0 = x0, m1 = x1, m2 = x2, m3 = x3;
-    const bool hit = (__builtin_fabs(a) == 1.0) | (__builtin_fabs(b) == 1.0) | (__builtin_fabs(d) == 1.0) |
-                     (__builtin_fabs(e) == 1.0);
-    if (hit) {
-      fs_fp_record(ctr, fac, fcap, p, a, n);
-      fs_fp_record(ctr, fac, fcap, p, b, n + 1);
-      fs_fp_record(ctr, fac, fcap, p, d, n + 2);
-      fs_fp_record(ctr, fac, fcap, p, e, n + 3);
-    }
-  }
-  for (; n < n1; ++n) {
-    fs_fp_record(ctr, fac, fcap, p, s, n);
-    s = fs_fp_mulmod(s, nt[n - n0], pd, pinv);
-  }
-  c.res[i] = fs_fp_to_canon(s, p);
+  c.res[i] = fs_cand_iter_fp64(p, c.res[i], n0, n1, nt, FS_REC(p));
   c.nnext[i] = n1;
-}
-
-__global__ void k_fill_ntab(double* t, uint32_t first, uint32_t count) {
-  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < count) t[j] = (double)(first + j);
 }
 
 __global__ __launch_bounds__(FS_BLOCK) void k_iterate_hybrid(fs_cands c, uint32_t* ctr, fs_factor* fac,
@@ -326,39 +146,23 @@ __global__ __launch_bounds__(FS_BLOCK) void k_iterate_hybrid(fs_cands c, uint32_
   if (i >= fs_live(ctr, c.cap)) return;
   const uint64_t p = c.p[i];
   if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
-  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
-  const uint64_t pp1 = p + 1, pm1 = p - 1, p2m1 = 2 * p - 1;
-  uint64_t s = c.res[i];
-  for (uint32_t n = n0; n < n1; ++n) {
-    const bool plus1 = (s == 1) | (s == pp1);
-    const bool minus1 = (s == pm1) | (s == p2m1);
-    if (plus1 | minus1) fs_record(ctr, fac, fcap, p, plus1 ? -(int32_t)n : (int32_t)n);
-    uint32_t m = n + 1;
-    asm volatile("" : "+s"(m));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
-    s = fs_hy_mulmod(s, m, nt[n - n0], p, pinv, pinv32);
-  }
-  c.res[i] = s >= p ? s - p : s;
+  c.res[i] = fs_cand_iter_hybrid(p, c.res[i], n0, n1, nt, FS_REC(p));
   c.nnext[i] = n1;
 }
 
-// Reference formulation (iterate.cl:54-80): Montgomery residue s6 and
-// Montgomery counter s7 advanced by one each step.
 __global__ __launch_bounds__(FS_BLOCK) void k_iterate_mont(fs_cands c, uint32_t* ctr, fs_factor* fac,
                                                            uint32_t fcap, uint32_t n0, uint32_t n1) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
-  const uint64_t p = c.p[i], q = c.q[i], r2 = c.r2[i];
+  const uint64_t p = c.p[i];
   if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
-  const uint64_t one = fs_mont_mul(1, r2, p, q), nmo = p - one;
-  uint64_t s6 = fs_mont_mul(c.res[i], r2, p, q);
-  uint64_t s7 = fs_mont_mul(n0 < p ? (uint64_t)n0 : (uint64_t)n0 % p, r2, p, q);
-  for (uint32_t n = n0; n < n1; ++n) {
-    s7 = fs_add_mod(s7, one, p);
-    if (s6 == one || s6 == nmo) fs_record(ctr, fac, fcap, p, (s6 == one) ? -(int32_t)n : (int32_t)n);
-    s6 = fs_mont_mul(s6, s7, p, q);
-  }
-  c.res[i] = fs_mont_mul(s6, 1, p, q);
+  c.res[i] = fs_cand_iter_mont(p, c.q[i], c.r2[i], c.res[i], n0, n1, FS_REC(p));
   c.nnext[i] = n1;
+}
+
+__global__ void k_fill_ntab(double* t, uint32_t first, uint32_t count) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < count) t[j] = (double)(first + j);
 }
 
 // ---- check: checksum fold + count + validation (check.cl:50-109)
@@ -369,14 +173,9 @@ __global__ __launch_bounds__(FS_BLOCK) void k_check(fs_cands c, uint32_t* ctr, u
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   uint64_t v = 0;
   if (i < live) {
-    const uint64_t p = c.p[i], q = c.q[i], r2 = c.r2[i], s = c.res[i];
-    const uint64_t s6 = fs_mont_mul(s, r2, p, q);  // mont(nmax! mod p)
-    const uint64_t nm = (p > nmax) ? (uint64_t)nmax : (uint64_t)nmax % p;
-    const uint64_t s7 = fs_mont_mul(nm, r2, p, q);  // mont(nmax mod p)
-    v = s6 + s7;
-    const bool ok = (s < p) && (c.nnext[i] == nmax) && (fs_mont_mul(s6, 1, p, q) == s) &&
-                    (fs_mont_mul(s7, 1, p, q) == nm);
-    if (!ok) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
+    bool ok;
+    v = fs_cand_fold(c.p[i], c.q[i], c.r2[i], c.res[i], nmax, ok);
+    if (!ok || c.nnext[i] != nmax) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
   }
   v = fs_wave_sum(v);
   if ((threadIdx.x & 63) == 0 && v != 0) atomicAdd((unsigned long long*)&sum[1], (unsigned long long)v);

@@ -439,22 +439,22 @@ size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbos
   return kept;
 }
 
-namespace {
-
-double now_s() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
 // Upper estimate of candidates in [lo, lo + len): the probable primes have
 // density ~1/ln p, bounded here by 1/(ln lo - 1.5) plus slack, and never more
 // than the survivors of the 113-sieve (density 0.1188).
-uint64_t cand_capacity(uint64_t lo, uint64_t len) {
+uint64_t fs_cand_capacity(uint64_t lo, uint64_t len) {
   const double l = (double)(lo < 3 ? 3 : lo);
   const double sieve_bound = (double)len * 0.1190 + 4096.0;
   const double dens = (log(l) > 4.0) ? 1.0 / (log(l) - 1.5) : 1.0;
   const double est = (double)len * dens * 1.15 + 65536.0;
   const double c = std::min(sieve_bound, est);
   return (uint64_t)std::min(c, 4.0e9);
+}
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 struct DevBufs {
@@ -600,7 +600,7 @@ static int finalize_file(const WorkStatus& st, const fs_options& o) {
   return FS_OK;
 }
 
-static int validate_ranges(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax) {
+int fs_validate_ranges(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax) {
   // setupSearch (cl_sieve.cpp:557-589)
   if (pmin == 0 || pmax == 0) {
     fs_set_error_msg("-p and -P arguments are required");
@@ -652,7 +652,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   fs_result r;
   memset(&r, 0, sizeof r);
   if (nfactors) *nfactors = 0;
-  int rc = validate_ranges(pmin, pmax, nmin, nmax);
+  int rc = fs_validate_ranges(pmin, pmax, nmin, nmax);
   if (rc) return rc;
   const bool verbose = !o.quiet;
   const int path = (o.path == FS_PATH_AUTO) ? fs_select_path(pmax) : o.path;
@@ -762,7 +762,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     uint64_t stop = st.p + len;
     if (stop > st.pmax || stop < st.p) stop = st.pmax;
     len = stop - st.p;
-    const uint64_t need = cand_capacity(st.p, len);
+    const uint64_t need = fs_cand_capacity(st.p, len);
     if (need > d.c.cap) {
       fs_cands_free(&d.c);
       if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>(need + need / 4, 0xFFFFFF00ull)))) return rc;

@@ -18,3 +18,7 @@ void fs_sort_factors(fs_factor* f, size_t n);
 int64_t fs_verify_all(const fs_factor* f, size_t n, int threads);
 size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose);
 uint32_t fs_auto_segment_candidates();
+// Range checks of setupSearch (cl_sieve.cpp:557-589); FS_OK or FS_E_ARG.
+int fs_validate_ranges(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax);
+// Upper estimate of the candidates in [lo, lo + len).
+uint64_t fs_cand_capacity(uint64_t lo, uint64_t len);

@@ -1,0 +1,278 @@
+// fs_kern.hpp -- per-candidate device routines shared by the single-job kernels
+// (fs_device.hip) and the batched / serving kernels (fs_batch.hip).
+//
+// Each routine works on one candidate modulus p held in registers:
+//   setup:   residue <- residue * (setup program terms [start, end))
+//   iterate: for n in [n0, n1): report n! = +-1, then residue <- residue * (n+1)
+//   fold:    mont(n! mod p) + mont(nmax mod p) for the checksum, plus validation
+// Residues enter and leave in canonical form [0, p).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "fs_arith.hpp"
+#include "../../include/factsieve.h"
+
+// Bounded factor append; `tag` goes to the record's pad word (job id in the
+// batched form, 0 otherwise).
+__device__ __forceinline__ void fs_append_factor(uint32_t* nf, uint32_t* ovf, fs_factor* fac, uint32_t cap,
+                                                 uint64_t p, int32_t nc, int32_t tag) {
+  const uint32_t idx = atomicAdd(nf, 1u);
+  if (idx < cap) {
+    fac[idx].p = p;
+    fac[idx].nc = nc;
+    fac[idx].pad = tag;
+  } else {
+    atomicOr(ovf, 1u);
+  }
+}
+
+// Wave64 sum of a 64-bit value (result valid in every lane).
+__device__ __forceinline__ uint64_t fs_wave_sum(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ------------------------------------------------------------- candidates
+
+#define FS_BLOCK 256
+
+// LDS copy of the small-prime masks (1591 entries of 64 bits).
+__device__ __forceinline__ void fs_sieve_table_init(uint64_t* s_tab) {
+  for (uint32_t k = threadIdx.x; k < FS_NMASK; k += FS_BLOCK) {
+    int j = 0;
+    while (j + 1 < FS_NSMALL && fs_small_off(j + 1) <= k) ++j;
+    s_tab[k] = fs_small_mask(kSmallQ[j], k - fs_small_off(j));
+  }
+  __syncthreads();
+}
+
+// One 256-thread block of candidate generation over [base, high), base odd:
+// thread t of block `blk` owns the 64 odd numbers base + 128*(blk*256 + t) + 2i
+// as one bit mask, built by OR-ing the masks of the 29 small primes indexed by
+// B mod q (the reference's mod-30 wheel plus bit tables, getsegprimes.cl:112-197).
+// Each wave then deals its survivors out round-robin (exclusive prefix of
+// popcounts + select-bit) so all 64 lanes run the base-2 strong probable-prime
+// test (getsegprimes.cl:199-229) on consecutive survivors; passers are handed
+// to app(N, q = N^-1 mod 2^64, r2 = 2^128 mod N).  All threads must call it.
+template <class Append>
+__device__ __forceinline__ void fs_primegen_block(uint64_t base, uint64_t high, uint64_t blk, const uint64_t* s_tab,
+                                                  uint64_t* s_mask, uint32_t* s_excl, Append app) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wbase = threadIdx.x & ~63u;
+  const uint64_t t = blk * FS_BLOCK + threadIdx.x;
+  const uint64_t span = high > base ? high - base : 0;
+  uint64_t surv = 0;
+  if (t * 128 < span) {
+    const uint64_t B = base + t * 128;
+    uint64_t comp = 0;
+#pragma unroll
+    for (int j = 0; j < FS_NSMALL; ++j) comp |= s_tab[fs_small_off(j) + (uint32_t)(B % kSmallQ[j])];
+    surv = ~comp;
+    const uint64_t left = (high - B + 1) >> 1;  // odd numbers B + 2i < high
+    if (left < 64) surv &= (((uint64_t)1 << left) - 1);
+    if (B <= 113) {  // the small primes themselves are never candidates
+      for (uint32_t i = 0; i < 64; ++i)
+        if (B + 2 * i <= 113) surv &= ~((uint64_t)1 << i);
+    }
+  }
+  s_mask[threadIdx.x] = surv;
+  const uint32_t cnt = __popcll(surv);
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o, 64);
+    if (lane >= (uint32_t)o) incl += v;
+  }
+  s_excl[threadIdx.x] = incl - cnt;
+  const uint32_t total = __shfl(incl, 63, 64);
+  __syncthreads();
+  const uint64_t tw = blk * FS_BLOCK + wbase;
+  for (uint32_t k = lane; k < total; k += 64) {
+    uint32_t lo = 0, hi = 63;  // owner lane: last L with excl[L] <= k
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (s_excl[wbase + mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    uint64_t m = s_mask[wbase + lo];
+    for (uint32_t j = k - s_excl[wbase + lo]; j > 0; --j) m &= m - 1;
+    const uint32_t bit = __builtin_ctzll(m);
+    const uint64_t N = base + (tw + lo) * 128 + 2 * (uint64_t)bit;
+    const uint64_t q = fs_invert(N);
+    const uint64_t one = (0 - N) % N;
+    if (fs_spsp2(N, q, one)) app(N, q, fs_r2_from_one(one, N, q));
+  }
+}
+
+// ------------------------------------------------------------------ setup
+
+// FP64 program (p < 2^52): multipliers < 2^48 as doubles, 0.0 = square.
+__device__ __forceinline__ uint64_t fs_cand_setup_fp64(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
+                                                       const double* __restrict__ prog, uint32_t start,
+                                                       uint32_t end) {
+  const double pd = (double)p, pinv = 1.0 / pd;
+  double s = (start == 0) ? 1.0 : fs_canon_to_fp(s_in, p);
+  for (uint32_t k = start; k < end; ++k) {
+    const double m = prog[k];
+    if (m == 0.0) {
+      const uint64_t a = fs_fp_to_canon(s, p);
+      s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+    } else {
+      s = fs_fp_mulmod(s, m, pd, pinv);
+    }
+  }
+  return fs_fp_to_canon(s, p);
+}
+
+// Hybrid program (p < 2^63): multipliers < 2^31, 0 = square.  The running
+// product is x * 2^-E mod p (below 2p) because each multiply is a REDC by 2^32
+// and each square a Montgomery product by 2^64; prog[terms] holds the total E,
+// removed after the last term.
+__device__ __forceinline__ uint64_t fs_cand_setup_hybrid(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
+                                                         const uint64_t* __restrict__ prog, uint32_t terms,
+                                                         uint32_t start, uint32_t end) {
+  const uint32_t pn32 = (uint32_t)(0 - q);
+  uint64_t s = (start == 0) ? 1 : s_in;
+  for (uint32_t k = start; k < end; ++k) {
+    const uint32_t m = (uint32_t)prog[k];
+    if (m == 0) {
+      const uint64_t a = s >= p ? s - p : s;
+      s = fs_mont_mul(a, a, p, q);
+    } else {
+      s = fs_redc32_mul(s, m, p, pn32);
+    }
+  }
+  s = s >= p ? s - p : s;
+  if (end == terms && prog[terms] != 0) {
+    const uint64_t one = fs_mont_mul(1, r2, p, q);
+    s = fs_mont_mul(s, fs_mont_pow2(prog[terms], p, q, one), p, q);
+  }
+  return s;
+}
+
+// Reference power-table evaluation (setup.cl:49-103) on the Montgomery path:
+// prog = [bases[0..cnt) | (exp | curbit << 32)[0..cnt)], entry 0 = the prime 2.
+__device__ __forceinline__ uint64_t fs_cand_setup_mont(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
+                                                       const uint64_t* __restrict__ prog, uint32_t cnt,
+                                                       uint32_t start, uint32_t end) {
+  const uint64_t one = fs_mont_mul(1, r2, p, q);
+  uint64_t s6 = (start == 0) ? one : fs_mont_mul(s_in, r2, p, q);
+  uint32_t k = start;
+  if (k == 0 && end > 0) {  // base 2: square-and-double
+    const uint64_t w = prog[cnt];
+    const uint32_t e = (uint32_t)w;
+    uint32_t cb = (uint32_t)(w >> 32);
+    uint64_t a = fs_add_mod(one, one, p);
+    while (cb) {
+      a = fs_mont_mul(a, a, p, q);
+      if (e & cb) a = fs_add_mod(a, a, p);
+      cb >>= 1;
+    }
+    s6 = a;
+    k = 1;
+  }
+  for (; k < end; ++k) {
+    const uint64_t w = prog[cnt + k];
+    const uint32_t e = (uint32_t)w;
+    uint32_t cb = (uint32_t)(w >> 32);
+    const uint64_t b = fs_mont_mul(prog[k], r2, p, q);
+    uint64_t a = b;
+    if (e != 1) {
+      while (cb) {
+        a = fs_mont_mul(a, a, p, q);
+        if (e & cb) a = fs_mont_mul(a, b, p, q);
+        cb >>= 1;
+      }
+    }
+    s6 = fs_mont_mul(s6, a, p, q);
+  }
+  return fs_mont_mul(s6, 1, p, q);
+}
+
+// ---------------------------------------------------------------- iterate
+
+// FP64 path.  nt[j] = (double)(n0 + 1 + j), read with scalar loads, with at
+// least 4 entries past n1 - n0; six FP64 operations and one compare per step,
+// four steps share one branch on the OR of their +-1 tests.
+template <class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
+                                                      const double* __restrict__ nt, Rec rec) {
+  const double pd = (double)p, pinv = 1.0 / pd;
+  double s = fs_canon_to_fp(s_in, p);
+  uint32_t n = n0;
+  const uint32_t n4 = n0 + ((n1 - n0) & ~3u);
+  double m0 = nt[0], m1 = nt[1], m2 = nt[2], m3 = nt[3];
+  for (; n < n4; n += 4) {
+    const double* mn = nt + (n - n0) + 4;  // next block, loaded one iteration ahead
+    const double x0 = mn[0], x1 = mn[1], x2 = mn[2], x3 = mn[3];
+    const double a = s;
+    const double b = fs_fp_mulmod(a, m0, pd, pinv);
+    const double d = fs_fp_mulmod(b, m1, pd, pinv);
+    const double e = fs_fp_mulmod(d, m2, pd, pinv);
+    s = fs_fp_mulmod(e, m3, pd, pinv);
+    m0 = x0, m1 = x1, m2 = x2, m3 = x3;
+    const bool hit = (__builtin_fabs(a) == 1.0) | (__builtin_fabs(b) == 1.0) | (__builtin_fabs(d) == 1.0) |
+                     (__builtin_fabs(e) == 1.0);
+    if (hit) {
+      if (__builtin_fabs(a) == 1.0) rec(a > 0.0 ? -(int32_t)n : (int32_t)n);
+      if (__builtin_fabs(b) == 1.0) rec(b > 0.0 ? -(int32_t)(n + 1) : (int32_t)(n + 1));
+      if (__builtin_fabs(d) == 1.0) rec(d > 0.0 ? -(int32_t)(n + 2) : (int32_t)(n + 2));
+      if (__builtin_fabs(e) == 1.0) rec(e > 0.0 ? -(int32_t)(n + 3) : (int32_t)(n + 3));
+    }
+  }
+  for (; n < n1; ++n) {
+    if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
+    s = fs_fp_mulmod(s, nt[n - n0], pd, pinv);
+  }
+  return fs_fp_to_canon(s, p);
+}
+
+// Hybrid path (p < 2^63): lazily reduced s in [0, 2p), 64x32 integer product
+// and FP64 quotient estimate; nt as for the FP64 path.
+template <class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
+                                                        const double* __restrict__ nt, Rec rec) {
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+  const uint64_t pp1 = p + 1, pm1 = p - 1, p2m1 = 2 * p - 1;
+  uint64_t s = s_in;
+  for (uint32_t n = n0; n < n1; ++n) {
+    const bool plus1 = (s == 1) | (s == pp1);
+    const bool minus1 = (s == pm1) | (s == p2m1);
+    if (plus1 | minus1) rec(plus1 ? -(int32_t)n : (int32_t)n);
+    uint32_t m = n + 1;
+    asm volatile("" : "+s"(m));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
+    s = fs_hy_mulmod(s, m, nt[n - n0], p, pinv, pinv32);
+  }
+  return s >= p ? s - p : s;
+}
+
+// Reference formulation (iterate.cl:54-80): Montgomery residue s6 and
+// Montgomery counter s7 advanced by one each step.  Any odd p < 2^64.
+template <class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_mont(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
+                                                      uint32_t n0, uint32_t n1, Rec rec) {
+  const uint64_t one = fs_mont_mul(1, r2, p, q), nmo = p - one;
+  uint64_t s6 = fs_mont_mul(s_in, r2, p, q);
+  uint64_t s7 = fs_mont_mul(n0 < p ? (uint64_t)n0 : (uint64_t)n0 % p, r2, p, q);
+  for (uint32_t n = n0; n < n1; ++n) {
+    s7 = fs_add_mod(s7, one, p);
+    if (s6 == one || s6 == nmo) rec((s6 == one) ? -(int32_t)n : (int32_t)n);
+    s6 = fs_mont_mul(s6, s7, p, q);
+  }
+  return fs_mont_mul(s6, 1, p, q);
+}
+
+// ------------------------------------------------------------------- fold
+
+// mont(s) + mont(nmax mod p) for the checksum (check.cl:60-64); ok is false if
+// the residue is not canonical or the Montgomery round trip fails
+// (check.cl:66-78 validates the same way with REDC(s7) == nmax).
+__device__ __forceinline__ uint64_t fs_cand_fold(uint64_t p, uint64_t q, uint64_t r2, uint64_t s, uint32_t nmax,
+                                                 bool& ok) {
+  const uint64_t s6 = fs_mont_mul(s, r2, p, q);
+  const uint64_t nm = (p > nmax) ? (uint64_t)nmax : (uint64_t)nmax % p;
+  const uint64_t s7 = fs_mont_mul(nm, r2, p, q);
+  ok = (s < p) && (fs_mont_mul(s6, 1, p, q) == s) && (fs_mont_mul(s7, 1, p, q) == nm);
+  return s6 + s7;
+}

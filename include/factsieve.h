@@ -231,6 +231,67 @@ int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax, const f
  * number of passing cases (5 = all). */
 int fs_self_test(const fs_options* opt, int* passed_mask);
 
+/* ---------------------------------------- batched persistent form (many jobs) */
+typedef struct {
+  uint64_t pmin, pmax;
+  uint32_t nmin, nmax;
+} fs_job;
+
+typedef struct {
+  uint64_t factorcount, primecount, checksum, raw_factors, steps;
+  int status;               /* FS_OK or error for this job */
+  int path;
+} fs_job_result;
+
+typedef struct {
+  uint32_t tile_candidates;   /* candidates per tile, a multiple of 256 (default 256) */
+  uint32_t persistent_blocks; /* resident workgroups of the persistent sieve kernel (0 = 8 per CU) */
+  uint32_t use_graph;         /* 1: the launches of a batch run as one captured hipGraph */
+  uint32_t lpt_order;         /* 1: tiles queued longest job first */
+  int verify_threads;         /* CPU threads verifying factors */
+} fs_batch_options;
+
+void fs_default_batch_options(fs_batch_options* o);
+/* Many sieve jobs (no reference counterpart: the reference runs one job per
+ * process, cl_sieve.cpp:918).  One candidate-generation launch for all jobs,
+ * then one persistent launch whose workgroups pull (job, 256-candidate) tiles
+ * from a device queue and run setup + iterate + checksum fold per candidate.
+ * Results per job are bit-identical to fs_sieve on the job alone.  factors:
+ * optional concatenated kept factors (job order, each sorted); per_job_nf:
+ * kept count per job; seconds: wall time of the call. */
+int fs_batch(const fs_job* jobs, uint32_t njobs, const fs_batch_options* opt, fs_job_result* out,
+             fs_factor* factors, uint64_t cap, uint64_t* per_job_nf, double* seconds);
+/* Time the launch shapes (tile size, persistent workgroups, queue order) on a
+ * 1/8 p-range sample of the jobs and write the fastest into *opt. */
+int fs_batch_autotune(const fs_job* jobs, uint32_t njobs, fs_batch_options* opt);
+
+/* ------------------------------------------------------ serving loop */
+/* Continuous batching of workunits (no reference counterpart: BOINC hands one
+ * workunit to one process).  Admission by device memory; residues stay in HBM
+ * between quanta; each result equals fs_sieve on the workunit alone. */
+typedef void* fs_server;
+typedef struct {
+  uint64_t mem_budget;      /* device bytes the server may hold for admitted work (0 = 50% free) */
+  uint32_t max_slots;       /* concurrently resident workunits */
+  uint32_t n_chunk;         /* n values per scheduling quantum */
+  int verify_threads;
+} fs_server_options;
+
+void fs_default_server_options(fs_server_options* o);
+int fs_server_create(const fs_server_options* o, fs_server* srv);
+/* Submit a workunit; returns its id (>= 0) or FS_E_BUSY/FS_E_ARG. Never blocks. */
+int64_t fs_server_submit(fs_server srv, const fs_job* job);
+/* Advance the device work by one scheduling quantum (admit queued work when
+ * memory and slots free up, advance every active workunit by n_chunk n-values,
+ * harvest finished workunits).  Returns the number finished (>= 0) or an error. */
+int fs_server_step(fs_server srv);
+/* Fetch a finished workunit's result (1 = done and copied, 0 = not yet). */
+int fs_server_poll(fs_server srv, int64_t id, fs_job_result* out, fs_factor* factors, uint64_t cap,
+                   uint64_t* nfactors);
+int fs_server_stats(fs_server srv, uint32_t* queued, uint32_t* active, uint32_t* done,
+                    uint64_t* bytes_in_use);
+int fs_server_destroy(fs_server srv);
+
 #ifdef __cplusplus
 }
 #endif

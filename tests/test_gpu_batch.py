@@ -1,0 +1,109 @@
+"""GPU parity of the batched persistent form and the serving loop: every job's
+factors, candidate count and checksum must equal the plain single-job sieve
+(and the oracle on the small jobs), whatever the mix, tile shape, queue order,
+graph capture or admission schedule."""
+import pytest
+
+import oracle_lib as ol
+from factsieve_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def small_mix():
+    jobs = []
+    for pmin, pmax, nmin, nmax in workloads.mixed_jobs(20, seed=7):
+        jobs.append((pmin, pmin + max(1, (pmax - pmin) // 256), nmin, nmin + max(1, (nmax - nmin) // 16)))
+    jobs += [
+        (10 ** 8, 10 ** 8 + 300000, 1000, 3000),        # FP64, factors present
+        (127, 20000, 127, 30000),                       # p <= nmax: many factors
+        (10 ** 9, 10 ** 9 + 100000, 5000, 5000),        # empty n-range
+        (2 ** 63 + 1000, 2 ** 63 + 200000, 2000, 2600), # Montgomery path
+        (2 ** 64 - 100000, 2 ** 64 - 1, 500, 900),      # top of the range
+        (2 ** 52 - 50000, 2 ** 52 + 50000, 3000, 3300), # hybrid across 2^52
+    ]
+    return jobs
+
+
+def plain(gpu, job):
+    r = gpu.sieve(*job, verify_threads=16)
+    return (r.factorcount, r.primecount, r.checksum, r.factors)
+
+
+def outcome(o):
+    return (o.factorcount, o.primecount, o.checksum, o.factors)
+
+
+def test_batch_matches_plain_on_mixed_jobs(gpu):
+    jobs = small_mix()
+    res, sec = gpu.batch(jobs)
+    assert len(res) == len(jobs)
+    for job, o in zip(jobs, res):
+        print("job", job, flush=True)
+        assert o.status == 0
+        assert outcome(o) == plain(gpu, job)
+
+
+def test_batch_small_jobs_match_oracle(gpu):
+    jobs = small_mix()[-6:-1]
+    res, _ = gpu.batch(jobs)
+    for job, o in zip(jobs, res):
+        want = ol.sieve(*job, threads=8)
+        assert (o.factorcount, o.primecount, o.checksum) == (want["factorcount"], want["primecount"],
+                                                             want["checksum"])
+        assert o.factors == want["factors"]
+
+
+@pytest.mark.parametrize("opts", [dict(tile_candidates=512), dict(persistent_blocks=64), dict(use_graph=0),
+                                  dict(lpt_order=0)])
+def test_batch_shape_invariance(gpu, opts):
+    jobs = small_mix()
+    base, _ = gpu.batch(jobs)
+    other, _ = gpu.batch(jobs, **opts)
+    assert [outcome(o) for o in base] == [outcome(o) for o in other]
+
+
+def test_batch_invalid_job_is_isolated(gpu):
+    jobs = [(10 ** 8, 10 ** 8 + 100000, 1000, 2000), (1000, 500, 100, 200), (10 ** 8, 10 ** 8 + 100000, 1000, 2000)]
+    res, _ = gpu.batch(jobs)
+    assert res[1].status == gpu.FS_E_ARG
+    assert res[0].status == res[2].status == 0 and outcome(res[0]) == outcome(res[2]) == plain(gpu, jobs[0])
+
+
+def test_batch_autotune_returns_valid_shape(gpu):
+    jobs = small_mix()[:8]
+    o = gpu.batch_autotune(jobs)
+    assert o["tile_candidates"] in (256, 512) and o["persistent_blocks"] > 0
+    res, _ = gpu.batch(jobs, **o)
+    assert all(r.status == 0 for r in res)
+
+
+@pytest.mark.parametrize("budget,chunk", [(0, 700), (64 << 20, 5000)])
+def test_server_results_equal_alone(gpu, budget, chunk):
+    """Workunits arriving while others run are admitted as slots/memory free
+    up; each result equals the same workunit run alone."""
+    jobs = small_mix()
+    srv = gpu.Server(mem_budget=budget, n_chunk=chunk)
+    try:
+        ids = [srv.submit(j) for j in jobs[:10]]
+        srv.step()
+        srv.step()
+        ids += [srv.submit(j) for j in jobs[10:]]
+        got = {}
+        for _ in range(100000):
+            srv.step()
+            for i in ids:
+                if i not in got:
+                    r = srv.poll(i)
+                    if r is not None:
+                        got[i] = r
+            if len(got) == len(ids):
+                break
+        assert len(got) == len(ids)
+        st = srv.stats()
+        assert st["active"] == 0 and st["queued"] == 0
+        for i, job in zip(ids, jobs):
+            assert got[i].status == 0
+            assert outcome(got[i]) == plain(gpu, job), job
+    finally:
+        srv.close()
