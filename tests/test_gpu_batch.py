@@ -90,7 +90,7 @@ def test_server_results_equal_alone(gpu, budget, chunk):
         srv.step()
         ids += [srv.submit(j) for j in jobs[10:]]
         got = {}
-        for _ in range(100000):
+        for _ in range(20000):
             srv.step()
             for i in ids:
                 if i not in got:
