@@ -44,9 +44,10 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine
 OPS_PER_STEP = {1: 7, 2: 14, 3: 45}             # VALU lane-ops per p*n step by path
 INT32_MUL_PER_STEP = 11                         # 64-bit Montgomery step: 4 + 3 + 4 partial products
 INT32_MUL_PEAK = VALU_PEAK_TOPS * 1e12          # full-rate v_mad_u64_u32 / v_mul_*_u32 (measured)
-# HBM bytes per candidate per iterate launch, from rocprofv3 --pmc FETCH_SIZE /
-# WRITE_SIZE on the c2 workload, scaled by the copy-kernel calibration factor.
-TRAFFIC_BYTES_PER_CAND = None
+# HBM bytes per candidate per k_iterate_fp64 launch, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on the c2 workload (23.2 B read + 12.0 B written),
+# each scaled by the counter's reading for a 1 GiB device copy (x2.0 fetch, x1.0 write).
+TRAFFIC_BYTES_PER_CAND = {1: 35.2}
 
 
 def cpu_baseline(pmin, nmin, nmax, target_s=15.0):
@@ -149,7 +150,20 @@ def main():
     ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and serving legs")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run one device-to-device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import factsieve_amd as fs
+        fs.init(0)
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        fs.check(fs.lib().fs_malloc(ctypes.byref(a), args.copy_calibration))
+        fs.check(fs.lib().fs_malloc(ctypes.byref(b), args.copy_calibration))
+        fs.check(fs.lib().fs_memset(a, 1, args.copy_calibration, None))
+        fs.check(fs.lib().fs_memcpy_dtod(b, a, args.copy_calibration, None))
+        fs.check(fs.lib().fs_stream_sync(None))
+        print(json.dumps({"copy_bytes": args.copy_calibration}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the sieve shards over p across GPUs with a host-side checksum sum, not run here)",
@@ -179,8 +193,8 @@ def main():
     ops = OPS_PER_STEP[ref.path]
     achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
     traffic = None
-    if TRAFFIC_BYTES_PER_CAND is not None:
-        traffic = TRAFFIC_BYTES_PER_CAND * ref.primecount / max(ref.iterate_launches, 1)
+    if ref.path in TRAFFIC_BYTES_PER_CAND:
+        traffic = TRAFFIC_BYTES_PER_CAND[ref.path] * ref.primecount / max(ref.iterate_launches, 1)
     line = {
         "metric": "p*n residue steps/s (plain single job)",
         "value": value,
@@ -200,7 +214,7 @@ def main():
                    "segments": ref.segments, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "kernel": "k_iterate_" + {1: "fp64", 2: "hybrid", 3: "mont"}[ref.path],
                      "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s (VALU lane-ops)",
-                     "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic,
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic, "traffic_unit": "bytes per launch",
                      "ops_per_step": ops, "avg_launch_ms": avg_launch_ms,
                      "kernel_steps_per_s": steps_total / (it_ms * 1e-3)},
         "int32_mul_equiv": {"frac": steps_total / (it_ms * 1e-3) * INT32_MUL_PER_STEP / INT32_MUL_PEAK,

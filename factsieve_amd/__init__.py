@@ -93,6 +93,7 @@ _SIGS = [
     ("fs_memcpy_htod", ctypes.c_int, [vp, vp, ctypes.c_size_t, vp, ctypes.c_int]),
     ("fs_memcpy_dtoh", ctypes.c_int, [vp, vp, ctypes.c_size_t, vp, ctypes.c_int]),
     ("fs_memset", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_size_t, vp]),
+    ("fs_memcpy_dtod", ctypes.c_int, [vp, vp, ctypes.c_size_t, vp]),
     ("fs_cands_alloc", ctypes.c_int, [ctypes.POINTER(Cands), u32]),
     ("fs_cands_free", ctypes.c_int, [ctypes.POINTER(Cands)]),
     ("fs_stream_create", ctypes.c_int, [ctypes.POINTER(vp)]),

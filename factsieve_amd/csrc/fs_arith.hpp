@@ -19,9 +19,9 @@
 //    s == +-1 (mod p) iff |s| == 1.  Seven full-rate FP64 VALU operations.
 //
 //  * Hybrid (p < 2^63): lazily reduced s in [0, 2p), multiplier m < 2^31; the
-//    product s*m mod 2^64 comes from two v_mad_u64_u32, the quotient from an
-//    FP64 estimate biased low by 1/256 so k in {floor(X)-1, floor(X)}, and
-//    s' = s*m - k*p lands in [0, 2p) exactly (computed mod 2^64).
+//    quotient comes from an FP64 estimate so that k is floor(X)-1 or floor(X),
+//    and s' = s*m - k*p in [0, 2p) is formed exactly mod 2^64 from 32x32->64
+//    multiply-adds.
 //
 // The error bounds are proved in the design notes shipped with the library and
 // are exercised by the parity tests at the edges of each range.
@@ -112,26 +112,22 @@ FS_HD double fs_canon_to_fp(uint64_t r, uint64_t p) {
 
 // ----------------------------------------------------------------- hybrid path
 
-// s*m mod p with s in [0, 2p), 0 <= m < 2^31, p < 2^63; result in [0, 2p).
-// pinv = fl(1/p), pinv32 = 2^32 * pinv, md = (double)m.
-FS_HD uint64_t fs_hy_mulmod(uint64_t s, uint32_t m, double md, uint64_t p,
-                                                 double pinv, double pinv32) {
+// s*m mod p with s in [0, 2p), 0 < m < 2^31, p < 2^63; result in [0, 2p).
+// pn = 2^64 - p, pinv = fl(1/p), pinv32 = 2^32 * pinv, md = (double)m.
+// With X = s*m/p < 2^32, x = fl(y*m + 1 - 2^-8) lies in (X, X + 1) (its error is
+// below 2^-18), so k = trunc(x) - 1 is floor(X) - 1 or floor(X) and
+// s*m - k*p lies in [0, 2p); it is formed mod 2^64 as s*m + p + trunc(x)*pn from
+// four 32x32->64 multiply-adds.
+FS_HD uint64_t fs_hy_mulmod(uint64_t s, uint32_t m, double md, uint64_t p, uint64_t pn, double pinv,
+                            double pinv32) {
 #pragma clang fp contract(off)
   const uint32_t s0 = (uint32_t)s, s1 = (uint32_t)(s >> 32);
-  // quotient estimate X ~ s*m/p, biased low so that k in {floor(X)-1, floor(X)}
   const double y = __builtin_fma((double)s1, pinv32, (double)s0 * pinv);
-  double x = __builtin_fma(y, md, -0.00390625);
-  x = __builtin_fmax(x, 0.0);
-  const uint32_t k = (uint32_t)x;
-  // s*m - k*p mod 2^64 (exact value lies in [0, 2p)); each 64-bit product is
-  // one full 32x32->64 partial product plus the low word of a second one
-  const uint64_t t0 = (uint64_t)s0 * m;
-  const uint32_t th = (uint32_t)(t0 >> 32) + s1 * m;
-  const uint64_t k0 = (uint64_t)k * (uint32_t)p;
-  const uint32_t kh = (uint32_t)(k0 >> 32) + k * (uint32_t)(p >> 32);
-  const uint64_t t = ((uint64_t)th << 32) | (uint32_t)t0;
-  const uint64_t kp = ((uint64_t)kh << 32) | (uint32_t)k0;
-  return t - kp;
+  const uint32_t k = (uint32_t)__builtin_fma(y, md, 0.99609375);
+  const uint64_t a = (uint64_t)s0 * m + p;
+  const uint64_t b = (uint64_t)k * (uint32_t)pn + a;
+  const uint32_t hi = (uint32_t)((uint64_t)s1 * m + (b >> 32)) + k * (uint32_t)(pn >> 32);
+  return ((uint64_t)hi << 32) | (uint32_t)b;
 }
 
 // --------------------------------------------- 64x32 Montgomery (REDC by 2^32)

@@ -356,6 +356,12 @@ extern "C" int fs_memcpy_dtoh(void* dst, const void* src, size_t bytes, fs_strea
   return FS_OK;
 }
 
+extern "C" int fs_memcpy_dtod(void* dst, const void* src, size_t bytes, fs_stream s) {
+  if (!bytes) return FS_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(s)), "hipMemcpyAsync(DtoD)");
+  return FS_OK;
+}
+
 extern "C" int fs_memset(void* d, int v, size_t bytes, fs_stream s) {
   if (!bytes) return FS_OK;
   HIPCHK(hipMemsetAsync(d, v, bytes, S(s)), "hipMemsetAsync");

@@ -115,7 +115,7 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
       const bool minus1 = (s == pm1) | (s == p2m1);
       if (plus1 | minus1) rec(plus1 ? -(int32_t)n : (int32_t)n);
       const uint32_t m = n + 1;
-      s = fs_hy_mulmod(s, m, (double)m, p, pinv, pinv32);
+      s = fs_hy_mulmod(s, m, (double)m, p, 0 - p, pinv, pinv32);
     }
     *res = s >= p ? s - p : s;
   } else if (path == FS_PATH_MONT) {

@@ -234,7 +234,7 @@ template <class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                         const double* __restrict__ nt, Rec rec) {
   const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
-  const uint64_t pp1 = p + 1, pm1 = p - 1, p2m1 = 2 * p - 1;
+  const uint64_t pp1 = p + 1, pm1 = p - 1, p2m1 = 2 * p - 1, pn = 0 - p;
   uint64_t s = s_in;
   for (uint32_t n = n0; n < n1; ++n) {
     const bool plus1 = (s == 1) | (s == pp1);
@@ -242,7 +242,7 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_i
     if (plus1 | minus1) rec(plus1 ? -(int32_t)n : (int32_t)n);
     uint32_t m = n + 1;
     asm volatile("" : "+s"(m));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
-    s = fs_hy_mulmod(s, m, nt[n - n0], p, pinv, pinv32);
+    s = fs_hy_mulmod(s, m, nt[n - n0], p, pn, pinv, pinv32);
   }
   return s >= p ? s - p : s;
 }

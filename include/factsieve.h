@@ -108,6 +108,8 @@ int fs_free(void* dptr);                                  /* sclReleaseMemObject
 int fs_memcpy_htod(void* dst, const void* src, size_t bytes, fs_stream s, int blocking); /* sclWrite[NB] */
 int fs_memcpy_dtoh(void* dst, const void* src, size_t bytes, fs_stream s, int blocking); /* sclRead[NB] */
 int fs_memset(void* dptr, int value, size_t bytes, fs_stream s);
+int fs_memcpy_dtod(void* dst, const void* src, size_t bytes, fs_stream s); /* device copy (also the
+                                                   HBM-counter calibration kernel of the benchmark) */
 int fs_cands_alloc(fs_cands* c, uint32_t cap);
 int fs_cands_free(fs_cands* c);
 
