@@ -13,7 +13,9 @@ jobs.  One JSON line is printed.
 roofline: the dominant kernel is the residue iteration.  Its work is counted
 as 7 VALU lane-operations per p*n step (the FP64 formulation: 2 multiplies,
 2 FMAs, 1 add, 1 round, 1 compare), timed with HIP events on the launch
-stream; peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
+stream (12 on the hybrid path of p > 2^52: 6 FP64 quotient ops, 2
+v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, 1 for the +-1 filter: 4 ops per 4 steps);
+peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
 vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
 int32_mul_equiv: the same steps/s expressed in the 11 32-bit multiplies a
 64-bit Montgomery step needs, over the measured int32-multiply peak (full
@@ -41,7 +43,7 @@ WORKLOADS = {
            "full-width modulus stress: n in [5e5,5.1e5), p in [2^62,2^62+2^30)"),
 }
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine clock
-OPS_PER_STEP = {1: 7, 2: 14, 3: 45}             # VALU lane-ops per p*n step by path
+OPS_PER_STEP = {1: 7, 2: 12, 3: 45}             # VALU lane-ops per p*n step by path (from the ISA)
 INT32_MUL_PER_STEP = 11                         # 64-bit Montgomery step: 4 + 3 + 4 partial products
 INT32_MUL_PEAK = VALU_PEAK_TOPS * 1e12          # full-rate v_mad_u64_u32 / v_mul_*_u32 (measured)
 # HBM bytes per candidate per k_iterate_fp64 launch, measured with rocprofv3 --pmc

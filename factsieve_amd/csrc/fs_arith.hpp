@@ -18,10 +18,10 @@
 //    gives s' == s*m - k*p exactly with |s'| <= 0.57 p.  Because |s| < p - 1,
 //    s == +-1 (mod p) iff |s| == 1.  Seven full-rate FP64 VALU operations.
 //
-//  * Hybrid (p < 2^63): lazily reduced s in [0, 2p), multiplier m < 2^31; the
-//    quotient comes from an FP64 estimate so that k is floor(X)-1 or floor(X),
-//    and s' = s*m - k*p in [0, 2p) is formed exactly mod 2^64 from 32x32->64
-//    multiply-adds.
+//  * Hybrid (2^52 <= p < 2^64 - 2^46): signed residue |s| <= 0.51p, multiplier
+//    m < 2^31; the quotient is an FP64 estimate rounded to nearest, and
+//    s' = s*m - k*p is formed exactly mod 2^64 from 32x32->64 multiply-adds.
+//    The setup program multiplies by a REDC by 2^32 when p < 2^63.
 //
 // The error bounds are proved in the design notes shipped with the library and
 // are exercised by the parity tests at the edges of each range.
@@ -30,6 +30,10 @@
 #include <stdint.h>
 
 #define FS_HD __host__ __device__ __forceinline__
+
+// Setup program word flag (FP64 and hybrid paths): square the running product
+// before multiplying by this term; set only on the first term of a 4-term block.
+#define FS_PROG_SQUARE 0x8000000000000000ull
 
 // ------------------------------------------------------------------ Montgomery
 
@@ -112,23 +116,42 @@ FS_HD double fs_canon_to_fp(uint64_t r, uint64_t p) {
 
 // ----------------------------------------------------------------- hybrid path
 
-// s*m mod p with s in [0, 2p), 0 < m < 2^31, p < 2^63; result in [0, 2p).
-// pn = 2^64 - p, pinv = fl(1/p), pinv32 = 2^32 * pinv, md = (double)m.
-// With X = s*m/p < 2^32, x = fl(y*m + 1 - 2^-8) lies in (X, X + 1) (its error is
-// below 2^-18), so k = trunc(x) - 1 is floor(X) - 1 or floor(X) and
-// s*m - k*p lies in [0, 2p); it is formed mod 2^64 as s*m + p + trunc(x)*pn from
-// four 32x32->64 multiply-adds.
-FS_HD uint64_t fs_hy_mulmod(uint64_t s, uint32_t m, double md, uint64_t p, uint64_t pn, double pinv,
-                            double pinv32) {
+// Largest modulus of the hybrid path (exclusive bound on pmax is this + 1).
+#define FS_HYBRID_PMAX (0xFFFFFFFFFFFFFFFFull - ((uint64_t)1 << 46) + 1)
+
+// Signed hybrid step: s*m mod p for a signed residue |s| <= 0.51 p, 0 < m < 2^31,
+// 2^52 <= p <= FS_HYBRID_PMAX; c31 = 2^31 p mod 2^64, pn = 2^64 - p,
+// pinv = fl(1/p), pinv32 = 2^32 * pinv, md = (double)m.
+// X = s*m/p is estimated as y*m with |error| < 2^-19; k = trunc(y*m + 2^31 + 1/2)
+// is round(X) + 2^31, so r = s*m - (k - 2^31) p has |r| <= (1/2 + 2^-19) p < 2^63
+// and is formed exactly mod 2^64 as s*m + c31 + k*pn from four 32x32->64
+// multiply-adds.  Because |r| < p - 1, r == +-1 (mod p) iff r == +-1.
+// *yq receives y ~ s/p (|y - s/p| < 2^-50 |s/p| + 2^-60), which the iterate
+// loops use as a filter for s == +-1: |y| <= 1.5 pinv holds for s in {-1,0,1}
+// and fails for every |s| >= 2.
+FS_HD int64_t fs_sh_mulmod(int64_t s, uint32_t m, double md, uint64_t c31, uint64_t pn, double pinv,
+                           double pinv32, double* yq) {
 #pragma clang fp contract(off)
-  const uint32_t s0 = (uint32_t)s, s1 = (uint32_t)(s >> 32);
+  const uint32_t s0 = (uint32_t)s;
+  const int32_t s1 = (int32_t)(s >> 32);
   const double y = __builtin_fma((double)s1, pinv32, (double)s0 * pinv);
-  const uint32_t k = (uint32_t)__builtin_fma(y, md, 0.99609375);
-  const uint64_t a = (uint64_t)s0 * m + p;
+  *yq = y;
+  const uint32_t k = (uint32_t)__builtin_fma(y, md, 2147483648.5);
+  const uint64_t a = (uint64_t)s0 * m + c31;
   const uint64_t b = (uint64_t)k * (uint32_t)pn + a;
-  const uint32_t hi = (uint32_t)((uint64_t)s1 * m + (b >> 32)) + k * (uint32_t)(pn >> 32);
-  return ((uint64_t)hi << 32) | (uint32_t)b;
+  const uint32_t hi = (uint32_t)((uint64_t)(uint32_t)s1 * m + (b >> 32)) + k * (uint32_t)(pn >> 32);
+  return (int64_t)(((uint64_t)hi << 32) | (uint32_t)b);
 }
+
+FS_HD int64_t fs_sh_mulmod(int64_t s, uint32_t m, double md, uint64_t c31, uint64_t pn, double pinv,
+                           double pinv32) {
+  double y;
+  return fs_sh_mulmod(s, m, md, c31, pn, pinv, pinv32, &y);
+}
+
+// Symmetric signed residue from a canonical one, and back.
+FS_HD int64_t fs_canon_to_signed(uint64_t r, uint64_t p) { return r > (p >> 1) ? (int64_t)(r - p) : (int64_t)r; }
+FS_HD uint64_t fs_signed_to_canon(int64_t s, uint64_t p) { return s < 0 ? (uint64_t)s + p : (uint64_t)s; }
 
 // --------------------------------------------- 64x32 Montgomery (REDC by 2^32)
 

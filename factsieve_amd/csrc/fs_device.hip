@@ -542,6 +542,10 @@ extern "C" int fs_setup(const fs_cands* c, const uint32_t* d_ctr, const void* d_
   if (rc) return rc;
   if (end > prog_terms) end = prog_terms;
   if (start > end) start = end;
+  if (path != FS_PATH_MONT && ((start | end) & 3)) {
+    g_last_error = "setup: FP64/hybrid program chunks must start and end on 4-term blocks";
+    return FS_E_ARG;
+  }
   if (!d_prog && end > start) {
     g_last_error = "setup: null program";
     return FS_E_ARG;

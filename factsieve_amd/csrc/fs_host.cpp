@@ -132,7 +132,7 @@ std::vector<uint64_t> fs_horner_program(uint32_t nmin, uint64_t bound) {
 
 extern "C" int fs_select_path(uint64_t pmax) {
   if (pmax <= ((uint64_t)1 << 52)) return FS_PATH_FP64;
-  if (pmax <= ((uint64_t)1 << 63)) return FS_PATH_HYBRID;
+  if (pmax <= FS_HYBRID_PMAX + 1) return FS_PATH_HYBRID;  // every p < pmax is covered
   return FS_PATH_MONT;
 }
 
@@ -155,16 +155,36 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
     return FS_E_ARG;
   }
   const uint64_t bound = (path == FS_PATH_FP64) ? ((uint64_t)1 << 48) : ((uint64_t)1 << 31);
-  const std::vector<uint64_t> prog = fs_horner_program(nmin, bound);
+  const std::vector<uint64_t> horner = fs_horner_program(nmin, bound);
+  // Blocks of 4 terms: a square is a flag (bit 63) on the first term of a
+  // block, so each group C_j is padded with multipliers 1 to a whole block.
+  std::vector<uint64_t> prog;
+  bool pending_square = false;
+  for (size_t i = 0; i <= horner.size(); ++i) {
+    if (i == horner.size() || horner[i] == 0) {
+      while (prog.size() % 4) prog.push_back(1);
+      if (i == horner.size()) break;
+      if (pending_square) prog.insert(prog.end(), {FS_PROG_SQUARE | 1, 1, 1, 1});  // empty group
+      pending_square = true;
+      continue;
+    }
+    prog.push_back(horner[i] | (pending_square ? FS_PROG_SQUARE : 0));
+    pending_square = false;
+  }
+  if (pending_square) prog.insert(prog.end(), {FS_PROG_SQUARE | 1, 1, 1, 1});
   if (words) {
     uint64_t E = 0;  // HYBRID: power of 2^-1 accumulated by REDC-by-2^32 / Montgomery squarings
     for (size_t i = 0; i < prog.size(); ++i) {
+      const uint64_t m = prog[i] & ~FS_PROG_SQUARE;
+      if (prog[i] & FS_PROG_SQUARE) E = 2 * E + 64;
+      E += 32;
       if (path == FS_PATH_FP64) {
-        const double d = (double)prog[i];  // exact: < 2^48
-        memcpy(&words[i], &d, 8);
+        const double d = (double)m;  // exact: < 2^48
+        uint64_t bits;
+        memcpy(&bits, &d, 8);
+        words[i] = bits | (prog[i] & FS_PROG_SQUARE);
       } else {
         words[i] = prog[i];
-        E = (prog[i] == 0) ? 2 * E + 64 : E + 32;
       }
     }
     if (path == FS_PATH_HYBRID) words[prog.size()] = E;
@@ -661,7 +681,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     return FS_E_ARG;
   }
   if ((path == FS_PATH_FP64 && pmax > ((uint64_t)1 << 52)) ||
-      (path == FS_PATH_HYBRID && pmax > ((uint64_t)1 << 63))) {
+      (path == FS_PATH_HYBRID && pmax > FS_HYBRID_PMAX + 1)) {
     fs_set_error_msg("forced arithmetic path does not cover pmax");
     return FS_E_ARG;
   }

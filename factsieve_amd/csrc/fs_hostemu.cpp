@@ -24,32 +24,44 @@ extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* r
     const double pd = (double)p, pinv = 1.0 / pd;
     double s = 1.0;
     for (int64_t k = 0; k < T; ++k) {
-      double m;
-      memcpy(&m, &w[k], 8);
-      if (m == 0.0) {
+      if (w[k] & FS_PROG_SQUARE) {
         const uint64_t a = fs_fp_to_canon(s, p);
         s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
-      } else {
-        s = fs_fp_mulmod(s, m, pd, pinv);
       }
+      double m;
+      const uint64_t bits = w[k] & ~FS_PROG_SQUARE;
+      memcpy(&m, &bits, 8);
+      s = fs_fp_mulmod(s, m, pd, pinv);
     }
     *res = fs_fp_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {
-    if (p >= ((uint64_t)1 << 63)) return FS_E_ARG;
-    const uint32_t pn32 = (uint32_t)(0 - q);
-    uint64_t s = 1;
-    for (int64_t k = 0; k < T; ++k) {
-      const uint32_t m = (uint32_t)w[k];
-      if (m == 0) {
-        const uint64_t a = s >= p ? s - p : s;
-        s = fs_mont_mul(a, a, p, q);
-      } else {
-        s = fs_redc32_mul(s, m, p, pn32);
+    if (p > FS_HYBRID_PMAX) return FS_E_ARG;
+    if (p < ((uint64_t)1 << 63)) {
+      const uint32_t pn32 = (uint32_t)(0 - q);
+      uint64_t s = 1;
+      for (int64_t k = 0; k < T; ++k) {
+        if (w[k] & FS_PROG_SQUARE) {
+          const uint64_t a = s >= p ? s - p : s;
+          s = fs_mont_mul(a, a, p, q);
+        }
+        s = fs_redc32_mul(s, (uint32_t)w[k], p, pn32);
       }
+      s = s >= p ? s - p : s;
+      if (w[T] != 0) s = fs_mont_mul(s, fs_mont_pow2(w[T], p, q, one), p, q);
+      *res = s;
+    } else {
+      const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+      int64_t s = 1;
+      for (int64_t k = 0; k < T; ++k) {
+        if (w[k] & FS_PROG_SQUARE) {
+          const uint64_t a = fs_signed_to_canon(s, p);
+          s = fs_canon_to_signed(fs_mul_mod(a, a, p, q, r2), p);
+        }
+        const uint32_t m = (uint32_t)w[k];
+        s = fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32);
+      }
+      *res = fs_signed_to_canon(s, p);
     }
-    s = s >= p ? s - p : s;
-    if (w[T] != 0) s = fs_mont_mul(s, fs_mont_pow2(w[T], p, q, one), p, q);
-    *res = s;
   } else if (path == FS_PATH_MONT) {
     uint64_t s6 = one;
     for (int64_t k = 0; k < T; ++k) {
@@ -106,18 +118,21 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     }
     *res = fs_fp_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {
-    if (p >= ((uint64_t)1 << 63)) return FS_E_ARG;
+    if (p > FS_HYBRID_PMAX) return FS_E_ARG;
     const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
-    const uint64_t pp1 = p + 1, pm1 = p - 1, p2m1 = 2 * p - 1;
-    uint64_t s = *res;
+    const double thr = 1.5 * pinv;  // the device loops' filter on the quotient estimate
+    int64_t s = fs_canon_to_signed(*res, p);
     for (uint32_t n = n0; n < n1; ++n) {
-      const bool plus1 = (s == 1) | (s == pp1);
-      const bool minus1 = (s == pm1) | (s == p2m1);
-      if (plus1 | minus1) rec(plus1 ? -(int32_t)n : (int32_t)n);
       const uint32_t m = n + 1;
-      s = fs_hy_mulmod(s, m, (double)m, p, 0 - p, pinv, pinv32);
+      double y;
+      const int64_t t = fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32, &y);
+      if ((s == 1) | (s == -1)) {
+        if (!(__builtin_fabs(y) <= thr)) return FS_E_VALIDATE;  // the filter would have missed it
+        rec(s == 1 ? -(int32_t)n : (int32_t)n);
+      }
+      s = t;
     }
-    *res = s >= p ? s - p : s;
+    *res = fs_signed_to_canon(s, p);
   } else if (path == FS_PATH_MONT) {
     const uint64_t nmo = p - one;
     uint64_t s6 = fs_mont_mul(*res, r2, p, q);

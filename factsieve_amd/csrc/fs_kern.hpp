@@ -107,48 +107,90 @@ __device__ __forceinline__ void fs_primegen_block(uint64_t base, uint64_t high, 
 
 // ------------------------------------------------------------------ setup
 
-// FP64 program (p < 2^52): multipliers < 2^48 as doubles, 0.0 = square.
+// FP64 and hybrid programs come in 32-byte blocks of four terms (multipliers);
+// FS_PROG_SQUARE on a block's first term squares the running product first.
+// A block is one scalar load, issued one block ahead; start and end are
+// multiples of 4 (block boundaries) and the program words are 32-byte aligned.
+struct alignas(32) fs_prog_block {
+  uint64_t w[4];
+};
+
+// FP64 program (p < 2^52): multipliers < 2^48 as double bits.
 __device__ __forceinline__ uint64_t fs_cand_setup_fp64(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
-                                                       const double* __restrict__ prog, uint32_t start,
+                                                       const double* __restrict__ progd, uint32_t start,
                                                        uint32_t end) {
+  const fs_prog_block* __restrict__ prog = (const fs_prog_block*)progd;
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = (start == 0) ? 1.0 : fs_canon_to_fp(s_in, p);
-  for (uint32_t k = start; k < end; ++k) {
-    const double m = prog[k];
-    if (m == 0.0) {
-      const uint64_t a = fs_fp_to_canon(s, p);
-      s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
-    } else {
-      s = fs_fp_mulmod(s, m, pd, pinv);
+  const uint32_t b0 = start >> 2, b1 = end >> 2;
+  if (b0 < b1) {
+    fs_prog_block cur = prog[b0];
+    for (uint32_t b = b0; b < b1; ++b) {
+      const fs_prog_block nxt = prog[b + 1 < b1 ? b + 1 : b];
+      if (cur.w[0] & FS_PROG_SQUARE) {
+        const uint64_t a = fs_fp_to_canon(s, p);
+        s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+      }
+      s = fs_fp_mulmod(s, __longlong_as_double(cur.w[0] & ~FS_PROG_SQUARE), pd, pinv);
+      s = fs_fp_mulmod(s, __longlong_as_double(cur.w[1]), pd, pinv);
+      s = fs_fp_mulmod(s, __longlong_as_double(cur.w[2]), pd, pinv);
+      s = fs_fp_mulmod(s, __longlong_as_double(cur.w[3]), pd, pinv);
+      cur = nxt;
     }
   }
   return fs_fp_to_canon(s, p);
 }
 
-// Hybrid program (p < 2^63): multipliers < 2^31, 0 = square.  The running
-// product is x * 2^-E mod p (below 2p) because each multiply is a REDC by 2^32
-// and each square a Montgomery product by 2^64; prog[terms] holds the total E,
-// removed after the last term.
+// Hybrid program: multipliers < 2^31.  For p < 2^63 each multiply is a REDC
+// by 2^32 and each square a Montgomery product by 2^64, so the running product
+// is x * 2^-E mod p (below 2p); word [terms] holds the total E, removed after
+// the last block.  For 2^63 <= p the multiplies are exact signed hybrid steps.
 __device__ __forceinline__ uint64_t fs_cand_setup_hybrid(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
-                                                         const uint64_t* __restrict__ prog, uint32_t terms,
+                                                         const uint64_t* __restrict__ progw, uint32_t terms,
                                                          uint32_t start, uint32_t end) {
-  const uint32_t pn32 = (uint32_t)(0 - q);
-  uint64_t s = (start == 0) ? 1 : s_in;
-  for (uint32_t k = start; k < end; ++k) {
-    const uint32_t m = (uint32_t)prog[k];
-    if (m == 0) {
-      const uint64_t a = s >= p ? s - p : s;
-      s = fs_mont_mul(a, a, p, q);
-    } else {
-      s = fs_redc32_mul(s, m, p, pn32);
+  const fs_prog_block* __restrict__ prog = (const fs_prog_block*)progw;
+  const uint32_t b0 = start >> 2, b1 = end >> 2;
+  if (p < ((uint64_t)1 << 63)) {
+    const uint32_t pn32 = (uint32_t)(0 - q);
+    uint64_t s = (start == 0) ? 1 : s_in;
+    if (b0 < b1) {
+      fs_prog_block cur = prog[b0];
+      for (uint32_t b = b0; b < b1; ++b) {
+        const fs_prog_block nxt = prog[b + 1 < b1 ? b + 1 : b];
+        if (cur.w[0] & FS_PROG_SQUARE) {
+          const uint64_t a = s >= p ? s - p : s;
+          s = fs_mont_mul(a, a, p, q);
+        }
+        s = fs_redc32_mul(s, (uint32_t)cur.w[0], p, pn32);
+        s = fs_redc32_mul(s, (uint32_t)cur.w[1], p, pn32);
+        s = fs_redc32_mul(s, (uint32_t)cur.w[2], p, pn32);
+        s = fs_redc32_mul(s, (uint32_t)cur.w[3], p, pn32);
+        cur = nxt;
+      }
+    }
+    s = s >= p ? s - p : s;
+    if (end == terms && progw[terms] != 0) {
+      const uint64_t one = fs_mont_mul(1, r2, p, q);
+      s = fs_mont_mul(s, fs_mont_pow2(progw[terms], p, q, one), p, q);
+    }
+    return s;
+  }
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+  const uint64_t pn = 0 - p, c31 = p << 31;
+  int64_t s = (start == 0) ? 1 : fs_canon_to_signed(s_in, p);
+  for (uint32_t b = b0; b < b1; ++b) {
+    const fs_prog_block cur = prog[b];
+    if (cur.w[0] & FS_PROG_SQUARE) {
+      const uint64_t a = fs_signed_to_canon(s, p);
+      s = fs_canon_to_signed(fs_mul_mod(a, a, p, q, r2), p);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t m = (uint32_t)cur.w[j];
+      s = fs_sh_mulmod(s, m, (double)m, c31, pn, pinv, pinv32);
     }
   }
-  s = s >= p ? s - p : s;
-  if (end == terms && prog[terms] != 0) {
-    const uint64_t one = fs_mont_mul(1, r2, p, q);
-    s = fs_mont_mul(s, fs_mont_pow2(prog[terms], p, q, one), p, q);
-  }
-  return s;
+  return fs_signed_to_canon(s, p);
 }
 
 // Reference power-table evaluation (setup.cl:49-103) on the Montgomery path:
@@ -228,23 +270,45 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in,
   return fs_fp_to_canon(s, p);
 }
 
-// Hybrid path (p < 2^63): lazily reduced s in [0, 2p), 64x32 integer product
-// and FP64 quotient estimate; nt as for the FP64 path.
+// Hybrid path (p <= FS_HYBRID_PMAX): signed residue |s| <= 0.51 p, exact
+// 64x32 products mod 2^64 with an FP64 quotient; nt as for the FP64 path.
+// n! = +-1 (mod p) iff s == +-1.  Four steps share one test: the quotient
+// estimates y ~ s/p that the steps compute anyway pass |y| <= 1.5/p only for
+// s in {-1, 0, 1}, and the branch then tests s exactly.
 template <class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                         const double* __restrict__ nt, Rec rec) {
-  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
-  const uint64_t pp1 = p + 1, pm1 = p - 1, p2m1 = 2 * p - 1, pn = 0 - p;
-  uint64_t s = s_in;
-  for (uint32_t n = n0; n < n1; ++n) {
-    const bool plus1 = (s == 1) | (s == pp1);
-    const bool minus1 = (s == pm1) | (s == p2m1);
-    if (plus1 | minus1) rec(plus1 ? -(int32_t)n : (int32_t)n);
-    uint32_t m = n + 1;
-    asm volatile("" : "+s"(m));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
-    s = fs_hy_mulmod(s, m, nt[n - n0], p, pn, pinv, pinv32);
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0, thr = 1.5 * pinv;
+  const uint64_t pn = 0 - p, c31 = p << 31;
+  int64_t s = fs_canon_to_signed(s_in, p);
+  uint32_t n = n0;
+  const uint32_t n4 = n0 + ((n1 - n0) & ~3u);
+  for (; n < n4; n += 4) {
+    const double* m = nt + (n - n0);
+    uint32_t mm = n + 1;
+    asm volatile("" : "+s"(mm));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
+    double ya, yb, yd, ye;
+    const int64_t a = s;
+    const int64_t b = fs_sh_mulmod(a, mm, m[0], c31, pn, pinv, pinv32, &ya);
+    const int64_t d = fs_sh_mulmod(b, mm + 1, m[1], c31, pn, pinv, pinv32, &yb);
+    const int64_t e = fs_sh_mulmod(d, mm + 2, m[2], c31, pn, pinv, pinv32, &yd);
+    s = fs_sh_mulmod(e, mm + 3, m[3], c31, pn, pinv, pinv32, &ye);
+    const double ymin = __builtin_fmin(__builtin_fmin(__builtin_fabs(ya), __builtin_fabs(yb)),
+                                       __builtin_fmin(__builtin_fabs(yd), __builtin_fabs(ye)));
+    if (ymin <= thr) {
+      if ((a == 1) | (a == -1)) rec(a == 1 ? -(int32_t)n : (int32_t)n);
+      if ((b == 1) | (b == -1)) rec(b == 1 ? -(int32_t)(n + 1) : (int32_t)(n + 1));
+      if ((d == 1) | (d == -1)) rec(d == 1 ? -(int32_t)(n + 2) : (int32_t)(n + 2));
+      if ((e == 1) | (e == -1)) rec(e == 1 ? -(int32_t)(n + 3) : (int32_t)(n + 3));
+    }
   }
-  return s >= p ? s - p : s;
+  for (; n < n1; ++n) {
+    if ((s == 1) | (s == -1)) rec(s == 1 ? -(int32_t)n : (int32_t)n);
+    uint32_t mm = n + 1;
+    asm volatile("" : "+s"(mm));
+    s = fs_sh_mulmod(s, mm, nt[n - n0], c31, pn, pinv, pinv32);
+  }
+  return fs_signed_to_canon(s, p);
 }
 
 // Reference formulation (iterate.cl:54-80): Montgomery residue s6 and

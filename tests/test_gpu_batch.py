@@ -18,8 +18,8 @@ def small_mix():
         (10 ** 8, 10 ** 8 + 300000, 1000, 3000),        # FP64, factors present
         (127, 20000, 127, 30000),                       # p <= nmax: many factors
         (10 ** 9, 10 ** 9 + 100000, 5000, 5000),        # empty n-range
-        (2 ** 63 + 1000, 2 ** 63 + 200000, 2000, 2600), # Montgomery path
-        (2 ** 64 - 100000, 2 ** 64 - 1, 500, 900),      # top of the range
+        (2 ** 63 + 1000, 2 ** 63 + 200000, 2000, 2600), # hybrid above 2^63
+        (2 ** 64 - 100000, 2 ** 64 - 1, 500, 900),      # top of the range: Montgomery
         (2 ** 52 - 50000, 2 ** 52 + 50000, 3000, 3300), # hybrid across 2^52
     ]
     return jobs

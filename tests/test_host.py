@@ -52,7 +52,10 @@ def test_select_path(fs):
     assert fs.select_path(2 ** 52) == fs.PATH_FP64
     assert fs.select_path(2 ** 52 + 1) == fs.PATH_HYBRID
     assert fs.select_path(2 ** 63) == fs.PATH_HYBRID
-    assert fs.select_path(2 ** 63 + 1) == fs.PATH_MONT
+    assert fs.select_path(2 ** 63 + 1) == fs.PATH_HYBRID
+    # the signed hybrid step needs (1/2 + 2^-19) p < 2^63: p <= 2^64 - 2^46
+    assert fs.select_path(2 ** 64 - 2 ** 46 + 1) == fs.PATH_HYBRID
+    assert fs.select_path(2 ** 64 - 2 ** 46 + 2) == fs.PATH_MONT
 
 
 @pytest.mark.parametrize("nmin", [127, 1000, 10000, 100000, 500000])
@@ -80,13 +83,21 @@ def test_power_table_term_counts(fs):
 def test_setup_program_evaluates_to_factorial(fs, path, nmin):
     import struct
     n, words = fs.setup_program(nmin, path)
+    assert n % 4 == 0
     bound = 2 ** 48 if path == 1 else 2 ** 31
-    acc = 1
-    for w in words[:n]:
+    acc, e = 1, 0
+    for i, w in enumerate(words[:n]):
+        sq = w >> 63
+        assert not sq or i % 4 == 0          # squares only open a 4-term block
+        w &= (1 << 63) - 1
         m = int(struct.unpack("<d", struct.pack("<Q", w))[0]) if path == 1 else w
-        assert 0 <= m < bound
-        acc = acc * acc if m == 0 else acc * m
+        assert 1 <= m < bound
+        if sq:
+            acc, e = acc * acc, 2 * e + 64
+        acc, e = acc * m, e + 32
     assert acc == math.factorial(nmin)
+    if path == 2:
+        assert words[n] == e                  # exponent of the REDC factors 2^-32 / 2^-64
 
 
 def test_is_prime_and_verify_match_oracle(fs):
@@ -125,6 +136,8 @@ EDGE_RANGES = [
     ((1 << 52) + 1, (1 << 52) + 3000, [2, 3]),
     ((1 << 62) - 3000, 1 << 62, [2, 3]),
     ((1 << 63) - 3000, 1 << 63, [2, 3]),
+    ((1 << 63) + 1, (1 << 63) + 3000, [2, 3]),
+    ((1 << 64) - (1 << 46) - 3000, (1 << 64) - (1 << 46) + 1, [2, 3]),
     ((1 << 64) - 3000, (1 << 64) - 1, [3]),
 ]
 
@@ -161,11 +174,29 @@ def test_host_arithmetic_matches_oracle(fs, lo, hi, paths):
 
 def test_host_arithmetic_planted_factor(fs):
     """Start residues at +1 and -1: every formulation must report the factor."""
-    for p, paths in [(1000003, [1, 2, 3]), (2 ** 61 - 1, [2, 3]), (2 ** 64 - 59, [3])]:
+    for p, paths in [(1000003, [1, 2, 3]), (2 ** 61 - 1, [2, 3]), (2 ** 63 + 29, [2, 3]), (2 ** 64 - 59, [3])]:
         for start, sign in [(1, -1), (p - 1, 1)]:
             for path in paths:
                 _, facs, _ = fs.hostemu_iterate(path, p, 500, 502, start)
                 assert facs[0] == (p, sign * 500), (p, path)
+
+
+def test_hybrid_quotient_filter_sound(fs):
+    """The hybrid iterate tests s == +-1 only where the quotient estimate
+    passes |y| <= 1.5/p; the host execution returns an error if a residue of
+    +-1 ever fails that filter.  Start residues +-1 (and +-1 times a few
+    multipliers) over odd moduli from 3 to the top of the hybrid range."""
+    rng = random.Random(5)
+    ps = [3, 5, 7, 127, 65537, 2 ** 31 - 1, 2 ** 32 + 15, 2 ** 52 - 1, 2 ** 52 + 1, 2 ** 63 - 25, 2 ** 63 + 29,
+          2 ** 64 - 2 ** 46 - 1, 2 ** 64 - 2 ** 46 - 3]
+    ps += [rng.randrange(3, 2 ** rng.randrange(3, 65)) | 1 for _ in range(300)]
+    for p in ps:
+        if p > 2 ** 64 - 2 ** 46:
+            continue
+        for start in (1, p - 1):
+            n0 = rng.randrange(1, 2 ** 31 - 10)
+            _, facs, nf = fs.hostemu_iterate(fs.PATH_HYBRID, p, n0, n0 + 3, start)
+            assert nf >= 1 and facs[0] == (p, (-1 if start == 1 else 1) * n0), p
 
 
 @pytest.mark.parametrize("lo,hi", [(127, 100000), (10 ** 8, 10 ** 8 + 200000), (2 ** 52 - 100000, 2 ** 52 + 100000),
