@@ -10,15 +10,17 @@
 //    every odd p < 2^64.  64x64->128 products are built from 32x32->64
 //    v_mad_u64_u32 partial products.
 //
-//  * FP64 error-free product with a rounded quotient (p < 2^52): the residue is
+//  * FP64 error-free product with a rounded quotient (p < 2^53): the residue is
 //    a double s with |s| <= 0.75 p; for an integer multiplier m < 2^48,
-//      h = fl(s*m), l = fma(s, m, -h)        (h + l == s*m exactly)
-//      k = rint(fl(h * (1/p)))               (|k - s*m/p| <= 0.57)
-//      s' = fma(-k, p, h) + l                (exact: |fma result| < 2^53)
-//    gives s' == s*m - k*p exactly with |s'| <= 0.57 p.  Because |s| < p - 1,
+//      h = fl(s*m), l = fma(s, m, -h)        (h + l == s*m exactly, |l| <= 2^47)
+//      k = rint(fl(h * (1/p)))               (|k - s*m/p| <= 0.5 + 3*2^-53*0.75*2^48 < 0.571)
+//      s' = fma(-k, p, h) + l                (exact: h - k*p is an integer of
+//                                             magnitude <= 0.571 p + 2^47 < 2^53)
+//    gives s' == s*m - k*p exactly with |s'| <= 0.571 p.  Every integer below
+//    2^53 (p, residues, n) is an exact double.  Because |s| < p - 1,
 //    s == +-1 (mod p) iff |s| == 1.  Seven full-rate FP64 VALU operations.
 //
-//  * Hybrid (2^52 <= p < 2^64 - 2^46): signed residue |s| <= 0.51p, multiplier
+//  * Hybrid (p < 2^64 - 2^46, used above 2^53): signed residue |s| <= 0.51p, multiplier
 //    m < 2^31; the quotient is an FP64 estimate rounded to nearest, and
 //    s' = s*m - k*p is formed exactly mod 2^64 from 32x32->64 multiply-adds.
 //    The setup program multiplies by a REDC by 2^32 when p < 2^63.
@@ -93,7 +95,7 @@ FS_HD uint64_t fs_r2_from_one(uint64_t one, uint64_t p, uint64_t q) {
 
 // ------------------------------------------------------------------ FP64 path
 
-// s*m mod p for |s| <= 0.75p, integer 0 <= m < 2^48, p < 2^52 (see header).
+// s*m mod p for |s| <= 0.75p, integer 0 <= m < 2^48, odd p < 2^53 (see header).
 FS_HD double fs_fp_mulmod(double s, double m, double p, double pinv) {
 #pragma clang fp contract(off)
   const double h = s * m;
@@ -120,7 +122,7 @@ FS_HD double fs_canon_to_fp(uint64_t r, uint64_t p) {
 #define FS_HYBRID_PMAX (0xFFFFFFFFFFFFFFFFull - ((uint64_t)1 << 46) + 1)
 
 // Signed hybrid step: s*m mod p for a signed residue |s| <= 0.51 p, 0 < m < 2^31,
-// 2^52 <= p <= FS_HYBRID_PMAX; c31 = 2^31 p mod 2^64, pn = 2^64 - p,
+// 3 <= p <= FS_HYBRID_PMAX; c31 = 2^31 p mod 2^64, pn = 2^64 - p,
 // pinv = fl(1/p), pinv32 = 2^32 * pinv, md = (double)m.
 // X = s*m/p is estimated as y*m with |error| < 2^-19; k = trunc(y*m + 2^31 + 1/2)
 // is round(X) + 2^31, so r = s*m - (k - 2^31) p has |r| <= (1/2 + 2^-19) p < 2^63

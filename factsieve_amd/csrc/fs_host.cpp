@@ -131,7 +131,7 @@ std::vector<uint64_t> fs_horner_program(uint32_t nmin, uint64_t bound) {
 }
 
 extern "C" int fs_select_path(uint64_t pmax) {
-  if (pmax <= ((uint64_t)1 << 52)) return FS_PATH_FP64;
+  if (pmax <= ((uint64_t)1 << 53)) return FS_PATH_FP64;
   if (pmax <= FS_HYBRID_PMAX + 1) return FS_PATH_HYBRID;  // every p < pmax is covered
   return FS_PATH_MONT;
 }
@@ -680,7 +680,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     fs_set_error_msg("unknown arithmetic path");
     return FS_E_ARG;
   }
-  if ((path == FS_PATH_FP64 && pmax > ((uint64_t)1 << 52)) ||
+  if ((path == FS_PATH_FP64 && pmax > ((uint64_t)1 << 53)) ||
       (path == FS_PATH_HYBRID && pmax > FS_HYBRID_PMAX + 1)) {
     fs_set_error_msg("forced arithmetic path does not cover pmax");
     return FS_E_ARG;

@@ -20,7 +20,7 @@ extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* r
   std::vector<uint64_t> w((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1);
   fs_setup_program(nmin, path, w.data());
   if (path == FS_PATH_FP64) {
-    if (p >= ((uint64_t)1 << 52)) return FS_E_ARG;
+    if (p >= ((uint64_t)1 << 53)) return FS_E_ARG;
     const double pd = (double)p, pinv = 1.0 / pd;
     double s = 1.0;
     for (int64_t k = 0; k < T; ++k) {
@@ -109,7 +109,7 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     ++cnt;
   };
   if (path == FS_PATH_FP64) {
-    if (p >= ((uint64_t)1 << 52)) return FS_E_ARG;
+    if (p >= ((uint64_t)1 << 53)) return FS_E_ARG;
     const double pd = (double)p, pinv = 1.0 / pd;
     double s = fs_canon_to_fp(*res, p);
     for (uint32_t n = n0; n < n1; ++n) {

@@ -58,8 +58,8 @@ extern "C" {
 /* Arithmetic formulation of the residue kernels.  All produce bit-identical
  * residues; the driver picks by pmax unless a run forces one. */
 #define FS_PATH_AUTO 0
-#define FS_PATH_FP64 1       /* p < 2^52: exact FP64 error-free products + rounded quotient */
-#define FS_PATH_HYBRID 2     /* p < 2^63: exact 64x32 integer product, FP64 quotient */
+#define FS_PATH_FP64 1       /* p < 2^53: exact FP64 error-free products + rounded quotient */
+#define FS_PATH_HYBRID 2     /* p <= 2^64 - 2^46: exact 64x32 integer product, FP64 quotient */
 #define FS_PATH_MONT 3       /* any odd p < 2^64: 64-bit Montgomery REDC (reference formulation) */
 
 /* 16-byte factor record, identical layout to `factor` (cl_sieve.h:4-7,
@@ -169,10 +169,13 @@ int fs_verifyresult(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_strea
  * arrays to get the term count only. */
 int64_t fs_power_table(uint32_t nmin, uint64_t* bases, uint32_t* exps, uint32_t* curbits);
 /* Setup program for `path` in words of 8 bytes; returns the term count T
- * (pass NULL words to get it).  FP64: T doubles, HYBRID: T integers; each term
- * is a multiplier (< 2^48 resp. < 2^31) or 0 = square the running product, and
- * the ordered evaluation from 1 yields nmin!.  MONT: the reference power table,
- * 2*T words: bases[T] then (exp | curbit << 32)[T]. */
+ * (pass NULL words to get it).  FP64 and HYBRID: T a multiple of 4, in blocks of
+ * four multipliers (FP64: doubles < 2^48 as bits; HYBRID: integers < 2^31;
+ * padding multipliers are 1).  Bit 63 set on a block's first word (FS_PROG_SQUARE
+ * in the kernels) squares the running product before that block; evaluating
+ * from 1 yields nmin!.  HYBRID has one more word, [T], the total power of 2^-1
+ * that its REDC-by-2^32 multiplies leave (used for p < 2^63).  MONT: the
+ * reference power table, 2*T words: bases[T] then (exp | curbit << 32)[T]. */
 int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words);
 /* Path the driver uses for p < pmax. */
 int fs_select_path(uint64_t pmax);

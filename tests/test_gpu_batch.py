@@ -20,7 +20,8 @@ def small_mix():
         (10 ** 9, 10 ** 9 + 100000, 5000, 5000),        # empty n-range
         (2 ** 63 + 1000, 2 ** 63 + 200000, 2000, 2600), # hybrid above 2^63
         (2 ** 64 - 100000, 2 ** 64 - 1, 500, 900),      # top of the range: Montgomery
-        (2 ** 52 - 50000, 2 ** 52 + 50000, 3000, 3300), # hybrid across 2^52
+        (2 ** 52 - 50000, 2 ** 52 + 50000, 3000, 3300), # FP64 across 2^52
+        (2 ** 53 - 50000, 2 ** 53 + 50000, 3000, 3300), # hybrid across 2^53
     ]
     return jobs
 

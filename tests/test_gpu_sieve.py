@@ -18,7 +18,7 @@ C1 = json.load(open(os.path.join(GOLDEN, "c1.json")))
 
 
 def paths_for(fs, pmax):
-    if pmax <= 2 ** 52:
+    if pmax <= 2 ** 53:
         return [fs.PATH_FP64, fs.PATH_HYBRID, fs.PATH_MONT]
     if pmax <= 2 ** 64 - 2 ** 46 + 1:
         return [fs.PATH_HYBRID, fs.PATH_MONT]
@@ -147,6 +147,8 @@ def test_segmentation_and_chunking_invariance(gpu):
     (114, 126, 114, 200),                     # no candidates
     (2 ** 64 - 200000, 2 ** 64 - 1, 1000, 1500),
     (2 ** 52 - 100000, 2 ** 52 + 100000, 3000, 3500),
+    (2 ** 53 - 100000, 2 ** 53, 3000, 3500),                              # top of the FP64 range
+    (2 ** 53 - 100000, 2 ** 53 + 100000, 3000, 3500),
     (2 ** 63 - 100000, 2 ** 63 + 100000, 3000, 3500),
     (2 ** 64 - 2 ** 46 - 100000, 2 ** 64 - 2 ** 46 + 1, 3000, 3500),   # top of the hybrid range
     (2 ** 64 - 2 ** 46 - 50000, 2 ** 64 - 2 ** 46 + 50000, 3000, 3500),  # straddles it: Montgomery

@@ -50,7 +50,8 @@ def test_no_device_gives_clean_error(fs):
 
 def test_select_path(fs):
     assert fs.select_path(2 ** 52) == fs.PATH_FP64
-    assert fs.select_path(2 ** 52 + 1) == fs.PATH_HYBRID
+    assert fs.select_path(2 ** 53) == fs.PATH_FP64
+    assert fs.select_path(2 ** 53 + 1) == fs.PATH_HYBRID
     assert fs.select_path(2 ** 63) == fs.PATH_HYBRID
     assert fs.select_path(2 ** 63 + 1) == fs.PATH_HYBRID
     # the signed hybrid step needs (1/2 + 2^-19) p < 2^63: p <= 2^64 - 2^46
@@ -133,7 +134,9 @@ def test_parse_matches_oracle(fs):
 EDGE_RANGES = [
     (10 ** 8, 10 ** 8 + 4000, [1, 2, 3]),
     ((1 << 52) - 3000, 1 << 52, [1, 2, 3]),
-    ((1 << 52) + 1, (1 << 52) + 3000, [2, 3]),
+    ((1 << 52) + 1, (1 << 52) + 3000, [1, 2, 3]),
+    ((1 << 53) - 3000, 1 << 53, [1, 2, 3]),
+    ((1 << 53) + 1, (1 << 53) + 3000, [2, 3]),
     ((1 << 62) - 3000, 1 << 62, [2, 3]),
     ((1 << 63) - 3000, 1 << 63, [2, 3]),
     ((1 << 63) + 1, (1 << 63) + 3000, [2, 3]),
@@ -174,7 +177,8 @@ def test_host_arithmetic_matches_oracle(fs, lo, hi, paths):
 
 def test_host_arithmetic_planted_factor(fs):
     """Start residues at +1 and -1: every formulation must report the factor."""
-    for p, paths in [(1000003, [1, 2, 3]), (2 ** 61 - 1, [2, 3]), (2 ** 63 + 29, [2, 3]), (2 ** 64 - 59, [3])]:
+    for p, paths in [(1000003, [1, 2, 3]), (2 ** 53 - 111, [1, 2, 3]), (2 ** 61 - 1, [2, 3]), (2 ** 63 + 29, [2, 3]),
+                     (2 ** 64 - 59, [3])]:
         for start, sign in [(1, -1), (p - 1, 1)]:
             for path in paths:
                 _, facs, _ = fs.hostemu_iterate(path, p, 500, 502, start)
