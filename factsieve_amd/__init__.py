@@ -125,6 +125,7 @@ _SIGS = [
     ("fs_select_path", ctypes.c_int, [u64]),
     ("fs_is_prime", ctypes.c_int, [u64]),
     ("fs_verify_factor", ctypes.c_int, [u64, u32, i32]),
+    ("fs_verify_factors", i64, [ctypes.POINTER(Factor), u64, ctypes.c_int]),
     ("fs_parse_uint64", ctypes.c_int, [ctypes.POINTER(u64), ctypes.c_char_p, u64, u64]),
     ("fs_hostemu_setup", ctypes.c_int, [ctypes.c_int, u64, u32, ctypes.POINTER(u64)]),
     ("fs_hostemu_iterate", ctypes.c_int, [ctypes.c_int, u64, u32, u32, ctypes.POINTER(u64),
@@ -200,6 +201,18 @@ def is_prime(p):
 
 def verify_factor(p, n, c):
     return bool(lib().fs_verify_factor(p, n, c))
+
+
+def verify_factors(factors, threads=16):
+    """CPU verification of [(p, nc), ...] (nc < 0: n!-1); index of the first
+    factor that fails, or -1."""
+    arr = (Factor * max(len(factors), 1))()
+    for i, (p, nc) in enumerate(factors):
+        arr[i].p, arr[i].nc, arr[i].pad = p, nc, 0
+    r = lib().fs_verify_factors(arr, len(factors), threads)
+    if r < -1:
+        raise FactSieveError(r, last_error())
+    return r
 
 
 def parse_uint64(s, lo, hi):

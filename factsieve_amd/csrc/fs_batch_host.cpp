@@ -36,25 +36,6 @@ double setup_cost(int path) { return path == FS_PATH_FP64 ? 6.0 : path == FS_PAT
 
 }  // namespace
 
-// Setup programs by (nmin, path), words ready for upload (HYBRID: with the
-// trailing power-of-two exponent; MONT: bases then exp|curbit words).
-const std::vector<uint64_t>& fs_program_cached(uint32_t nmin, int path, uint32_t* terms) {
-  static std::mutex mu;
-  static std::map<std::pair<uint32_t, int>, std::pair<uint32_t, std::vector<uint64_t>>> cache;
-  std::lock_guard<std::mutex> lk(mu);
-  auto key = std::make_pair(nmin, path);
-  auto it = cache.find(key);
-  if (it == cache.end()) {
-    if (cache.size() > 256) cache.clear();
-    const int64_t T = fs_setup_program(nmin, path, nullptr);
-    std::vector<uint64_t> w((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1, 0);
-    fs_setup_program(nmin, path, w.data());
-    it = cache.emplace(key, std::make_pair((uint32_t)T, std::move(w))).first;
-  }
-  *terms = it->second.first;
-  return it->second.second;
-}
-
 namespace {
 template <class T>
 int grow(T** p, size_t want, const char* what) {
@@ -232,7 +213,7 @@ struct Plan {
   int status = FS_OK;
   int path = 0;
   uint32_t terms = 0;
-  const std::vector<uint64_t>* words = nullptr;
+  std::shared_ptr<const fs_program> prog;
   uint32_t cap = 0;
   uint32_t pg_blocks = 0;
   double cost = 0;
@@ -265,7 +246,7 @@ int run_batch(fs_batch_bufs& E, const fs_job* jobs, uint32_t nj, std::vector<Pla
     b.path = (uint32_t)plan[j].path;
     b.prog_off = (uint32_t)words.size();
     b.prog_terms = plan[j].terms;
-    words.insert(words.end(), plan[j].words->begin(), plan[j].words->end());
+    words.insert(words.end(), plan[j].prog->words.begin(), plan[j].prog->words.end());
     while (words.size() % 8) words.push_back(0);  // 64-byte aligned programs
     b.cand_off = cand_off;
     b.cand_cap = plan[j].cap;
@@ -305,7 +286,12 @@ int plan_jobs(const fs_job* jobs, uint32_t nj, const fs_batch_options& o, std::v
       continue;
     }
     P.path = fs_select_path(J.pmax);
-    P.words = &fs_program_cached(J.nmin, P.path, &P.terms);
+    P.prog = fs_program_get(J.nmin, P.path);
+    if (!P.prog) {
+      P.status = FS_E_NOMEM;
+      continue;
+    }
+    P.terms = P.prog->terms;
     const uint64_t c = fs_cand_capacity(J.pmin, J.pmax - J.pmin);
     if (c > 0x7fffffffull) {
       fs_set_error_msg("batch: job p-range too large for one batch");
@@ -322,19 +308,27 @@ int plan_jobs(const fs_job* jobs, uint32_t nj, const fs_batch_options& o, std::v
 }  // namespace
 
 // Verify, filter and fold the factors of one job; returns kept factors.
-static int finish_job(std::vector<fs_factor>& f, int threads, uint64_t* checksum, uint64_t* kept) {
-  for (auto& x : f) x.pad = 0;
-  fs_sort_factors(f.data(), f.size());
-  const int64_t bad = fs_verify_all(f.data(), f.size(), threads);
-  if (bad >= 0) {
-    char buf[160];
-    snprintf(buf, sizeof buf, "CPU factor verification failed for p=%llu", (unsigned long long)f[bad].p);
-    fs_set_error_msg(buf);
-    return FS_E_VERIFY;
+// Sorts each listed job's factors and verifies all of them in one pass over
+// the CPU threads (factors of different jobs share the pool).  ok[j] = 0 for a
+// job with a factor that fails verification.
+static void verify_jobs(std::vector<std::vector<fs_factor>>& per, const std::vector<uint32_t>& list, int threads,
+                        std::vector<char>& ok) {
+  std::vector<fs_factor> all;
+  for (uint32_t j : list) {
+    for (auto& x : per[j]) x.pad = 0;
+    fs_sort_factors(per[j].data(), per[j].size());
+    all.insert(all.end(), per[j].begin(), per[j].end());
   }
-  *kept = fs_filter_factors(f.data(), f.size(), checksum, false);
-  f.resize(*kept);
-  return FS_OK;
+  if (fs_verify_all(all.data(), all.size(), threads) < 0) return;
+  for (uint32_t j : list) {  // a failure: find the job(s) it belongs to
+    const int64_t bad = fs_verify_all(per[j].data(), per[j].size(), threads);
+    if (bad >= 0) {
+      char buf[160];
+      snprintf(buf, sizeof buf, "CPU factor verification failed for p=%llu", (unsigned long long)per[j][bad].p);
+      fs_set_error_msg(buf);
+      ok[j] = 0;
+    }
+  }
 }
 
 extern "C" int fs_batch(const fs_job* jobs, uint32_t nj, const fs_batch_options* opt, fs_job_result* out,
@@ -382,6 +376,11 @@ extern "C" int fs_batch(const fs_job* jobs, uint32_t nj, const fs_batch_options*
   std::vector<std::vector<fs_factor>> per(nj);
   for (const fs_factor& f : dev.fac)
     if ((uint32_t)f.pad < nj) per[f.pad].push_back(f);
+  std::vector<uint32_t> finished;
+  for (uint32_t j = 0; j < nj; ++j)
+    if (plan[j].status == FS_OK && !(dev.flag[j] & FS_BJF_VALIDATE)) finished.push_back(j);
+  std::vector<char> verified(nj, 1);
+  verify_jobs(per, finished, o.verify_threads, verified);
   uint64_t written = 0;
   for (uint32_t j = 0; j < nj; ++j) {
     fs_job_result& r = out[j];
@@ -397,11 +396,13 @@ extern "C" int fs_batch(const fs_job* jobs, uint32_t nj, const fs_batch_options*
     r.primecount = dev.cnt[j];
     r.steps = (uint64_t)dev.cnt[j] * (jobs[j].nmax - jobs[j].nmin);
     r.raw_factors = per[j].size();
-    uint64_t ck = dev.sum[j], kept = 0;
-    if ((rc = finish_job(per[j], o.verify_threads, &ck, &kept))) {
-      r.status = rc;
+    if (!verified[j]) {
+      r.status = FS_E_VERIFY;
       continue;
     }
+    uint64_t ck = dev.sum[j];
+    const uint64_t kept = fs_filter_factors(per[j].data(), per[j].size(), &ck, false);
+    per[j].resize(kept);
     r.checksum = ck;
     r.factorcount = kept;
     if (per_job_nf) per_job_nf[j] = kept;

@@ -74,8 +74,6 @@ struct fs_batch_bufs {
   void release();
 };
 uint32_t fs_device_cus();
-// Setup program words of (nmin, path), cached; *terms = term count.
-const std::vector<uint64_t>& fs_program_cached(uint32_t nmin, int path, uint32_t* terms);
 // Upload of a job table / tile order / chunks / programs and the launch.
 int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const std::vector<uint32_t>& order,
                         const std::vector<uint32_t>& chunk, const std::vector<uint64_t>& words, uint32_t pg_blocks,

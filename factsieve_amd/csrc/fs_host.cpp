@@ -15,6 +15,10 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <future>
+#include <map>
+#include <memory>
+#include <new>
 #include <string>
 #include <mutex>
 #include <thread>
@@ -113,19 +117,46 @@ std::vector<uint64_t> fs_horner_program(uint32_t nmin, uint64_t bound) {
     emax = std::max(emax, ex[i]);
   }
   const int J = 31 - __builtin_clz(emax);
+  // Each level's primes are packed first-fit-decreasing: a word starts with the
+  // largest unused prime and takes the largest unused prime that still fits,
+  // until none does.  "Largest unused index <= x" is a union-find over the
+  // ascending list (slot k stands for g[k-1]; slot 0 means none).
+  std::vector<uint32_t> g;
+  std::vector<uint32_t> up;
+  auto find = [&](uint32_t x) {
+    while (up[x] != x) {
+      up[x] = up[up[x]];
+      x = up[x];
+    }
+    return x;
+  };
   for (int j = J; j >= 0; --j) {
     if (j != J) prog.push_back(0);
-    uint64_t cur = 1;
-    for (size_t i = 0; i < pr.size(); ++i) {
-      if (!((ex[i] >> j) & 1)) continue;
-      if ((u128)cur * pr[i] >= bound) {
-        prog.push_back(cur);
-        cur = pr[i];
-      } else {
-        cur *= pr[i];
+    g.clear();
+    for (size_t i = 0; i < pr.size(); ++i)
+      if ((ex[i] >> j) & 1) g.push_back(pr[i]);
+    const uint32_t n = (uint32_t)g.size();
+    up.resize(n + 1);
+    for (uint32_t k = 0; k <= n; ++k) up[k] = k;
+    uint32_t lo = 0;  // smallest index that may be unused
+    for (;;) {
+      uint32_t k = find(n);
+      if (k == 0) break;
+      up[k] = k - 1;
+      uint64_t cur = g[k - 1];
+      for (;;) {
+        const uint64_t lim = (bound - 1) / cur;  // cur * q < bound
+        while (lo < n && up[lo + 1] != lo + 1) ++lo;
+        if (lo == n || lim < g[lo]) break;  // not even the smallest unused prime fits
+        const uint32_t pos =
+            lim >= g[n - 1] ? n : (uint32_t)(std::upper_bound(g.begin(), g.begin() + k, lim) - g.begin());
+        k = find(pos);
+        if (k == 0) break;
+        up[k] = k - 1;
+        cur *= g[k - 1];
       }
+      prog.push_back(cur);
     }
-    if (cur > 1) prog.push_back(cur);
   }
   return prog;
 }
@@ -192,6 +223,32 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
   return (int64_t)prog.size();
 }
 
+std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path) {
+  static std::mutex mu;
+  static std::map<std::pair<uint32_t, int>, std::shared_ptr<const fs_program>> cache;
+  const auto key = std::make_pair(nmin, path);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  try {
+    auto pr = std::make_shared<fs_program>();
+    const int64_t T = fs_setup_program(nmin, path, nullptr);
+    if (T < 0) return nullptr;
+    pr->terms = (uint32_t)T;
+    pr->words.assign((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1, 0);
+    fs_setup_program(nmin, path, pr->words.data());
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() >= 256) cache.clear();  // holders keep their own reference
+    cache[key] = pr;
+    return pr;
+  } catch (const std::bad_alloc&) {
+    fs_set_error_msg("setup program: host allocation failed");
+    return nullptr;
+  }
+}
+
 // ============================================================ CPU verifier
 
 // Deterministic primality for odd 64-bit p: strong probable-prime tests to the
@@ -234,25 +291,37 @@ extern "C" int fs_is_prime(uint64_t p) {
 
 // O(n) recomputation of n! mod p; true iff n! = +1 (c = -1) or -1 (c = +1)
 // (verify, verifyprime.c:137-164).
-extern "C" int fs_verify_factor(uint64_t p, uint32_t n, int32_t c) {
-  if (p < 3 || (c != 1 && c != -1)) return 0;
-  uint64_t r;
-  if (n >= p) {
-    r = 0;  // p | n!
-  } else if (p < 0xFFFFFFFFull) {
-    r = 1;
-    for (uint64_t i = 2; i <= n; ++i) r = r * i % p;
-  } else {
-    const uint64_t q = fs_invert(p), one = (0 - p) % p;
-    uint64_t rm = one, im = one;  // Montgomery forms of r = 1 and i = 1
-    for (uint32_t i = 2; i <= n; ++i) {
-      im = fs_add_mod(im, one, p);
-      rm = fs_mont_mul(rm, im, p, q);
-    }
-    r = fs_mont_mul(rm, 1, p, q);
+// lo * (lo+1) * ... * hi mod p for odd p >= 3 and 2 <= lo <= hi < p: plain
+// 64-bit remainders below 2^32, Montgomery products with a Montgomery counter
+// above (the verify loop of verifyprime.c:137-164).
+static uint64_t range_product_mod(uint64_t p, uint64_t lo, uint64_t hi) {
+  if (p < 0xFFFFFFFFull) {
+    uint64_t r = 1;
+    for (uint64_t i = lo; i <= hi; ++i) r = r * i % p;
+    return r;
   }
-  if (c == -1) return r == 1;
-  return r == p - 1;
+  const uint64_t q = fs_invert(p), one = (0 - p) % p, r2 = fs_r2_from_one(one, p, q);
+  uint64_t rm = one, im = fs_mont_mul(lo, r2, p, q);  // Montgomery forms of 1 and lo
+  for (uint64_t i = lo; i <= hi; ++i) {
+    rm = fs_mont_mul(rm, im, p, q);
+    im = fs_add_mod(im, one, p);
+  }
+  return fs_mont_mul(rm, 1, p, q);
+}
+
+// n! mod p (0 when p <= n); p odd, >= 3.
+static uint64_t factorial_mod(uint64_t p, uint32_t n) {
+  if (n >= p) return 0;
+  return n < 2 ? 1 : range_product_mod(p, 2, n);
+}
+
+static bool plus_minus_one(uint64_t r, uint64_t p, int32_t c) { return c == -1 ? r == 1 : r == p - 1; }
+
+// An even p never divides n! +- 1 (odd for n >= 2; 2 for n < 2 and c = +1,
+// which p > 2 cannot divide).
+extern "C" int fs_verify_factor(uint64_t p, uint32_t n, int32_t c) {
+  if (p < 3 || (p & 1) == 0 || (c != 1 && c != -1)) return 0;
+  return plus_minus_one(factorial_mod(p, n), p, c);
 }
 
 // Number parsing of the -p/-P/-n/-N/-v options with the suffixes of
@@ -412,30 +481,72 @@ void fs_sort_factors(fs_factor* f, size_t n) { std::stable_sort(f, f + n, factor
 
 // Verify every factor on `threads` CPU threads; returns the index of the first
 // failure or -1 (the OpenMP loop of cl_sieve.cpp:497-517).
+// The reference verifies the factors in an OpenMP loop, one factor per
+// iteration (cl_sieve.cpp:491-515).  Here each factor's product 2..n is cut
+// into ranges of 2^15 multipliers, the ranges of all factors share the
+// threads, and each factor's partial products are then multiplied together:
+// a few large-n factors still use every thread.
 int64_t fs_verify_all(const fs_factor* f, size_t n, int threads) {
   if (n == 0) return -1;
+  struct Range {
+    size_t i;
+    uint64_t lo, hi;
+  };
+  const uint64_t kRange = 1u << 15;
+  std::vector<Range> rg;
+  std::vector<size_t> first(n + 1);
+  std::vector<char> direct(n, 0);  // decided without a product
+  for (size_t i = 0; i < n; ++i) {
+    first[i] = rg.size();
+    const uint64_t p = f[i].p;
+    const uint32_t fn = (uint32_t)(f[i].nc < 0 ? -(int64_t)f[i].nc : f[i].nc);
+    if (p < 3 || (p & 1) == 0 || fn >= p || fn < 2) {
+      direct[i] = 1;
+      continue;
+    }
+    for (uint64_t lo = 2; lo <= fn; lo += kRange) rg.push_back({i, lo, std::min<uint64_t>(lo + kRange - 1, fn)});
+  }
+  first[n] = rg.size();
+  std::vector<uint64_t> part(rg.size());
   if (threads < 1) threads = 1;
-  if ((size_t)threads > n) threads = (int)n;
-  std::atomic<int64_t> bad(-1);
+  if ((size_t)threads > rg.size()) threads = (int)std::max<size_t>(rg.size(), 1);
   std::atomic<size_t> next(0);
   auto work = [&]() {
     for (;;) {
-      const size_t i = next.fetch_add(1);
-      if (i >= n) break;
-      const uint32_t fn = (uint32_t)(f[i].nc < 0 ? -f[i].nc : f[i].nc);
-      const int32_t fc = f[i].nc < 0 ? -1 : 1;
-      if (!fs_verify_factor(f[i].p, fn, fc)) {
-        int64_t cur = bad.load();
-        while ((cur < 0 || (int64_t)i < cur) && !bad.compare_exchange_weak(cur, (int64_t)i)) {
-        }
-      }
+      const size_t t = next.fetch_add(1);
+      if (t >= rg.size()) break;
+      part[t] = range_product_mod(f[rg[t].i].p, rg[t].lo, rg[t].hi);
     }
   };
   std::vector<std::thread> pool;
   for (int t = 1; t < threads; ++t) pool.emplace_back(work);
   work();
   for (auto& th : pool) th.join();
-  return bad.load();
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t p = f[i].p;
+    const int32_t fc = f[i].nc < 0 ? -1 : 1;
+    bool ok;
+    if (direct[i]) {
+      const uint32_t fn = (uint32_t)(f[i].nc < 0 ? -(int64_t)f[i].nc : f[i].nc);
+      ok = fs_verify_factor(p, fn, fc) != 0;
+    } else {
+      uint64_t r = 1;
+      for (size_t t = first[i]; t < first[i + 1]; ++t) r = (uint64_t)((u128)r * part[t] % p);
+      ok = plus_minus_one(r, p, fc);
+    }
+    if (!ok) return (int64_t)i;
+  }
+  return -1;
+}
+
+extern "C" int64_t fs_verify_factors(const fs_factor* f, uint64_t n, int threads) {
+  if (!f && n) return FS_E_ARG;
+  try {
+    return fs_verify_all(f, (size_t)n, threads);
+  } catch (const std::exception&) {
+    fs_set_error_msg("fs_verify_factors: host allocation or thread creation failed");
+    return FS_E_NOMEM;
+  }
 }
 
 // Filter composites (2-PRP p) and fold kept factors into the checksum
@@ -686,6 +797,14 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     return FS_E_ARG;
   }
   r.path = path;
+  // The setup program is built on another host thread while the checkpoint
+  // state, the stream and the buffers are prepared.
+  std::future<std::shared_ptr<const fs_program>> prog_fut;
+  try {
+    prog_fut = std::async(std::launch::async, [nmin, path] { return fs_program_get(nmin, path); });
+  } catch (const std::system_error&) {
+    prog_fut = std::async(std::launch::deferred, [nmin, path] { return fs_program_get(nmin, path); });
+  }
 
   WorkStatus st;
   memset(&st, 0, sizeof st);
@@ -746,11 +865,10 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
 
   // setup program and the reference power table (verifypow, and setup on MONT)
   {
-    const int64_t T = fs_setup_program(nmin, path, nullptr);
-    if (T < 0) return (int)T;
-    std::vector<uint64_t> w((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1);
-    fs_setup_program(nmin, path, w.data());
-    d.prog_terms = (uint32_t)T;
+    const std::shared_ptr<const fs_program> pr = prog_fut.get();
+    if (!pr) return FS_E_NOMEM;
+    const std::vector<uint64_t>& w = pr->words;
+    d.prog_terms = pr->terms;
     if ((rc = fs_malloc(&d.prog, w.size() * 8)) || (rc = fs_memcpy_htod(d.prog, w.data(), w.size() * 8, d.s, 1)))
       return rc;
     const int64_t m = fs_power_table(nmin, nullptr, nullptr, nullptr);

@@ -3,6 +3,7 @@
 #pragma once
 #include <stddef.h>
 #include <stdint.h>
+#include <memory>
 #include <vector>
 #include "../../include/factsieve.h"
 
@@ -12,6 +13,13 @@ extern "C" void fs_set_error_msg(const char* msg);
 std::vector<uint32_t> fs_primes_upto(uint32_t limit);
 // Horner program of nmin!: multipliers < bound, 0 = square.
 std::vector<uint64_t> fs_horner_program(uint32_t nmin, uint64_t bound);
+// Setup program words of (nmin, path) as fs_setup_program writes them, from a
+// small shared cache; holders keep their program alive past cache eviction.
+struct fs_program {
+  uint32_t terms;
+  std::vector<uint64_t> words;
+};
+std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path);
 // Sort by (p, |n|); verify on CPU threads (index of first failure or -1);
 // drop composite p and fold kept factors into *checksum (returns kept count).
 void fs_sort_factors(fs_factor* f, size_t n);

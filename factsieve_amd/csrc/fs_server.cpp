@@ -36,7 +36,7 @@ struct WU {
   int state = 0;  // 0 queued, 1 active, 2 done
   int path = 0;
   uint32_t terms = 0;
-  const std::vector<uint64_t>* words = nullptr;
+  std::shared_ptr<const fs_program> prog;
   uint32_t cap = 0, off = 0, slot = 0;
   uint32_t nnext = 0;
   bool fresh = false;
@@ -172,6 +172,16 @@ extern "C" int fs_server_step(fs_server h) {
   while (!s->queue.empty()) {
     const int64_t id = s->queue.front();
     WU& w = s->wus[id];
+    if (!w.prog && !(w.prog = fs_program_get(w.job.nmin, w.path))) {
+      memset(&w.res, 0, sizeof w.res);  // host allocation failed: the workunit ends with an error
+      w.res.status = FS_E_NOMEM;
+      w.res.path = w.path;
+      w.state = 2;
+      ++s->done;
+      s->queue.pop_front();
+      continue;
+    }
+    w.terms = w.prog->terms;
     auto slot = std::find(s->slots.begin(), s->slots.end(), (int64_t)-1);
     if (slot == s->slots.end()) break;
     uint32_t off;
@@ -183,7 +193,6 @@ extern "C" int fs_server_step(fs_server h) {
     w.state = 1;
     w.fresh = true;
     w.nnext = w.job.nmin;
-    w.words = &fs_program_cached(w.job.nmin, w.path, &w.terms);
   }
   // 2. one quantum over all active workunits
   const uint32_t nj = (uint32_t)s->slots.size();
@@ -205,7 +214,7 @@ extern "C" int fs_server_step(fs_server h) {
     b.prog_off = (uint32_t)words.size();
     b.prog_terms = w.terms;
     if (w.nnext == w.job.nmin) {  // setup runs in this quantum
-      words.insert(words.end(), w.words->begin(), w.words->end());
+      words.insert(words.end(), w.prog->words.begin(), w.prog->words.end());
       while (words.size() % 8) words.push_back(0);
     }
     b.cand_off = w.off;

@@ -183,6 +183,10 @@ int fs_select_path(uint64_t pmax);
 /* ------------------------------------------------ CPU verifier (verifyprime.c) */
 int fs_is_prime(uint64_t p);                              /* isPrime, verifyprime.c:96-132 */
 int fs_verify_factor(uint64_t p, uint32_t n, int32_t c);  /* verify, verifyprime.c:137-164 */
+/* The OpenMP verify loop over a factor list (cl_sieve.cpp:491-515) on `threads`
+ * CPU threads; the sign of nc gives c (negative: n!-1).  Returns the index of
+ * the first factor that fails, -1 if all pass, or a negative FS_E_* below -1. */
+int64_t fs_verify_factors(const fs_factor* f, uint64_t n, int threads);
 int fs_parse_uint64(uint64_t* result, const char* str, uint64_t lo, uint64_t hi); /* putil.c:33-115 */
 
 /* ----------------------------- host execution of the kernel arithmetic */

@@ -119,6 +119,35 @@ def test_is_prime_and_verify_match_oracle(fs):
                 assert fs.verify_factor(p, n, c) == bool(O.ora_verify(p, n, c))
 
 
+def test_verify_factor_list_matches_single(fs):
+    """The threaded list verifier (ranges of each factor's n! on all threads)
+    agrees with the one-factor verifier: real factors from the golden C1 list
+    and KAT3-like small p, each with a planted wrong twin; n across several
+    2^15 ranges; n >= p; even p; p above 2^32."""
+    O = ol.oracle()
+    good = [tuple(f) for f in json.load(open(os.path.join(GOLDEN, "c1.json")))["factors"]]
+    rng = random.Random(9)
+    cases = list(good)
+    for p, nc in good[:10]:
+        cases += [(p, -nc), (p, nc + (1 if nc > 0 else -1))]
+    # (p-1)! = -1 and (p-2)! = +1 (mod p): Wilson, n up to ~2e5 (7 ranges)
+    for p in [100003, 199999, 131071, 65537, 4294967311]:
+        if p < 300000:
+            cases += [(p, p - 1), (p, -(p - 2)), (p, -(p - 1)), (p, p - 2)]
+    cases += [(1000003, 2000000), (1000003, -5), (100, 7), (4294967311, 70000), (2 ** 61 - 1, -100000)]
+    rng.shuffle(cases)
+    want = [fs.verify_factor(p, abs(nc), -1 if nc < 0 else 1) for p, nc in cases]
+    for i, (p, nc) in enumerate(cases):
+        if p < 2 ** 63 and p % 2 and abs(nc) < 300000:
+            assert want[i] == bool(O.ora_verify(p, abs(nc), -1 if nc < 0 else 1)), (p, nc)
+    first_bad = want.index(False) if False in want else -1
+    assert fs.verify_factors(cases, threads=8) == first_bad
+    ok = [c for c, w in zip(cases, want) if w]
+    assert len(ok) > 30 and fs.verify_factors(ok, threads=8) == -1
+    assert fs.verify_factors(ok, threads=1) == -1
+    assert fs.verify_factors([], threads=4) == -1
+
+
 def test_parse_matches_oracle(fs):
     O = ol.oracle()
     strs = ["127", "1e8", "4K", "4M", "1k", "1g", "3b62", "0x7f", "12x", "1e99", "99999999999999999999",
