@@ -207,6 +207,59 @@ FS_HD bool fs_spsp2(uint64_t N, uint64_t q, uint64_t one) {
   return false;
 }
 
+// 2^64 mod N for odd N >= 3: 2^64 - N above 2^63; below, 2^63 mod N from one
+// exact signed step 2^33 * 2^30 (valid as 2^33 <= 0.51 N for N >= 2^34), then
+// doubled; a plain 64-bit remainder below 2^34.
+FS_HD uint64_t fs_r64_mod(uint64_t N) {
+  if (N > ((uint64_t)1 << 63)) return 0 - N;
+  if (N < ((uint64_t)1 << 34)) return (0 - N) % N;
+  const double pinv = 1.0 / (double)N;
+  const int64_t h = fs_sh_mulmod((int64_t)1 << 33, 1u << 30, 1073741824.0, N << 31, 0 - N, pinv, pinv * 4294967296.0);
+  const uint64_t r = fs_signed_to_canon(h, N);
+  return fs_add_mod(r, r, N);
+}
+
+// Largest N (exclusive) of the FP64 probable-prime test.
+#define FS_SPRP_FP_MAX ((uint64_t)1 << 47)
+
+// Base-2 strong probable-prime test of odd 5 <= N < 2^47 in FP64 arithmetic
+// (same predicate as fs_spsp2).  Residues are integer doubles |a| <= 0.75 N:
+// a square is fs_fp_mulmod with the multiplier a itself (|a| < 2^48, and
+// a^2/N < 2^46.2 keeps the quotient error below 2^-5, so |a'| <= 0.53 N);
+// a doubling is reduced back to |a| <= N/2.  a == +-1 (mod N) iff |a| == 1.
+FS_HD bool fs_spsp2_fp(uint64_t N) {
+  const double nd = (double)N, ninv = 1.0 / nd, half = 0.5 * nd;
+  uint64_t d = N - 1;
+  const int t = __builtin_ctzll(d);
+  d >>= t;
+  double a = 2.0;
+  if (d > 1) {
+    const int top = 63 - __builtin_clzll(d);
+    for (int b = top - 1; b >= 0; --b) {
+      a = fs_fp_mulmod(a, a, nd, ninv);
+      if ((d >> b) & 1) {
+        a = a + a;
+        a = a > half ? a - nd : (a < -half ? a + nd : a);
+      }
+    }
+  }
+  if (__builtin_fabs(a) == 1.0) return true;
+  for (int s = 1; s < t; ++s) {
+    a = fs_fp_mulmod(a, a, nd, ninv);
+    if (a == -1.0) return true;
+  }
+  return false;
+}
+
+// The candidate test of a sieve survivor N (odd, > 113): base-2 strong
+// probable prime, in FP64 below FS_SPRP_FP_MAX and in Montgomery arithmetic
+// above.
+FS_HD bool fs_candidate_probable_prime(uint64_t N) {
+  if (N < FS_SPRP_FP_MAX) return fs_spsp2_fp(N);
+  const uint64_t q = fs_invert(N);
+  return fs_spsp2(N, q, fs_r64_mod(N));
+}
+
 // Odd primes 3..113 removed before the probable-prime test: 3 and 5 (the
 // reference's mod-30 wheel) and 7..113 (its bit tables, getsegprimes.cl:115-167).
 #define FS_NSMALL 29

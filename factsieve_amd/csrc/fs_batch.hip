@@ -67,24 +67,26 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bprimegen(const fs_bjob* __restric
   __shared__ uint64_t s_tab[FS_NMASK];
   __shared__ uint64_t s_mask[FS_BLOCK];
   __shared__ uint32_t s_excl[FS_BLOCK];
+  __shared__ uint32_t s_wsum[FS_BLOCK / 64 + 1];
   fs_sieve_table_init(s_tab);
   const uint32_t j = job_of_block(jobs, nj, blockIdx.x);
   const fs_bjob J = jobs[j];
   if (!(J.flags & FS_BJ_NEW)) return;  // uniform per workgroup
-  fs_primegen_block(J.pmin | 1, J.pmax, blockIdx.x - J.pg_first, s_tab, s_mask, s_excl,
-                    [&](uint64_t N, uint64_t q, uint64_t r2) {
-                      const uint32_t idx = atomicAdd(&jcnt[j], 1u);
-                      if (idx < J.cand_cap) {
-                        const uint32_t g = J.cand_off + idx;
-                        c.p[g] = N;
-                        c.q[g] = q;
-                        c.r2[g] = r2;
-                        c.res[g] = 0;
-                        c.nnext[g] = J.nmin;
-                      } else {
-                        atomicOr(&jflag[j], FS_BJF_CANDOVF);
-                      }
-                    });
+  fs_primegen_block(
+      J.pmin | 1, J.pmax, blockIdx.x - J.pg_first, s_tab, s_mask, s_excl, s_wsum,
+      [&](uint32_t n) { return atomicAdd(&jcnt[j], n); },
+      [&](uint32_t idx, uint64_t N, uint64_t q, uint64_t r2) {
+        if (idx < J.cand_cap) {
+          const uint32_t g = J.cand_off + idx;
+          c.p[g] = N;
+          c.q[g] = q;
+          c.r2[g] = r2;
+          c.res[g] = 0;
+          c.nnext[g] = J.nmin;
+        } else {
+          atomicOr(&jflag[j], FS_BJF_CANDOVF);
+        }
+      });
 }
 
 // Tile list for the jobs of `order` (cost order), covering n in [n0, n1) of

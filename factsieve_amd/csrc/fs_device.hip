@@ -79,19 +79,22 @@ __global__ __launch_bounds__(FS_BLOCK) void k_getsegprimes(uint64_t base, uint64
   __shared__ uint64_t s_tab[FS_NMASK];
   __shared__ uint64_t s_mask[FS_BLOCK];
   __shared__ uint32_t s_excl[FS_BLOCK];
+  __shared__ uint32_t s_wsum[FS_BLOCK / 64 + 1];
   fs_sieve_table_init(s_tab);
-  fs_primegen_block(base, high, blockIdx.x, s_tab, s_mask, s_excl, [&](uint64_t N, uint64_t q, uint64_t r2) {
-    const uint32_t idx = atomicAdd(&ctr[FS_CTR_COUNT], 1u);
-    if (idx < c.cap) {
-      c.p[idx] = N;
-      c.q[idx] = q;
-      c.r2[idx] = r2;
-      c.res[idx] = 0;
-      c.nnext[idx] = 0;
-    } else {
-      atomicOr(&ctr[FS_CTR_PRIMEOVF], 1u);
-    }
-  });
+  fs_primegen_block(
+      base, high, blockIdx.x, s_tab, s_mask, s_excl, s_wsum,
+      [&](uint32_t n) { return atomicAdd(&ctr[FS_CTR_COUNT], n); },
+      [&](uint32_t idx, uint64_t N, uint64_t q, uint64_t r2) {
+        if (idx < c.cap) {
+          c.p[idx] = N;
+          c.q[idx] = q;
+          c.r2[idx] = r2;
+          c.res[idx] = 0;
+          c.nnext[idx] = 0;
+        } else {
+          atomicOr(&ctr[FS_CTR_PRIMEOVF], 1u);
+        }
+      });
 }
 
 // ---- setup: res <- res * prod(program[start..end)), program word 0 = square

@@ -14,7 +14,7 @@
 
 extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* res) {
   if ((p & 1) == 0 || p < 3) return FS_E_ARG;
-  const uint64_t q = fs_invert(p), one = (0 - p) % p, r2 = fs_r2_from_one(one, p, q);
+  const uint64_t q = fs_invert(p), one = fs_r64_mod(p), r2 = fs_r2_from_one(one, p, q);
   const int64_t T = fs_setup_program(nmin, path, nullptr);
   if (T < 0) return (int)T;
   std::vector<uint64_t> w((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1);
@@ -98,7 +98,7 @@ extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* r
 extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1, uint64_t* res,
                                   fs_factor* fac, uint32_t cap, uint32_t* nf) {
   if ((p & 1) == 0 || p < 3 || n0 > n1 || *res >= p) return FS_E_ARG;
-  const uint64_t q = fs_invert(p), one = (0 - p) % p, r2 = fs_r2_from_one(one, p, q);
+  const uint64_t q = fs_invert(p), one = fs_r64_mod(p), r2 = fs_r2_from_one(one, p, q);
   uint32_t cnt = 0;
   auto rec = [&](int32_t nc) {
     if (cnt < cap) {
@@ -168,8 +168,7 @@ extern "C" int64_t fs_hostemu_candidates(uint64_t low, uint64_t high, uint64_t* 
       surv &= surv - 1;
       const uint64_t N = B + 2 * (uint64_t)bit;
       if (N <= 113) continue;
-      const uint64_t q = fs_invert(N), one = (0 - N) % N;
-      if (fs_spsp2(N, q, one)) {
+      if (fs_candidate_probable_prime(N)) {
         if (out && (uint64_t)n < cap) out[n] = N;
         ++n;
       }

@@ -53,11 +53,29 @@ __device__ __forceinline__ void fs_sieve_table_init(uint64_t* s_tab) {
 // B mod q (the reference's mod-30 wheel plus bit tables, getsegprimes.cl:112-197).
 // Each wave then deals its survivors out round-robin (exclusive prefix of
 // popcounts + select-bit) so all 64 lanes run the base-2 strong probable-prime
-// test (getsegprimes.cl:199-229) on consecutive survivors; passers are handed
-// to app(N, q = N^-1 mod 2^64, r2 = 2^128 mod N).  All threads must call it.
-template <class Append>
+// test (getsegprimes.cl:199-229) on consecutive survivors, keeping one pass bit
+// per round.  The block then reserves room for all its passers with a single
+// reserve(total) (one atomic per block, not per candidate), and each lane
+// stores its passers as store(index, N, q = N^-1 mod 2^64, r2 = 2^128 mod N).
+// All threads must call it.
+__device__ __forceinline__ uint64_t fs_primegen_survivor(uint64_t base, uint64_t tw, uint32_t wbase, uint32_t k,
+                                                         const uint64_t* s_mask, const uint32_t* s_excl) {
+  uint32_t lo = 0, hi = 63;  // owner lane: last L with excl[L] <= k
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (s_excl[wbase + mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  uint64_t m = s_mask[wbase + lo];
+  for (uint32_t j = k - s_excl[wbase + lo]; j > 0; --j) m &= m - 1;
+  const uint32_t bit = __builtin_ctzll(m);
+  return base + (tw + lo) * 128 + 2 * (uint64_t)bit;
+}
+
+template <class Reserve, class Store>
 __device__ __forceinline__ void fs_primegen_block(uint64_t base, uint64_t high, uint64_t blk, const uint64_t* s_tab,
-                                                  uint64_t* s_mask, uint32_t* s_excl, Append app) {
+                                                  uint64_t* s_mask, uint32_t* s_excl, uint32_t* s_wsum,
+                                                  Reserve reserve, Store store) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
   const uint64_t t = blk * FS_BLOCK + threadIdx.x;
@@ -87,21 +105,41 @@ __device__ __forceinline__ void fs_primegen_block(uint64_t base, uint64_t high, 
   s_excl[threadIdx.x] = incl - cnt;
   const uint32_t total = __shfl(incl, 63, 64);
   __syncthreads();
+  // 1. test: round r handles survivor k = lane + 64 r of this wave (r < 64)
   const uint64_t tw = blk * FS_BLOCK + wbase;
-  for (uint32_t k = lane; k < total; k += 64) {
-    uint32_t lo = 0, hi = 63;  // owner lane: last L with excl[L] <= k
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) >> 1;
-      if (s_excl[wbase + mid] <= k) lo = mid;
-      else hi = mid - 1;
+  uint64_t passed = 0;
+  for (uint32_t k = lane, r = 0; k < total; k += 64, ++r) {
+    const uint64_t N = fs_primegen_survivor(base, tw, wbase, k, s_mask, s_excl);
+    if (fs_candidate_probable_prime(N)) passed |= (uint64_t)1 << r;
+  }
+  // 2. one reservation per block; lane offsets from a block-wide prefix
+  const uint32_t np = __popcll(passed);
+  uint32_t pincl = np;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(pincl, o, 64);
+    if (lane >= (uint32_t)o) pincl += v;
+  }
+  if (lane == 63) s_wsum[threadIdx.x >> 6] = pincl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (uint32_t w = 0; w < FS_BLOCK / 64; ++w) {
+      const uint32_t v = s_wsum[w];
+      s_wsum[w] = acc;
+      acc += v;
     }
-    uint64_t m = s_mask[wbase + lo];
-    for (uint32_t j = k - s_excl[wbase + lo]; j > 0; --j) m &= m - 1;
-    const uint32_t bit = __builtin_ctzll(m);
-    const uint64_t N = base + (tw + lo) * 128 + 2 * (uint64_t)bit;
-    const uint64_t q = fs_invert(N);
-    const uint64_t one = (0 - N) % N;
-    if (fs_spsp2(N, q, one)) app(N, q, fs_r2_from_one(one, N, q));
+    s_wsum[FS_BLOCK / 64] = acc ? reserve(acc) : 0;
+  }
+  __syncthreads();
+  uint32_t idx = s_wsum[FS_BLOCK / 64] + s_wsum[threadIdx.x >> 6] + (pincl - np);
+  // 3. store the passers (N recomputed from its round)
+  while (passed) {
+    const uint32_t r = __builtin_ctzll(passed);
+    passed &= passed - 1;
+    const uint64_t N = fs_primegen_survivor(base, tw, wbase, lane + 64 * r, s_mask, s_excl);
+    const uint64_t q = fs_invert(N), one = fs_r64_mod(N);
+    store(idx++, N, q, fs_r2_from_one(one, N, q));
   }
 }
 
