@@ -131,6 +131,7 @@ _SIGS = [
     ("fs_hostemu_iterate", ctypes.c_int, [ctypes.c_int, u64, u32, u32, ctypes.POINTER(u64),
                                           ctypes.POINTER(Factor), u32, ctypes.POINTER(u32)]),
     ("fs_hostemu_candidates", i64, [u64, u64, vp, u64]),
+    ("fs_hostemu_mulmod", ctypes.c_int, [ctypes.c_int, u64, i64, u64, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
     ("fs_default_options", None, [ctypes.POINTER(Options)]),
     ("fs_sieve", ctypes.c_int, [u64, u64, u32, u32, ctypes.POINTER(Options), ctypes.POINTER(Result),
                                 ctypes.POINTER(Factor), u64, ctypes.POINTER(u64)]),
@@ -248,6 +249,13 @@ def hostemu_setup(path, p, nmin):
     r = u64()
     check(lib().fs_hostemu_setup(path, p, nmin, ctypes.byref(r)))
     return r.value
+
+
+def hostemu_mulmod(path, p, s, m):
+    """One residue step s*m mod p of a path on the host: (raw signed value, canonical)."""
+    raw, res = i64(), u64()
+    check(lib().fs_hostemu_mulmod(path, p, s, m, ctypes.byref(raw), ctypes.byref(res)))
+    return raw.value, res.value
 
 
 def hostemu_iterate(path, p, n0, n1, res, cap=64):

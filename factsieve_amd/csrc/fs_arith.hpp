@@ -11,16 +11,20 @@
 //    v_mad_u64_u32 partial products.
 //
 //  * FP64 error-free product with a rounded quotient (p < 2^53): the residue is
-//    a double s with |s| <= 0.75 p; for an integer multiplier m < 2^48,
-//      h = fl(s*m), l = fma(s, m, -h)        (h + l == s*m exactly, |l| <= 2^47)
-//      k = rint(fl(h * (1/p)))               (|k - s*m/p| <= 0.5 + 3*2^-53*0.75*2^48 < 0.571)
+//    a double s with |s| < 0.8 p; for an integer multiplier m < 2^50,
+//      h = fl(s*m), l = fma(s, m, -h)        (h + l == s*m exactly, |l| <= 2^49)
+//      k = rint(fl(h * (1/p)))               (three roundings: |fl(..) - s*m/p|
+//                                             < 3*2^-53 * 0.8*2^50 = 0.3, so
+//                                             |k - s*m/p| < 0.8)
 //      s' = fma(-k, p, h) + l                (exact: h - k*p is an integer of
-//                                             magnitude <= 0.571 p + 2^47 < 2^53)
-//    gives s' == s*m - k*p exactly with |s'| <= 0.571 p.  Every integer below
-//    2^53 (p, residues, n) is an exact double.  Because |s| < p - 1,
+//                                             magnitude < 0.8 p + 2^49 < 2^53)
+//    gives s' == s*m - k*p exactly with |s'| < 0.8 p (iterate's m < 2^31 gives
+//    |s'| <= 0.5 p + 2^-21 p).  Every integer below 2^53 (p, residues, n) is an
+//    exact double.  Because |s| < p - 1 (p > 5),
 //    s == +-1 (mod p) iff |s| == 1.  Seven full-rate FP64 VALU operations.
 //
-//  * Hybrid (p < 2^64 - 2^46, used above 2^53): signed residue |s| <= 0.51p, multiplier
+//  * Hybrid (p <= 2^64 - 2^46, used above 2^53): signed residue |s| <= 0.51p (and
+//    below 2^63: the step keeps |s| <= (1/2 + 2^-19) p), multiplier
 //    m < 2^31; the quotient is an FP64 estimate rounded to nearest, and
 //    s' = s*m - k*p is formed exactly mod 2^64 from 32x32->64 multiply-adds.
 //    The setup program multiplies by a REDC by 2^32 when p < 2^63.
@@ -95,7 +99,7 @@ FS_HD uint64_t fs_r2_from_one(uint64_t one, uint64_t p, uint64_t q) {
 
 // ------------------------------------------------------------------ FP64 path
 
-// s*m mod p for |s| <= 0.75p, integer 0 <= m < 2^48, odd p < 2^53 (see header).
+// s*m mod p for |s| < 0.8p, integer |m| < 2^50, odd 5 < p < 2^53 (see header).
 FS_HD double fs_fp_mulmod(double s, double m, double p, double pinv) {
 #pragma clang fp contract(off)
   const double h = s * m;
@@ -121,16 +125,20 @@ FS_HD double fs_canon_to_fp(uint64_t r, uint64_t p) {
 // Largest modulus of the hybrid path (exclusive bound on pmax is this + 1).
 #define FS_HYBRID_PMAX (0xFFFFFFFFFFFFFFFFull - ((uint64_t)1 << 46) + 1)
 
-// Signed hybrid step: s*m mod p for a signed residue |s| <= 0.51 p, 0 < m < 2^31,
-// 3 <= p <= FS_HYBRID_PMAX; c31 = 2^31 p mod 2^64, pn = 2^64 - p,
+// Signed hybrid step: s*m mod p for an int64 residue |s| <= 0.51 p, 0 < m < 2^31
+// and m < 2p (iterate multiplies by n + 1 <= p whenever s != 0, since p | n! for
+// n >= p), 3 <= p <= FS_HYBRID_PMAX; c31 = 2^31 p mod 2^64, pn = 2^64 - p,
 // pinv = fl(1/p), pinv32 = 2^32 * pinv, md = (double)m.
-// X = s*m/p is estimated as y*m with |error| < 2^-19; k = trunc(y*m + 2^31 + 1/2)
-// is round(X) + 2^31, so r = s*m - (k - 2^31) p has |r| <= (1/2 + 2^-19) p < 2^63
-// and is formed exactly mod 2^64 as s*m + c31 + k*pn from four 32x32->64
-// multiply-adds.  Because |r| < p - 1, r == +-1 (mod p) iff r == +-1.
-// *yq receives y ~ s/p (|y - s/p| < 2^-50 |s/p| + 2^-60), which the iterate
-// loops use as a filter for s == +-1: |y| <= 1.5 pinv holds for s in {-1,0,1}
-// and fails for every |s| >= 2.
+// y = s/p + err with |err| <= 2^-51.9 |s/p| + 2^-21/p (the second term from
+// rounding s0 * pinv, s0 < 2^32), so y*m misses X = s*m/p by at most
+// m*2^-51.9 + (m/p)*2^-21 + 2^-22 (final rounding) < 2^-19 for m < 2p, and
+// k = trunc(y*m + 2^31 + 1/2) is round(X) + 2^31: r = s*m - (k - 2^31) p has
+// |r| <= (1/2 + 2^-19) p < 2^63 and is formed exactly mod 2^64 as
+// s*m + c31 + k*pn from four 32x32->64 multiply-adds.  Because |r| < p - 1,
+// r == +-1 (mod p) iff r == +-1.
+// *yq receives y, which the iterate loops use as a filter for s == +-1: by the
+// error bound above, |y| <= 1.5 pinv holds for s in {-1, 0, 1} and fails for
+// every |s| >= 2.
 FS_HD int64_t fs_sh_mulmod(int64_t s, uint32_t m, double md, uint64_t c31, uint64_t pn, double pinv,
                            double pinv32, double* yq) {
 #pragma clang fp contract(off)
@@ -223,9 +231,9 @@ FS_HD uint64_t fs_r64_mod(uint64_t N) {
 #define FS_SPRP_FP_MAX ((uint64_t)1 << 47)
 
 // Base-2 strong probable-prime test of odd 5 <= N < 2^47 in FP64 arithmetic
-// (same predicate as fs_spsp2).  Residues are integer doubles |a| <= 0.75 N:
-// a square is fs_fp_mulmod with the multiplier a itself (|a| < 2^48, and
-// a^2/N < 2^46.2 keeps the quotient error below 2^-5, so |a'| <= 0.53 N);
+// (same predicate as fs_spsp2).  Residues are integer doubles |a| < 0.8 N:
+// a square is fs_fp_mulmod with the multiplier a itself (|a| < 2^47, and
+// a^2/N < 2^46.4 keeps the quotient error below 2^-5, so |a'| <= 0.53 N);
 // a doubling is reduced back to |a| <= N/2.  a == +-1 (mod N) iff |a| == 1.
 FS_HD bool fs_spsp2_fp(uint64_t N) {
   const double nd = (double)N, ninv = 1.0 / nd, half = 0.5 * nd;

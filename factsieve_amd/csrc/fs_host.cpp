@@ -185,7 +185,7 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
     fs_set_error_msg("fs_setup_program: unknown path");
     return FS_E_ARG;
   }
-  const uint64_t bound = (path == FS_PATH_FP64) ? ((uint64_t)1 << 48) : ((uint64_t)1 << 31);
+  const uint64_t bound = (path == FS_PATH_FP64) ? ((uint64_t)1 << 50) : ((uint64_t)1 << 31);
   const std::vector<uint64_t> horner = fs_horner_program(nmin, bound);
   // Blocks of 4 terms: a square is a flag (bit 63) on the first term of a
   // block, so each group C_j is padded with multipliers 1 to a whole block.
@@ -210,7 +210,7 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
       if (prog[i] & FS_PROG_SQUARE) E = 2 * E + 64;
       E += 32;
       if (path == FS_PATH_FP64) {
-        const double d = (double)m;  // exact: < 2^48
+        const double d = (double)m;  // exact: < 2^50
         uint64_t bits;
         memcpy(&bits, &d, 8);
         words[i] = bits | (prog[i] & FS_PROG_SQUARE);

@@ -12,6 +12,31 @@
 #include "fs_internal.hpp"
 #include "../../include/factsieve.h"
 
+extern "C" int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, int64_t* raw, uint64_t* res) {
+  if ((p & 1) == 0 || p < 7 || !raw || !res) return FS_E_ARG;
+  const uint64_t as = s < 0 ? (uint64_t)0 - (uint64_t)s : (uint64_t)s;
+  if (path == FS_PATH_FP64) {
+    if (p >= ((uint64_t)1 << 53) || m >= ((uint64_t)1 << 50) || (double)as >= 0.8 * (double)p) return FS_E_ARG;
+    const double pd = (double)p;
+    const double r = fs_fp_mulmod((double)s, (double)m, pd, 1.0 / pd);
+    *raw = (int64_t)r;
+    *res = fs_fp_to_canon(r, p);
+    return FS_OK;
+  }
+  if (path == FS_PATH_HYBRID) {
+    if (p > FS_HYBRID_PMAX || m >= ((uint64_t)1 << 31) || (p < ((uint64_t)1 << 31) && m >= 2 * p) ||
+        (double)as > 0.51 * (double)p ||
+        as >= ((uint64_t)1 << 63))
+      return FS_E_ARG;
+    const double pinv = 1.0 / (double)p;
+    const int64_t r = fs_sh_mulmod(s, (uint32_t)m, (double)m, p << 31, 0 - p, pinv, pinv * 4294967296.0);
+    *raw = r;
+    *res = fs_signed_to_canon(r, p);
+    return FS_OK;
+  }
+  return FS_E_ARG;
+}
+
 extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* res) {
   if ((p & 1) == 0 || p < 3) return FS_E_ARG;
   const uint64_t q = fs_invert(p), one = fs_r64_mod(p), r2 = fs_r2_from_one(one, p, q);

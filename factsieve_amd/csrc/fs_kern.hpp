@@ -153,7 +153,7 @@ struct alignas(32) fs_prog_block {
   uint64_t w[4];
 };
 
-// FP64 program (p < 2^53): multipliers < 2^48 as double bits.
+// FP64 program (p < 2^53): multipliers < 2^50 as double bits.
 __device__ __forceinline__ uint64_t fs_cand_setup_fp64(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
                                                        const double* __restrict__ progd, uint32_t start,
                                                        uint32_t end) {

@@ -170,7 +170,7 @@ int fs_verifyresult(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_strea
 int64_t fs_power_table(uint32_t nmin, uint64_t* bases, uint32_t* exps, uint32_t* curbits);
 /* Setup program for `path` in words of 8 bytes; returns the term count T
  * (pass NULL words to get it).  FP64 and HYBRID: T a multiple of 4, in blocks of
- * four multipliers (FP64: doubles < 2^48 as bits; HYBRID: integers < 2^31;
+ * four multipliers (FP64: doubles < 2^50 as bits; HYBRID: integers < 2^31;
  * padding multipliers are 1).  Bit 63 set on a block's first word (FS_PROG_SQUARE
  * in the kernels) squares the running product before that block; evaluating
  * from 1 yields nmin!.  HYBRID has one more word, [T], the total power of 2^-1
@@ -197,6 +197,11 @@ int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* res);
 int fs_hostemu_iterate(int path, uint64_t p, uint32_t startN, uint32_t endN, uint64_t* res,
                        fs_factor* factors, uint32_t cap, uint32_t* nfactors);
 int64_t fs_hostemu_candidates(uint64_t low, uint64_t high, uint64_t* out, uint64_t cap);
+/* One multiply s*m of a path's residue step at the edges of its error bounds:
+ * FP64 needs |s| < 0.8p, m < 2^50, p < 2^53; HYBRID |s| <= 0.51p (int64), m < min(2^31, 2p),
+ * p <= 2^64 - 2^46 (FS_E_ARG otherwise).  *raw = the step's own signed
+ * representation of the result, *res = its canonical value in [0, p). */
+int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, int64_t* raw, uint64_t* res);
 
 /* ------------------------------------------------------- whole sieve */
 typedef struct {

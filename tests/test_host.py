@@ -85,7 +85,7 @@ def test_setup_program_evaluates_to_factorial(fs, path, nmin):
     import struct
     n, words = fs.setup_program(nmin, path)
     assert n % 4 == 0
-    bound = 2 ** 48 if path == 1 else 2 ** 31
+    bound = 2 ** 50 if path == 1 else 2 ** 31
     acc, e = 1, 0
     for i, w in enumerate(words[:n]):
         sq = w >> 63
@@ -212,6 +212,35 @@ def test_host_arithmetic_planted_factor(fs):
             for path in paths:
                 _, facs, _ = fs.hostemu_iterate(path, p, 500, 502, start)
                 assert facs[0] == (p, sign * 500), (p, path)
+
+
+def test_residue_step_at_error_bound_edges(fs):
+    """Single steps at the extremes the error analysis allows: FP64 with
+    p just below 2^53, |s| just below 0.8p and m just below 2^50; hybrid with
+    p at 2^64 - 2^46, |s| up to min(0.51p, 2^63), m up to min(2^31, 2p) - 1.  Results are exact and the raw
+    residues stay inside the stated bounds (|s'| < 0.8p, |s'| <= (1/2 + 2^-19)p)."""
+    rng = random.Random(11)
+    fp_ps = [2 ** 53 - 111, 2 ** 53 - 1, 2 ** 52 + 1, 2 ** 40 + 15, 1000003, 127]
+    for p in fp_ps:
+        smax = int(0.8 * p) - 2
+        ms = [2 ** 50 - 1, 2 ** 50 - 3, 2 ** 49 + 1, 2 ** 31 - 1, 3] + [rng.randrange(1, 2 ** 50) for _ in range(200)]
+        ss = [smax, -smax, 1, -1, 0] + [rng.randrange(-smax, smax + 1) for _ in range(200)]
+        for s, m in zip(ss * 2, ms + ms[::-1]):
+            raw, res = fs.hostemu_mulmod(fs.PATH_FP64, p, s, m)
+            assert res == (s * m) % p, (p, s, m)
+            assert raw % p == res and abs(raw) < 0.8 * p, (p, s, m, raw)
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_mulmod(fs.PATH_FP64, 2 ** 53 - 111, 0, 2 ** 50)
+    hy_ps = [2 ** 64 - 2 ** 46 - 3, 2 ** 63 + 29, 2 ** 62 + 135, 2 ** 53 + 5, 1000003, 127]
+    for p in hy_ps:
+        smax = min(int(0.51 * p) - 2 ** 12, 2 ** 63 - 1) if p > 2 ** 40 else int(0.51 * p)
+        mtop = min(2 ** 31, 2 * p)  # the step needs m < 2p (iterate: n + 1 <= p while s != 0)
+        ms = [mtop - 1, mtop // 2, 3] + [rng.randrange(1, mtop) for _ in range(200)]
+        ss = [smax, -smax, 1, -1] + [rng.randrange(-smax, smax + 1) for _ in range(200)]
+        for s, m in zip(ss, ms):
+            raw, res = fs.hostemu_mulmod(fs.PATH_HYBRID, p, s, m)
+            assert res == (s * m) % p, (p, s, m)
+            assert raw % p == res and abs(raw) <= (0.5 + 2 ** -19) * p, (p, s, m, raw)
 
 
 def test_hybrid_quotient_filter_sound(fs):
