@@ -14,6 +14,7 @@ Reference interface mirrored here (mfl0p/FactSieve):
     the ten kernels       kernels/<name>.cl      -> Segment.<name>() launches
 """
 import ctypes
+import threading
 import os
 from collections import namedtuple
 
@@ -290,11 +291,24 @@ def default_options(**kw):
     return o
 
 
+_tls = threading.local()
+
+
+def _factor_buffer(cap):
+    """Per-thread reusable factor array (results are copied out right after a
+    call), so repeated calls do not pay for a fresh zeroed host buffer."""
+    buf = getattr(_tls, "fac", None)
+    if buf is None or len(buf) < cap:
+        buf = (Factor * cap)()
+        _tls.fac = buf
+    return buf
+
+
 def sieve(pmin, pmax, nmin, nmax, factor_cap=1 << 20, **opts):
     """Sieve p in [pmin, pmax), n in [nmin, nmax) on the GPU (cl_sieve)."""
     o = default_options(**opts)
     r = Result()
-    f = (Factor * factor_cap)()
+    f = _factor_buffer(factor_cap)
     nf = u64()
     check(lib().fs_sieve(pmin, pmax, nmin, nmax, ctypes.byref(o), ctypes.byref(r), f, factor_cap,
                          ctypes.byref(nf)))
@@ -466,7 +480,7 @@ def batch(jobs, factor_cap=1 << 20, **opts):
     o = default_batch_options(**opts)
     arr = _jobs_array(jobs)
     out = (JobResult * max(len(jobs), 1))()
-    fac = (Factor * factor_cap)()
+    fac = _factor_buffer(factor_cap)
     per = (u64 * max(len(jobs), 1))()
     sec = ctypes.c_double()
     check(lib().fs_batch(arr, len(jobs), ctypes.byref(o), out, fac, factor_cap, per, ctypes.byref(sec)))
@@ -505,7 +519,7 @@ class Server:
 
     def poll(self, wid, factor_cap=1 << 16):
         r = JobResult()
-        fac = (Factor * factor_cap)()
+        fac = _factor_buffer(factor_cap)
         nf = u64()
         done = check(lib().fs_server_poll(self.h, wid, ctypes.byref(r), fac, factor_cap, ctypes.byref(nf)))
         if not done:
