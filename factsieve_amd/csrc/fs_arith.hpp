@@ -228,13 +228,14 @@ FS_HD uint64_t fs_r64_mod(uint64_t N) {
 }
 
 // Largest N (exclusive) of the FP64 probable-prime test.
-#define FS_SPRP_FP_MAX ((uint64_t)1 << 47)
+#define FS_SPRP_FP_MAX ((uint64_t)1 << 50)
 
-// Base-2 strong probable-prime test of odd 5 <= N < 2^47 in FP64 arithmetic
+// Base-2 strong probable-prime test of odd 5 <= N < 2^50 in FP64 arithmetic
 // (same predicate as fs_spsp2).  Residues are integer doubles |a| < 0.8 N:
-// a square is fs_fp_mulmod with the multiplier a itself (|a| < 2^47, and
-// a^2/N < 2^46.4 keeps the quotient error below 2^-5, so |a'| <= 0.53 N);
-// a doubling is reduced back to |a| <= N/2.  a == +-1 (mod N) iff |a| == 1.
+// a square is fs_fp_mulmod with the multiplier a itself (|a| < 2^50; a^2/N
+// < 0.64 N keeps the quotient error below 3*2^-53*0.64*2^50 = 0.24, so
+// |a'| < 0.74 N, and |h - k N| < 0.74 N + 2^46 < 2^53); a doubling is reduced
+// back to |a| <= N/2.  a == +-1 (mod N) iff |a| == 1.
 FS_HD bool fs_spsp2_fp(uint64_t N) {
   const double nd = (double)N, ninv = 1.0 / nd, half = 0.5 * nd;
   uint64_t d = N - 1;

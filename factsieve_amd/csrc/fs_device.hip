@@ -13,6 +13,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <string>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include "fs_arith.hpp"
@@ -172,19 +173,30 @@ __global__ void k_fill_ntab(double* t, uint32_t first, uint32_t count) {
 
 __global__ __launch_bounds__(FS_BLOCK) void k_check(fs_cands c, uint32_t* ctr, uint64_t* sum,
                                                     uint32_t nmax) {
+  // Grid-stride over the live candidates; the wrapping sum is reduced per
+  // block and added with one atomic per block (the reference keeps one slot
+  // per workgroup, check.cl:84-98).
+  __shared__ uint64_t s_part[FS_BLOCK / 64];
   const uint32_t live = fs_live(ctr, c.cap);
-  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   uint64_t v = 0;
-  if (i < live) {
+  bool bad = false;
+  for (uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x; i < live; i += gridDim.x * FS_BLOCK) {
     bool ok;
-    v = fs_cand_fold(c.p[i], c.q[i], c.r2[i], c.res[i], nmax, ok);
-    if (!ok || c.nnext[i] != nmax) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
+    v += fs_cand_fold(c.p[i], c.q[i], c.r2[i], c.res[i], nmax, ok);
+    bad |= !ok || c.nnext[i] != nmax;
   }
+  if (bad) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
   v = fs_wave_sum(v);
-  if ((threadIdx.x & 63) == 0 && v != 0) atomicAdd((unsigned long long*)&sum[1], (unsigned long long)v);
-  if (i == 0) {
-    atomicAdd((unsigned long long*)&sum[0], (unsigned long long)live);
-    atomicMax(&ctr[FS_CTR_MAXCOUNT], ctr[FS_CTR_COUNT]);
+  if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (uint32_t w = 0; w < FS_BLOCK / 64; ++w) t += s_part[w];
+    if (t) atomicAdd((unsigned long long*)&sum[1], (unsigned long long)t);
+    if (blockIdx.x == 0) {
+      atomicAdd((unsigned long long*)&sum[0], (unsigned long long)live);
+      atomicMax(&ctr[FS_CTR_MAXCOUNT], ctr[FS_CTR_COUNT]);
+    }
   }
 }
 
@@ -652,7 +664,7 @@ extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_facto
 extern "C" int fs_check(const fs_cands* c, uint32_t* d_ctr, uint64_t* d_sum, uint32_t nmax, fs_stream s) {
   int rc = check_cands(c, "check");
   if (rc) return rc;
-  k_check<<<cand_grid(c), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_sum, nmax);
+  k_check<<<std::min<uint32_t>(cand_grid(c), 2048), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_sum, nmax);
   LAUNCHCHK("check");
   if (g_inject == FS_CTR_VALIDATE) {
     k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_VALIDATE);

@@ -265,6 +265,7 @@ def test_hybrid_quotient_filter_sound(fs):
                                    (2 ** 62, 2 ** 62 + 100000), (2 ** 64 - 300000, 2 ** 64 - 1), (1000, 1001),
                                    (127, 128), (2 ** 64 - 2, 2 ** 64 - 1), (2 ** 34 - 100000, 2 ** 34 + 100000),
                                    (2 ** 47 - 100000, 2 ** 47 + 100000), (2 ** 63 - 100000, 2 ** 63 + 100000),
+                                   (2 ** 50 - 200000, 2 ** 50 + 100000),
                                    (10 ** 5, 10 ** 6)])
 def test_candidate_masks_match_oracle(fs, lo, hi):
     assert fs.hostemu_candidates(lo, hi) == ol.candidates(lo, hi)
