@@ -46,10 +46,11 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine
 OPS_PER_STEP = {1: 7, 2: 12, 3: 45}             # VALU lane-ops per p*n step by path (from the ISA)
 INT32_MUL_PER_STEP = 11                         # 64-bit Montgomery step: 4 + 3 + 4 partial products
 INT32_MUL_PEAK = VALU_PEAK_TOPS * 1e12          # full-rate v_mad_u64_u32 / v_mul_*_u32 (measured)
-# HBM bytes per candidate per k_iterate_fp64 launch, measured with rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE on the c2 workload (23.2 B read + 12.0 B written),
-# each scaled by the counter's reading for a 1 GiB device copy (x2.0 fetch, x1.0 write).
-TRAFFIC_BYTES_PER_CAND = {1: 35.2}
+# HBM bytes per candidate per iterate launch, measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE (k_iterate_fp64 on c2: 23.2 B read + 12.0 B written; k_iterate_hybrid
+# on c3: 20.4 + 12.0), each scaled by the counter's reading for a 1 GiB device copy
+# (x2.0 fetch, x1.0 write).
+TRAFFIC_BYTES_PER_CAND = {1: 35.2, 2: 32.4}
 
 
 def cpu_baseline(pmin, nmin, nmax, target_s=15.0):
