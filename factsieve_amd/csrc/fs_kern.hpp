@@ -160,20 +160,43 @@ __device__ __forceinline__ uint64_t fs_cand_setup_fp64(uint64_t p, uint64_t q, u
   const fs_prog_block* __restrict__ prog = (const fs_prog_block*)progd;
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = (start == 0) ? 1.0 : fs_canon_to_fp(s_in, p);
+  auto square = [&](const fs_prog_block& blk) {  // first use of a block: waits for its load
+    if (blk.w[0] & FS_PROG_SQUARE) {
+      const uint64_t a = fs_fp_to_canon(s, p);
+      s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+    }
+  };
+  auto terms4 = [&](const fs_prog_block& blk) {
+    s = fs_fp_mulmod(s, __longlong_as_double(blk.w[0] & ~FS_PROG_SQUARE), pd, pinv);
+    s = fs_fp_mulmod(s, __longlong_as_double(blk.w[1]), pd, pinv);
+    s = fs_fp_mulmod(s, __longlong_as_double(blk.w[2]), pd, pinv);
+    s = fs_fp_mulmod(s, __longlong_as_double(blk.w[3]), pd, pinv);
+  };
+  // Scalar loads complete out of order, so any wait is for all of them: a
+  // block's load is issued right after the previous block's first use (its
+  // wait) and overlaps that block's four terms.  The empty asm (a memory
+  // clobber that also claims to update s) pins each load there: the load
+  // cannot sink below it and the terms cannot rise above it.
   const uint32_t b0 = start >> 2, b1 = end >> 2;
   if (b0 < b1) {
-    fs_prog_block cur = prog[b0];
-    for (uint32_t b = b0; b < b1; ++b) {
-      const fs_prog_block nxt = prog[b + 1 < b1 ? b + 1 : b];
-      if (cur.w[0] & FS_PROG_SQUARE) {
-        const uint64_t a = fs_fp_to_canon(s, p);
-        s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
-      }
-      s = fs_fp_mulmod(s, __longlong_as_double(cur.w[0] & ~FS_PROG_SQUARE), pd, pinv);
-      s = fs_fp_mulmod(s, __longlong_as_double(cur.w[1]), pd, pinv);
-      s = fs_fp_mulmod(s, __longlong_as_double(cur.w[2]), pd, pinv);
-      s = fs_fp_mulmod(s, __longlong_as_double(cur.w[3]), pd, pinv);
-      cur = nxt;
+    fs_prog_block x = prog[b0];
+    uint32_t b = b0;
+    for (; b + 2 < b1; b += 2) {
+      square(x);
+      const fs_prog_block y = prog[b + 1];
+      asm volatile("" : "+v"(s)::"memory");
+      terms4(x);
+      square(y);
+      x = prog[b + 2];
+      asm volatile("" : "+v"(s)::"memory");
+      terms4(y);
+    }
+    square(x);
+    terms4(x);
+    if (b + 1 < b1) {
+      const fs_prog_block y = prog[b + 1];
+      square(y);
+      terms4(y);
     }
   }
   return fs_fp_to_canon(s, p);
@@ -284,7 +307,9 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in,
   const uint32_t n4 = n0 + ((n1 - n0) & ~3u);
   double m0 = nt[0], m1 = nt[1], m2 = nt[2], m3 = nt[3];
   for (; n < n4; n += 4) {
-    const double* mn = nt + (n - n0) + 4;  // next block, loaded one iteration ahead
+    // next block, loaded one iteration ahead (left to the compiler's placement:
+    // pinning it as in the hybrid loop measured slower here)
+    const double* mn = nt + (n - n0) + 4;
     const double x0 = mn[0], x1 = mn[1], x2 = mn[2], x3 = mn[3];
     const double a = s;
     const double b = fs_fp_mulmod(a, m0, pd, pinv);
@@ -321,16 +346,24 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_i
   int64_t s = fs_canon_to_signed(s_in, p);
   uint32_t n = n0;
   const uint32_t n4 = n0 + ((n1 - n0) & ~3u);
+  double m0 = nt[0], m1 = nt[1], m2 = nt[2], m3 = nt[3];
   for (; n < n4; n += 4) {
-    const double* m = nt + (n - n0);
+    // Scalar loads complete out of order (a wait is for all of them): take
+    // this block's multipliers first, then issue the next block's load, pinned
+    // above this block's steps by the asm that claims to update s.
+    asm volatile("" ::"s"(m0), "s"(m1), "s"(m2), "s"(m3));
+    const double* mn = nt + (n - n0) + 4;
+    const double x0 = mn[0], x1 = mn[1], x2 = mn[2], x3 = mn[3];
+    asm volatile("" : "+v"(s)::"memory");
     uint32_t mm = n + 1;
     asm volatile("" : "+s"(mm));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
     double ya, yb, yd, ye;
     const int64_t a = s;
-    const int64_t b = fs_sh_mulmod(a, mm, m[0], c31, pn, pinv, pinv32, &ya);
-    const int64_t d = fs_sh_mulmod(b, mm + 1, m[1], c31, pn, pinv, pinv32, &yb);
-    const int64_t e = fs_sh_mulmod(d, mm + 2, m[2], c31, pn, pinv, pinv32, &yd);
-    s = fs_sh_mulmod(e, mm + 3, m[3], c31, pn, pinv, pinv32, &ye);
+    const int64_t b = fs_sh_mulmod(a, mm, m0, c31, pn, pinv, pinv32, &ya);
+    const int64_t d = fs_sh_mulmod(b, mm + 1, m1, c31, pn, pinv, pinv32, &yb);
+    const int64_t e = fs_sh_mulmod(d, mm + 2, m2, c31, pn, pinv, pinv32, &yd);
+    s = fs_sh_mulmod(e, mm + 3, m3, c31, pn, pinv, pinv32, &ye);
+    m0 = x0, m1 = x1, m2 = x2, m3 = x3;
     const double ymin = __builtin_fmin(__builtin_fmin(__builtin_fabs(ya), __builtin_fabs(yb)),
                                        __builtin_fmin(__builtin_fabs(yd), __builtin_fabs(ye)));
     if (ymin <= thr) {
