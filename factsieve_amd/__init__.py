@@ -50,6 +50,12 @@ class Options(ctypes.Structure):
                 ("checkpoint_secs", u32), ("workdir", ctypes.c_char_p), ("stop_after_segments", u64)]
 
 
+class WorkStatus(ctypes.Structure):
+    """fs_work_status: the 72-byte checkpoint record (workStatus)."""
+    _fields_ = [("pmin", u64), ("pmax", u64), ("p", u64), ("checksum", u64), ("primecount", u64),
+                ("factorcount", u64), ("last_trickle", u64), ("state_sum", u64), ("nmin", u32), ("nmax", u32)]
+
+
 class Result(ctypes.Structure):
     _fields_ = [("factorcount", u64), ("primecount", u64), ("checksum", u64), ("raw_factors", u64),
                 ("steps", u64), ("setup_mulmods", u64), ("seconds", ctypes.c_double),
@@ -127,6 +133,8 @@ _SIGS = [
     ("fs_is_prime", ctypes.c_int, [u64]),
     ("fs_verify_factor", ctypes.c_int, [u64, u32, i32]),
     ("fs_verify_factors", i64, [ctypes.POINTER(Factor), u64, ctypes.c_int]),
+    ("fs_read_checkpoint", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(WorkStatus), ctypes.POINTER(ctypes.c_int)]),
+    ("fs_write_checkpoint", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(WorkStatus), ctypes.c_int]),
     ("fs_parse_uint64", ctypes.c_int, [ctypes.POINTER(u64), ctypes.c_char_p, u64, u64]),
     ("fs_hostemu_setup", ctypes.c_int, [ctypes.c_int, u64, u32, ctypes.POINTER(u64)]),
     ("fs_hostemu_iterate", ctypes.c_int, [ctypes.c_int, u64, u32, u32, ctypes.POINTER(u64),
@@ -215,6 +223,20 @@ def verify_factors(factors, threads=16):
     if r < -1:
         raise FactSieveError(r, last_error())
     return r
+
+
+def read_checkpoint(directory, pmin, pmax, nmin, nmax):
+    """read_state: (WorkStatus, 0 for stateA / 1 for stateB) or None."""
+    st = WorkStatus(pmin=pmin, pmax=pmax, nmin=nmin, nmax=nmax)
+    which = ctypes.c_int(-1)
+    r = check(lib().fs_read_checkpoint(str(directory).encode(), ctypes.byref(st), ctypes.byref(which)))
+    return (st, which.value) if r == 1 else None
+
+
+def write_checkpoint(directory, st, which):
+    """write_state into stateA.ckp (which=0) or stateB.ckp (which=1); sets st.state_sum."""
+    check(lib().fs_write_checkpoint(str(directory).encode(), ctypes.byref(st), which))
+    return st
 
 
 def parse_uint64(s, lo, hi):

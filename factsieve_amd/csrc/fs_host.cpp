@@ -395,10 +395,7 @@ extern "C" void fs_default_options(fs_options* o) {
 
 // Checkpoint record, byte-identical to workStatus (cl_sieve.h:9-12): 8 x u64 then
 // 2 x u32, 72 bytes, state_sum = wrapping sum of the other fields.
-struct WorkStatus {
-  uint64_t pmin, pmax, p, checksum, primecount, factorcount, last_trickle, state_sum;
-  uint32_t nmin, nmax;
-};
+typedef fs_work_status WorkStatus;
 static_assert(sizeof(WorkStatus) == 72, "workStatus layout");
 
 static uint64_t state_sum(const WorkStatus& s) {
@@ -413,21 +410,35 @@ static std::string in_dir(const char* dir, const char* name) {
   return s + name;
 }
 
-// write_state (cl_sieve.cpp:100-129): alternate stateA.ckp / stateB.ckp.
-static void write_state(WorkStatus& st, bool& write_a_next, const char* dir) {
+// write_state (cl_sieve.cpp:100-129): stateA.ckp / stateB.ckp.
+static bool write_state_file(WorkStatus& st, bool file_a, const char* dir) {
   st.state_sum = state_sum(st);
-  const std::string fn = in_dir(dir, write_a_next ? "stateA.ckp" : "stateB.ckp");
+  const std::string fn = in_dir(dir, file_a ? "stateA.ckp" : "stateB.ckp");
   FILE* f = fopen(fn.c_str(), "wb");
   if (!f) {
     fprintf(stderr, "Cannot open %s !!!\n", fn.c_str());
-    return;
+    return false;
   }
   if (fwrite(&st, sizeof st, 1, f) != 1) {
     fprintf(stderr, "Cannot write checkpoint to file. Continuing...\n");
     fclose(f);
-    return;
+    return false;
   }
-  if (fclose(f) == 0) write_a_next = !write_a_next;
+  return fclose(f) == 0;
+}
+
+// Alternates the two files, as the reference's driver does.
+static void write_state(WorkStatus& st, bool& write_a_next, const char* dir) {
+  if (write_state_file(st, write_a_next, dir)) write_a_next = !write_a_next;
+}
+
+extern "C" int fs_write_checkpoint(const char* dir, fs_work_status* st, int which) {
+  if (!st || (which != 0 && which != 1)) return FS_E_ARG;
+  if (!write_state_file(*st, which == 0, dir)) {
+    fs_set_error_msg("cannot write checkpoint file");
+    return FS_E_IO;
+  }
+  return FS_OK;
 }
 
 // read_state (cl_sieve.cpp:132-222): a file is good when it parses, matches the
@@ -468,6 +479,16 @@ static int read_state(WorkStatus& st, bool& write_a_next, const char* dir, bool 
     }
   }
   return 0;
+}
+
+extern "C" int fs_read_checkpoint(const char* dir, fs_work_status* st, int* which) {
+  if (!st) return FS_E_ARG;
+  bool write_a_next = true;
+  WorkStatus w = *st;
+  if (!read_state(w, write_a_next, dir, false)) return 0;
+  *st = w;
+  if (which) *which = write_a_next ? 1 : 0;  // reading B makes A the next file written
+  return 1;
 }
 
 // Sort key of the drained factor list: p, then |n| (factorcompare, cl_sieve.cpp:422-436).

@@ -245,6 +245,23 @@ int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax, const f
  * number of passing cases (5 = all). */
 int fs_self_test(const fs_options* opt, int* passed_mask);
 
+/* Checkpoint record, byte-identical to workStatus (cl_sieve.h:9-12): 72 bytes,
+ * state_sum = wrapping sum of the other nine fields. */
+typedef struct {
+  uint64_t pmin, pmax, p, checksum, primecount, factorcount, last_trickle, state_sum;
+  uint32_t nmin, nmax;
+} fs_work_status;
+/* read_state (cl_sieve.cpp:132-222): in `dir` ("" = current directory), reads
+ * stateA.ckp and stateB.ckp; a file counts when it parses, its ranges equal
+ * st->pmin/pmax/nmin/nmax and its state_sum checks ("Cannot parse", "Invalid
+ * checkpoint file", "Checksum error" on stderr otherwise).  With two good files
+ * the larger p wins.  Returns 1 and fills *st and *which (0 = A, 1 = B) when one
+ * is found, 0 when none is. */
+int fs_read_checkpoint(const char* dir, fs_work_status* st, int* which);
+/* write_state (cl_sieve.cpp:100-129): sets st->state_sum and writes stateA.ckp
+ * (which = 0) or stateB.ckp (which = 1); FS_OK or FS_E_IO. */
+int fs_write_checkpoint(const char* dir, fs_work_status* st, int which);
+
 /* ---------------------------------------- batched persistent form (many jobs) */
 typedef struct {
   uint64_t pmin, pmax;

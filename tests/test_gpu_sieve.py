@@ -211,3 +211,39 @@ def test_checkpoint_resume_matches_uninterrupted(gpu, tmp_path):
     assert (full.factorcount, full.primecount, f"{full.checksum:016X}") == (25, 108529, C1["checksum"])
     lines = open(tmp_path / "factors.txt").read().splitlines()
     assert lines == [gpu.format_factor(p, nc) for p, nc in C1["factors"]] + [C1["checksum"]]
+
+
+def test_resume_skips_corrupt_checkpoint(gpu, tmp_path):
+    """The newest checkpoint is damaged: the run resumes from the other file
+    (recomputing the segments in between) and still ends with the reference's
+    count and checksum; every golden factor is in factors.txt."""
+    g = C1["range"]
+    opts = dict(write_files=1, resume=1, workdir=str(tmp_path), checkpoint_secs=0, seg_candidates=8000)
+    gpu.sieve(*g, stop_after_segments=6, **opts)
+    newest, which = gpu.read_checkpoint(tmp_path, *g)
+    name = tmp_path / ("stateB.ckp" if which else "stateA.ckp")
+    raw = bytearray(open(name, "rb").read())
+    raw[56] ^= 0xFF  # state_sum
+    open(name, "wb").write(raw)
+    older, which2 = gpu.read_checkpoint(tmp_path, *g)
+    assert which2 != which and older.p < newest.p
+    full = gpu.sieve(*g, **opts)
+    assert full.resumed == 1
+    assert (full.factorcount, full.primecount, f"{full.checksum:016X}") == (25, 108529, C1["checksum"])
+    lines = open(tmp_path / "factors.txt").read().splitlines()
+    assert lines[-1] == C1["checksum"]
+    assert set(lines[:-1]) == {gpu.format_factor(p, nc) for p, nc in C1["factors"]}
+
+
+def test_missing_factor_lines_detected(gpu, tmp_path):
+    """factors.txt lost lines between checkpoint and the end: the line-count
+    check of finalizeResults fails with FS_E_IO (cl_sieve.cpp:742-746)."""
+    g = C1["range"]
+    opts = dict(write_files=1, resume=1, workdir=str(tmp_path), checkpoint_secs=0, seg_candidates=8000)
+    part = gpu.sieve(*g, stop_after_segments=10, **opts)
+    st, _ = gpu.read_checkpoint(tmp_path, *g)
+    assert st.factorcount > 0
+    open(tmp_path / "factors.txt", "w").close()
+    with pytest.raises(gpu.FactSieveError) as e:
+        gpu.sieve(*g, **opts)
+    assert e.value.code == gpu.FS_E_IO and "Missing factors" in str(e.value)

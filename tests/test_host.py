@@ -6,6 +6,7 @@ import json
 import math
 import os
 import random
+import struct
 import re
 import subprocess
 
@@ -279,3 +280,60 @@ def test_cli_rejects_bad_arguments(fs):
     assert r.returncode != 0 and "out of range argument -n 99" in r.stdout
     r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "-N #" in r.stdout
+
+
+# ---------------------------------------------------------------- checkpoints
+RANGE = (10 ** 8, 10 ** 8 + 2 * 10 ** 6, 1000, 10000)
+
+
+def _state(fs, p, checksum=0x1234, primecount=77, factorcount=3):
+    pmin, pmax, nmin, nmax = RANGE
+    return fs.WorkStatus(pmin=pmin, pmax=pmax, p=p, checksum=checksum, primecount=primecount,
+                         factorcount=factorcount, last_trickle=1700000000, nmin=nmin, nmax=nmax)
+
+
+def test_checkpoint_record_layout_and_sum(fs, tmp_path):
+    """72-byte little-endian workStatus, state_sum = wrapping sum of the other fields."""
+    assert ctypes.sizeof(fs.WorkStatus) == 72
+    st = fs.write_checkpoint(tmp_path, _state(fs, 10 ** 8 + 5000), 0)
+    raw = open(tmp_path / "stateA.ckp", "rb").read()
+    assert len(raw) == 72
+    f = struct.unpack("<8Q2I", raw)
+    assert f[7] == sum(f[:7] + f[8:]) % 2 ** 64 == st.state_sum
+    assert f[2] == 10 ** 8 + 5000 and f[8:] == (1000, 10000)
+
+
+def test_checkpoint_selection_and_rejection(fs, tmp_path, capfd):
+    """read_state rules (cl_sieve.cpp:132-222): larger p of two good files;
+    a truncated, range-mismatched or checksum-broken file is skipped with the
+    reference's message; none good -> fresh start."""
+    d = tmp_path
+    assert fs.read_checkpoint(d, *RANGE) is None  # no files
+    fs.write_checkpoint(d, _state(fs, 10 ** 8 + 4000), 0)
+    fs.write_checkpoint(d, _state(fs, 10 ** 8 + 9000), 1)
+    st, which = fs.read_checkpoint(d, *RANGE)
+    assert (st.p, which) == (10 ** 8 + 9000, 1)
+    # checksum error in B -> A
+    raw = bytearray(open(d / "stateB.ckp", "rb").read())
+    raw[24] ^= 1  # checksum field
+    open(d / "stateB.ckp", "wb").write(raw)
+    st, which = fs.read_checkpoint(d, *RANGE)
+    assert (st.p, which) == (10 ** 8 + 4000, 0)
+    assert "Checksum error in stateB.ckp" in capfd.readouterr().err
+    # A for another range -> invalid; B broken -> none
+    other = _state(fs, 10 ** 8 + 4000)
+    other.nmax = 20000
+    fs.write_checkpoint(d, other, 0)
+    assert fs.read_checkpoint(d, *RANGE) is None
+    assert "Invalid checkpoint file stateA.ckp" in capfd.readouterr().err
+    # truncated B -> cannot parse; A good again -> A
+    fs.write_checkpoint(d, _state(fs, 10 ** 8 + 6000), 0)
+    open(d / "stateB.ckp", "wb").write(b"\0" * 40)
+    st, which = fs.read_checkpoint(d, *RANGE)
+    assert (st.p, which, st.primecount, st.factorcount, st.checksum) == (10 ** 8 + 6000, 0, 77, 3, 0x1234)
+    assert "Cannot parse stateB.ckp" in capfd.readouterr().err
+
+
+def test_checkpoint_write_error_reported(fs, tmp_path):
+    with pytest.raises(fs.FactSieveError):
+        fs.write_checkpoint(tmp_path / "no" / "such" / "dir", _state(fs, 10 ** 8), 0)
