@@ -141,6 +141,8 @@ _SIGS = [
                                           ctypes.POINTER(Factor), u32, ctypes.POINTER(u32)]),
     ("fs_hostemu_candidates", i64, [u64, u64, vp, u64]),
     ("fs_hostemu_mulmod", ctypes.c_int, [ctypes.c_int, u64, i64, u64, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
+    ("fs_hostemu_setup_mul", ctypes.c_int, [u64, u64, u64, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                            ctypes.POINTER(u32)]),
     ("fs_default_options", None, [ctypes.POINTER(Options)]),
     ("fs_sieve", ctypes.c_int, [u64, u64, u32, u32, ctypes.POINTER(Options), ctypes.POINTER(Result),
                                 ctypes.POINTER(Factor), u64, ctypes.POINTER(u64)]),
@@ -259,12 +261,12 @@ def power_table(nmin):
 
 def setup_program(nmin, path):
     """(terms, words): the device words of the setup program -- FP64: terms
-    doubles; HYBRID: terms multipliers then the power-of-two exponent word;
+    doubles; HYBRID: terms multipliers then the two power-of-two exponent words;
     MONT: the power table as bases[terms] then (exp | curbit << 32)[terms]."""
     n = check(lib().fs_setup_program(nmin, path, None))
-    words = (u64 * (max(2 * n, 1) + 1))()
+    words = (u64 * (max(2 * n, 1) + 2))()
     lib().fs_setup_program(nmin, path, words)
-    nw = {PATH_MONT: 2 * n, PATH_HYBRID: n + 1}.get(path, n)
+    nw = {PATH_MONT: 2 * n, PATH_HYBRID: n + 2}.get(path, n)
     return n, list(words[:nw])
 
 
@@ -279,6 +281,14 @@ def hostemu_mulmod(path, p, s, m):
     raw, res = i64(), u64()
     check(lib().fs_hostemu_mulmod(path, p, s, m, ctypes.byref(raw), ctypes.byref(res)))
     return raw.value, res.value
+
+
+def hostemu_setup_mul(p, s, m, square=False):
+    """One multiply (or square) of the hybrid setup program, s*m*2^-k mod p, on
+    the host: (the value the setup loop keeps, canonical, k)."""
+    raw, res, k = u64(), u64(), u32()
+    check(lib().fs_hostemu_setup_mul(p, s, m, int(square), ctypes.byref(raw), ctypes.byref(res), ctypes.byref(k)))
+    return raw.value, res.value, k.value
 
 
 def hostemu_iterate(path, p, n0, n1, res, cap=64):

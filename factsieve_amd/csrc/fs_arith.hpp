@@ -27,7 +27,8 @@
 //    below 2^63: the step keeps |s| <= (1/2 + 2^-19) p), multiplier
 //    m < 2^31; the quotient is an FP64 estimate rounded to nearest, and
 //    s' = s*m - k*p is formed exactly mod 2^64 from 32x32->64 multiply-adds.
-//    The setup program multiplies by a REDC by 2^32 when p < 2^63.
+//    The setup program multiplies by 60-bit words with Montgomery reductions
+//    (two REDC-by-2^31 stages below 2^64/3, one REDC by 2^64 above).
 //
 // The error bounds are proved in the design notes shipped with the library and
 // are exercised by the parity tests at the edges of each range.
@@ -163,18 +164,62 @@ FS_HD int64_t fs_sh_mulmod(int64_t s, uint32_t m, double md, uint64_t c31, uint6
 FS_HD int64_t fs_canon_to_signed(uint64_t r, uint64_t p) { return r > (p >> 1) ? (int64_t)(r - p) : (int64_t)r; }
 FS_HD uint64_t fs_signed_to_canon(int64_t s, uint64_t p) { return s < 0 ? (uint64_t)s + p : (uint64_t)s; }
 
-// --------------------------------------------- 64x32 Montgomery (REDC by 2^32)
+// --------------------------------------------- lazy Montgomery (REDC by 2^64)
 
-// s*m*2^-32 mod p for s < 2p, m < 2^31, odd p < 2^63, pn32 = -p^-1 mod 2^32;
-// result < 2p.  Five 32x32 partial products (four of them full 64-bit) and
-// one 64-bit add: the setup program's multiply on the hybrid path.
-FS_HD uint64_t fs_redc32_mul(uint64_t s, uint32_t m, uint64_t p, uint32_t pn32) {
-  const uint64_t a = (uint64_t)(uint32_t)s * m;                       // t mod 2^32 + carry
-  const uint64_t b = (uint64_t)(uint32_t)(s >> 32) * m + (a >> 32);  // t >> 32
-  const uint32_t u = (uint32_t)a * pn32;                              // t + u*p = 0 mod 2^32
-  const uint64_t c = (uint64_t)u * (uint32_t)p + (uint32_t)a;         // low word is zero
-  const uint64_t e = (uint64_t)u * (uint32_t)(p >> 32) + b;
-  return e + (c >> 32);
+// s*m*2^-64 mod p for s < 2p, m < 2^63, odd p < 2^63, qn = -p^-1 mod 2^64;
+// result < 2p (s*m < p*2^64, so (s*m + u*p) / 2^64 < 2p < 2^64).  The setup
+// program's square on the hybrid path below 2^63, and its multiply between
+// 2^64/3 and 2^63.  t = s*m = th:t1:t0 and
+// u = (t mod 2^64) * qn mod 2^64 = u1:u0 from 32x32 partial products; the
+// low 64 bits of t + u*p are zero, so the carry out of bit 63 is (t1 != 0)
+// once bits 0..31 have been folded into z00 (t1 + bits 32..63 of
+// t0 + u*p is 0 or 2^32, and 2^32 exactly when t1 != 0).
+FS_HD uint64_t fs_redc64_mul(uint64_t s, uint64_t m, uint64_t p, uint64_t qn) {
+  const uint32_t s0 = (uint32_t)s, s1 = (uint32_t)(s >> 32);
+  const uint32_t m0 = (uint32_t)m, m1 = (uint32_t)(m >> 32);
+  const uint64_t x00 = (uint64_t)s0 * m0;
+  const uint64_t x01 = (uint64_t)s0 * m1 + (x00 >> 32);
+  const uint64_t x10 = (uint64_t)s1 * m0 + (uint32_t)x01;
+  const uint64_t th = (uint64_t)s1 * m1 + (x01 >> 32) + (x10 >> 32);
+  const uint32_t t0 = (uint32_t)x00, t1 = (uint32_t)x10;
+  const uint64_t y = (uint64_t)t0 * (uint32_t)qn;
+  const uint32_t u0 = (uint32_t)y;
+  const uint32_t u1 = (uint32_t)(y >> 32) + t0 * (uint32_t)(qn >> 32) + t1 * (uint32_t)qn;
+  const uint64_t z00 = (uint64_t)u0 * (uint32_t)p + t0;  // bits 0..31 are zero
+  const uint64_t z01 = (uint64_t)u0 * (uint32_t)(p >> 32) + (z00 >> 32);
+  const uint64_t z10 = (uint64_t)u1 * (uint32_t)p + (uint32_t)z01;
+  const uint64_t w = (uint64_t)u1 * (uint32_t)(p >> 32) + (z01 >> 32);
+  return th + w + (z10 >> 32) + (t1 != 0);
+}
+
+// Largest modulus (exclusive) of fs_redc62_mul: its first stage leaves a value
+// below 3p, which must fit in 64 bits.
+#define FS_REDC62_PMAX 0x5555555555555555ull
+
+// s*m*2^-62 mod p for s < 2p, m < 2^60, odd p < 2^64/3, pn = -p^-1 mod 2^32;
+// result < 2p.  Two REDC-by-2^31 stages over m = m1*2^31 + m0 (m0 < 2^31,
+// m1 < 2^29): y = (s*m0 + u*p) / 2^31 < 3p, then (s*m1 + y + u'*p) / 2^31
+// < p/2 + p + 3p*2^-31 < 2p, with u, u' < 2^31 chosen to clear the low 31
+// bits.  Reducing by 2^31 rather than 2^32 keeps every 32x32+64 partial sum
+// below 2^64 (u*p0 < 2^63 next to s0*m0 < 2^63), and the division by 2^31 is
+// one 64-bit shift and one shift-add: (L + X*2^32) / 2^31 = 2X + L/2^31.
+// Eight 32x32->64 multiply-adds and two 32-bit multiplies per 60-bit word,
+// about three primes of a factorial's power table: the setup program's
+// multiply on the hybrid path below 2^64/3.
+FS_HD uint64_t fs_redc62_mul(uint64_t s, uint64_t m, uint64_t p, uint32_t pn) {
+  const uint32_t m0 = (uint32_t)m & 0x7FFFFFFFu, m1 = (uint32_t)(m >> 31);
+  const uint32_t p0 = (uint32_t)p, p1 = (uint32_t)(p >> 32);
+  const uint32_t s0 = (uint32_t)s, s1 = (uint32_t)(s >> 32);
+  const uint64_t a = (uint64_t)s0 * m0;  // < 2^63
+  const uint32_t u = ((uint32_t)a * pn) & 0x7FFFFFFFu;
+  const uint64_t l = (uint64_t)u * p0 + a;  // < 2^64, low 31 bits zero
+  const uint64_t x = (uint64_t)u * p1 + (uint64_t)s1 * m0;
+  const uint64_t y = (x << 1) + (l >> 31);  // < 3p
+  const uint64_t a2 = (uint64_t)s0 * m1 + (uint32_t)y;  // < 2^62
+  const uint32_t u2 = ((uint32_t)a2 * pn) & 0x7FFFFFFFu;
+  const uint64_t l2 = (uint64_t)u2 * p0 + a2;
+  const uint64_t x2 = (uint64_t)u2 * p1 + ((uint64_t)s1 * m1 + (y >> 32));
+  return (x2 << 1) + (l2 >> 31);
 }
 
 // 2^e * 2^64 mod p (Montgomery form of 2^e) for e >= 1: left-to-right

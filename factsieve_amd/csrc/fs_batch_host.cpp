@@ -31,8 +31,8 @@ double now_s() {
 
 // Relative cost of one p*n step and one setup term per path (VALU operations
 // per step of the respective kernels), used to order tiles longest first.
-double step_cost(int path) { return path == FS_PATH_FP64 ? 7.0 : path == FS_PATH_HYBRID ? 22.0 : 45.0; }
-double setup_cost(int path) { return path == FS_PATH_FP64 ? 6.0 : path == FS_PATH_HYBRID ? 7.0 : 90.0; }
+double step_cost(int path) { return path == FS_PATH_FP64 ? 7.0 : path == FS_PATH_HYBRID ? 12.25 : 45.0; }
+double setup_cost(int path) { return path == FS_PATH_FP64 ? 6.55 : path == FS_PATH_HYBRID ? 18.0 : 90.0; }
 
 }  // namespace
 

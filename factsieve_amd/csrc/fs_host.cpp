@@ -185,7 +185,9 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
     fs_set_error_msg("fs_setup_program: unknown path");
     return FS_E_ARG;
   }
-  const uint64_t bound = (path == FS_PATH_FP64) ? ((uint64_t)1 << 50) : ((uint64_t)1 << 31);
+  // FP64: multipliers below 2^50 (the FP64 step's bound); HYBRID: below 2^60, the
+  // bound of the two-stage REDC by 2^31 (fs_redc62_mul).
+  const uint64_t bound = (path == FS_PATH_FP64) ? ((uint64_t)1 << 50) : ((uint64_t)1 << 60);
   const std::vector<uint64_t> horner = fs_horner_program(nmin, bound);
   // Blocks of 4 terms: a square is a flag (bit 63) on the first term of a
   // block, so each group C_j is padded with multipliers 1 to a whole block.
@@ -204,11 +206,18 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
   }
   if (pending_square) prog.insert(prog.end(), {FS_PROG_SQUARE | 1, 1, 1, 1});
   if (words) {
-    uint64_t E = 0;  // HYBRID: power of 2^-1 accumulated by REDC-by-2^32 / Montgomery squarings
+    // HYBRID: powers of 2^-1 the Montgomery products accumulate; E62 with
+    // multiplies by 2^-62 (below 2^64/3), E64 with multiplies by 2^-64 (above);
+    // squares are by 2^-64 in both
+    uint64_t E62 = 0, E64 = 0;
     for (size_t i = 0; i < prog.size(); ++i) {
       const uint64_t m = prog[i] & ~FS_PROG_SQUARE;
-      if (prog[i] & FS_PROG_SQUARE) E = 2 * E + 64;
-      E += 32;
+      if (prog[i] & FS_PROG_SQUARE) {
+        E62 = 2 * E62 + 64;
+        E64 = 2 * E64 + 64;
+      }
+      E62 += 62;
+      E64 += 64;
       if (path == FS_PATH_FP64) {
         const double d = (double)m;  // exact: < 2^50
         uint64_t bits;
@@ -218,7 +227,10 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
         words[i] = prog[i];
       }
     }
-    if (path == FS_PATH_HYBRID) words[prog.size()] = E;
+    if (path == FS_PATH_HYBRID) {
+      words[prog.size()] = E62;
+      words[prog.size() + 1] = E64;
+    }
   }
   return (int64_t)prog.size();
 }
@@ -237,7 +249,7 @@ std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path) {
     const int64_t T = fs_setup_program(nmin, path, nullptr);
     if (T < 0) return nullptr;
     pr->terms = (uint32_t)T;
-    pr->words.assign((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1, 0);
+    pr->words.assign((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 2, 0);
     fs_setup_program(nmin, path, pr->words.data());
     std::lock_guard<std::mutex> lk(mu);
     if (cache.size() >= 256) cache.clear();  // holders keep their own reference

@@ -37,12 +37,39 @@ extern "C" int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, in
   return FS_E_ARG;
 }
 
+extern "C" int fs_hostemu_setup_mul(uint64_t p, uint64_t s, uint64_t m, int square, uint64_t* raw, uint64_t* res,
+                                    uint32_t* shift) {
+  if ((p & 1) == 0 || p < 3 || p > FS_HYBRID_PMAX || m >= ((uint64_t)1 << 60) || !raw || !res || !shift)
+    return FS_E_ARG;
+  const uint64_t q = fs_invert(p);
+  if (p < ((uint64_t)1 << 63)) {
+    if (s >= 2 * p) return FS_E_ARG;
+    if (square) {
+      const uint64_t a = s >= p ? s - p : s;  // as the setup loop squares
+      *raw = fs_redc64_mul(a, a, p, 0 - q);
+      *shift = 64;
+    } else if (p < FS_REDC62_PMAX) {
+      *raw = fs_redc62_mul(s, m, p, (uint32_t)(0 - q));
+      *shift = 62;
+    } else {
+      *raw = fs_redc64_mul(s, m, p, 0 - q);
+      *shift = 64;
+    }
+    *res = *raw >= p ? *raw - p : *raw;
+  } else {
+    if (s >= p) return FS_E_ARG;
+    *raw = *res = square ? fs_mont_mul(s, s, p, q) : fs_mont_mul(s, m, p, q);
+    *shift = 64;
+  }
+  return FS_OK;
+}
+
 extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* res) {
   if ((p & 1) == 0 || p < 3) return FS_E_ARG;
   const uint64_t q = fs_invert(p), one = fs_r64_mod(p), r2 = fs_r2_from_one(one, p, q);
   const int64_t T = fs_setup_program(nmin, path, nullptr);
   if (T < 0) return (int)T;
-  std::vector<uint64_t> w((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 1);
+  std::vector<uint64_t> w((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 2);
   fs_setup_program(nmin, path, w.data());
   if (path == FS_PATH_FP64) {
     if (p >= ((uint64_t)1 << 53)) return FS_E_ARG;
@@ -61,32 +88,26 @@ extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* r
     *res = fs_fp_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {
     if (p > FS_HYBRID_PMAX) return FS_E_ARG;
-    if (p < ((uint64_t)1 << 63)) {
-      const uint32_t pn32 = (uint32_t)(0 - q);
-      uint64_t s = 1;
-      for (int64_t k = 0; k < T; ++k) {
+    // the device routine's products (fs_cand_setup_hybrid), one term at a time
+    uint64_t s = 1;
+    const uint64_t qn = 0 - q;
+    for (int64_t k = 0; k < T; ++k) {
+      const uint64_t m = w[k] & ~FS_PROG_SQUARE;
+      if (p < ((uint64_t)1 << 63)) {
         if (w[k] & FS_PROG_SQUARE) {
           const uint64_t a = s >= p ? s - p : s;
-          s = fs_mont_mul(a, a, p, q);
+          s = fs_redc64_mul(a, a, p, qn);
         }
-        s = fs_redc32_mul(s, (uint32_t)w[k], p, pn32);
+        s = (p < FS_REDC62_PMAX) ? fs_redc62_mul(s, m, p, (uint32_t)qn) : fs_redc64_mul(s, m, p, qn);
+      } else {
+        if (w[k] & FS_PROG_SQUARE) s = fs_mont_mul(s, s, p, q);
+        s = fs_mont_mul(s, m, p, q);
       }
-      s = s >= p ? s - p : s;
-      if (w[T] != 0) s = fs_mont_mul(s, fs_mont_pow2(w[T], p, q, one), p, q);
-      *res = s;
-    } else {
-      const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
-      int64_t s = 1;
-      for (int64_t k = 0; k < T; ++k) {
-        if (w[k] & FS_PROG_SQUARE) {
-          const uint64_t a = fs_signed_to_canon(s, p);
-          s = fs_canon_to_signed(fs_mul_mod(a, a, p, q, r2), p);
-        }
-        const uint32_t m = (uint32_t)w[k];
-        s = fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32);
-      }
-      *res = fs_signed_to_canon(s, p);
     }
+    s = s >= p ? s - p : s;
+    const uint64_t E = (p < FS_REDC62_PMAX) ? w[T] : w[T + 1];
+    if (E != 0) s = fs_mont_mul(s, fs_mont_pow2(E, p, q, one), p, q);
+    *res = s;
   } else if (path == FS_PATH_MONT) {
     uint64_t s6 = one;
     for (int64_t k = 0; k < T; ++k) {

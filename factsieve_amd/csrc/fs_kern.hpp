@@ -202,56 +202,84 @@ __device__ __forceinline__ uint64_t fs_cand_setup_fp64(uint64_t p, uint64_t q, u
   return fs_fp_to_canon(s, p);
 }
 
-// Hybrid program: multipliers < 2^31.  For p < 2^63 each multiply is a REDC
-// by 2^32 and each square a Montgomery product by 2^64, so the running product
-// is x * 2^-E mod p (below 2p); word [terms] holds the total E, removed after
-// the last block.  For 2^63 <= p the multiplies are exact signed hybrid steps.
+// Hybrid program: multipliers < 2^60 (about three primes of the power table
+// per word).  Every product is a Montgomery product, so the running value is
+// x * 2^-E mod p; the E of the program is removed after the last block by one
+// product with Montgomery(2^E).  Below 2^64/3 a multiply is fs_redc62_mul
+// (factor 2^-62) and a square fs_redc64_mul (2^-64), both lazy (value below
+// 2p); between 2^64/3 and 2^63 both are fs_redc64_mul; from 2^63 on both are
+// the canonical subtractive form (fs_mont_mul).  Word [terms] holds the E of
+// the first range, word [terms + 1] the E of the other two.  The blocks are
+// read one ahead as in the FP64 program.
+template <class Mul, class Sq>
+__device__ __forceinline__ uint64_t fs_setup_blocks(uint64_t s, const fs_prog_block* __restrict__ prog, uint32_t b0,
+                                                    uint32_t b1, Mul mul, Sq sq) {
+  auto square = [&](const fs_prog_block& blk) {
+    if (blk.w[0] & FS_PROG_SQUARE) s = sq(s);
+  };
+  auto terms4 = [&](const fs_prog_block& blk) {
+    s = mul(s, blk.w[0] & ~FS_PROG_SQUARE);
+    s = mul(s, blk.w[1]);
+    s = mul(s, blk.w[2]);
+    s = mul(s, blk.w[3]);
+  };
+  if (b0 < b1) {
+    fs_prog_block x = prog[b0];
+    uint32_t b = b0;
+    for (; b + 2 < b1; b += 2) {
+      square(x);
+      const fs_prog_block y = prog[b + 1];
+      asm volatile("" : "+v"(s)::"memory");
+      terms4(x);
+      square(y);
+      x = prog[b + 2];
+      asm volatile("" : "+v"(s)::"memory");
+      terms4(y);
+    }
+    square(x);
+    terms4(x);
+    if (b + 1 < b1) {
+      const fs_prog_block y = prog[b + 1];
+      square(y);
+      terms4(y);
+    }
+  }
+  return s;
+}
+
 __device__ __forceinline__ uint64_t fs_cand_setup_hybrid(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
                                                          const uint64_t* __restrict__ progw, uint32_t terms,
                                                          uint32_t start, uint32_t end) {
   const fs_prog_block* __restrict__ prog = (const fs_prog_block*)progw;
   const uint32_t b0 = start >> 2, b1 = end >> 2;
+  uint64_t s = (start == 0) ? 1 : s_in;
+  uint64_t E;
   if (p < ((uint64_t)1 << 63)) {
-    const uint32_t pn32 = (uint32_t)(0 - q);
-    uint64_t s = (start == 0) ? 1 : s_in;
-    if (b0 < b1) {
-      fs_prog_block cur = prog[b0];
-      for (uint32_t b = b0; b < b1; ++b) {
-        const fs_prog_block nxt = prog[b + 1 < b1 ? b + 1 : b];
-        if (cur.w[0] & FS_PROG_SQUARE) {
-          const uint64_t a = s >= p ? s - p : s;
-          s = fs_mont_mul(a, a, p, q);
-        }
-        s = fs_redc32_mul(s, (uint32_t)cur.w[0], p, pn32);
-        s = fs_redc32_mul(s, (uint32_t)cur.w[1], p, pn32);
-        s = fs_redc32_mul(s, (uint32_t)cur.w[2], p, pn32);
-        s = fs_redc32_mul(s, (uint32_t)cur.w[3], p, pn32);
-        cur = nxt;
-      }
+    const uint64_t qn = 0 - q;
+    auto sq = [&](uint64_t v) {
+      const uint64_t a = v >= p ? v - p : v;  // a < p: a*a < p*2^64
+      return fs_redc64_mul(a, a, p, qn);
+    };
+    if (p < FS_REDC62_PMAX) {
+      const uint32_t pn = (uint32_t)qn;
+      s = fs_setup_blocks(s, prog, b0, b1, [&](uint64_t v, uint64_t m) { return fs_redc62_mul(v, m, p, pn); }, sq);
+      E = progw[terms];
+    } else {
+      s = fs_setup_blocks(s, prog, b0, b1, [&](uint64_t v, uint64_t m) { return fs_redc64_mul(v, m, p, qn); }, sq);
+      E = progw[terms + 1];
     }
     s = s >= p ? s - p : s;
-    if (end == terms && progw[terms] != 0) {
-      const uint64_t one = fs_mont_mul(1, r2, p, q);
-      s = fs_mont_mul(s, fs_mont_pow2(progw[terms], p, q, one), p, q);
-    }
-    return s;
+  } else {
+    s = fs_setup_blocks(
+        s, prog, b0, b1, [&](uint64_t v, uint64_t m) { return fs_mont_mul(v, m, p, q); },
+        [&](uint64_t v) { return fs_mont_mul(v, v, p, q); });
+    E = progw[terms + 1];
   }
-  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
-  const uint64_t pn = 0 - p, c31 = p << 31;
-  int64_t s = (start == 0) ? 1 : fs_canon_to_signed(s_in, p);
-  for (uint32_t b = b0; b < b1; ++b) {
-    const fs_prog_block cur = prog[b];
-    if (cur.w[0] & FS_PROG_SQUARE) {
-      const uint64_t a = fs_signed_to_canon(s, p);
-      s = fs_canon_to_signed(fs_mul_mod(a, a, p, q, r2), p);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t m = (uint32_t)cur.w[j];
-      s = fs_sh_mulmod(s, m, (double)m, c31, pn, pinv, pinv32);
-    }
+  if (end == terms && E != 0) {
+    const uint64_t one = fs_mont_mul(1, r2, p, q);
+    s = fs_mont_mul(s, fs_mont_pow2(E, p, q, one), p, q);
   }
-  return fs_signed_to_canon(s, p);
+  return s;
 }
 
 // Reference power-table evaluation (setup.cl:49-103) on the Montgomery path:

@@ -170,11 +170,12 @@ int fs_verifyresult(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_strea
 int64_t fs_power_table(uint32_t nmin, uint64_t* bases, uint32_t* exps, uint32_t* curbits);
 /* Setup program for `path` in words of 8 bytes; returns the term count T
  * (pass NULL words to get it).  FP64 and HYBRID: T a multiple of 4, in blocks of
- * four multipliers (FP64: doubles < 2^50 as bits; HYBRID: integers < 2^31;
+ * four multipliers (FP64: doubles < 2^50 as bits; HYBRID: integers < 2^60;
  * padding multipliers are 1).  Bit 63 set on a block's first word (FS_PROG_SQUARE
  * in the kernels) squares the running product before that block; evaluating
- * from 1 yields nmin!.  HYBRID has one more word, [T], the total power of 2^-1
- * that its REDC-by-2^32 multiplies leave (used for p < 2^63).  MONT: the
+ * from 1 yields nmin!.  HYBRID has two more words, the total power of 2^-1 its
+ * Montgomery products leave: [T] with multiplies by 2^-62 (p < 2^64/3), [T+1]
+ * with multiplies by 2^-64 (other p); squares are by 2^-64 in both.  MONT: the
  * reference power table, 2*T words: bases[T] then (exp | curbit << 32)[T]. */
 int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words);
 /* Path the driver uses for p < pmax. */
@@ -202,6 +203,14 @@ int64_t fs_hostemu_candidates(uint64_t low, uint64_t high, uint64_t* out, uint64
  * p <= 2^64 - 2^46 (FS_E_ARG otherwise).  *raw = the step's own signed
  * representation of the result, *res = its canonical value in [0, p). */
 int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, int64_t* raw, uint64_t* res);
+/* One multiply (square = 0) or square (square = 1, m ignored) of the HYBRID
+ * setup program, s*m*2^-k mod p: below 2^64/3 a multiply reduces by k = 62 and
+ * a square by k = 64, both lazily (s < 2p, result < 2p); below 2^63 both by
+ * k = 64, lazily; above, k = 64 in canonical form (s < p).  m < 2^60, odd
+ * 3 <= p <= 2^64 - 2^46 (FS_E_ARG otherwise).  *raw = the value as the setup
+ * loop keeps it, *res = its canonical value, *shift = k. */
+int fs_hostemu_setup_mul(uint64_t p, uint64_t s, uint64_t m, int square, uint64_t* raw, uint64_t* res,
+                         uint32_t* shift);
 
 /* ------------------------------------------------------- whole sieve */
 typedef struct {

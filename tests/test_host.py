@@ -86,8 +86,8 @@ def test_setup_program_evaluates_to_factorial(fs, path, nmin):
     import struct
     n, words = fs.setup_program(nmin, path)
     assert n % 4 == 0
-    bound = 2 ** 50 if path == 1 else 2 ** 31
-    acc, e = 1, 0
+    bound = 2 ** 50 if path == 1 else 2 ** 60
+    acc, e62, e64 = 1, 0, 0
     for i, w in enumerate(words[:n]):
         sq = w >> 63
         assert not sq or i % 4 == 0          # squares only open a 4-term block
@@ -95,11 +95,50 @@ def test_setup_program_evaluates_to_factorial(fs, path, nmin):
         m = int(struct.unpack("<d", struct.pack("<Q", w))[0]) if path == 1 else w
         assert 1 <= m < bound
         if sq:
-            acc, e = acc * acc, 2 * e + 64
-        acc, e = acc * m, e + 32
+            acc, e62, e64 = acc * acc, 2 * e62 + 64, 2 * e64 + 64
+        acc, e62, e64 = acc * m, e62 + 62, e64 + 64
     assert acc == math.factorial(nmin)
     if path == 2:
-        assert words[n] == e                  # exponent of the REDC factors 2^-32 / 2^-64
+        # exponents of the Montgomery factors: multiplies by 2^-62 or 2^-64, squares by 2^-64
+        assert words[n:n + 2] == [e62, e64]
+
+
+def test_hybrid_setup_multiply_at_edges(fs):
+    """The hybrid setup products s*m*2^-k mod p against big integers at the
+    edges of their bounds: below 2^64/3 the two-stage reduction by 2^31 (k = 62)
+    at s = 2p - 1, m = 2^60 - 1 and p just under 2^64/3, where its first stage
+    comes closest to 2^64; between 2^64/3 and 2^63 the lazy REDC by 2^64; above,
+    the canonical form; squares by 2^64 in every range.  Lazy values stay below
+    2p, canonical ones below p."""
+    rng = random.Random(17)
+    third = (2 ** 64 - 1) // 3
+    ps = [third - 2 if (third - 2) % 2 else third - 1, 2 ** 62 + 2 ** 30 + 3, 2 ** 62 + 135, 2 ** 62 - 57,
+          2 ** 53 + 5, 2 ** 32 + 15, 2 ** 32 - 5, 1000003, 127, 3,
+          third + 2 if (third + 2) % 2 else third + 1, 2 ** 63 - 25, 2 ** 63 - 259,
+          2 ** 64 - 2 ** 46 - 3, 2 ** 63 + 29]
+    for p in ps:
+        assert p % 2
+        stop = 2 * p if p < 2 ** 63 else p
+        ss = [stop - 1, stop - 2, p - 1, p, p + 1, 1, 0, 2 ** 32 - 1, 2 ** 32, 2 ** 31 - 1] + \
+             [rng.randrange(0, stop) for _ in range(300)]
+        ms = [2 ** 60 - 1, 2 ** 60 - 2, 2 ** 31 - 1, 2 ** 31, 2 ** 32 - 1, 1, 0, 2 ** 59 + 2 ** 31 - 1] + \
+             [rng.randrange(0, 2 ** 60) for _ in range(300)]
+        for i, s in enumerate(ss):
+            if s >= stop:
+                continue
+            for m in ms[:8] + [ms[8 + i % 300]]:
+                raw, res, k = fs.hostemu_setup_mul(p, s, m)
+                assert k == (62 if p < third else 64)
+                assert res == s * m * pow(2, -k, p) % p, (p, s, m)
+                assert raw % p == res and raw < stop, (p, s, m, raw)
+            raw, res, k = fs.hostemu_setup_mul(p, s, 0, square=True)
+            assert k == 64 and res == s * s * pow(2, -64, p) % p and raw % p == res and raw < stop, (p, s)
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_setup_mul(2 ** 62 + 135, 2 * (2 ** 62 + 135), 5)   # s >= 2p
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_setup_mul(2 ** 62 + 135, 5, 2 ** 60)                # m >= 2^60
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_setup_mul(2 ** 63 + 29, 2 ** 63 + 29, 5)            # canonical form: s >= p
 
 
 def test_is_prime_and_verify_match_oracle(fs):
