@@ -75,7 +75,7 @@ class JobResult(ctypes.Structure):
 
 class BatchOptions(ctypes.Structure):
     _fields_ = [("tile_candidates", u32), ("persistent_blocks", u32), ("use_graph", u32), ("lpt_order", u32),
-                ("verify_threads", ctypes.c_int)]
+                ("verify_threads", ctypes.c_int), ("lanes_per_prime", u32), ("n_chunk", u32), ("unroll", u32)]
 
 
 class ServerOptions(ctypes.Structure):

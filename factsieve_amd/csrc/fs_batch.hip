@@ -1,19 +1,23 @@
 // fs_batch.hip -- many sieve jobs in one persistent launch.
 //
-// A batch is run as four device launches on one stream (optionally captured
-// into a hipGraph):
+// A batch is run as device launches on one stream (optionally captured into a
+// hipGraph):
 //   k_bclear       zero the per-job counters and the tile queue
 //   k_bprimegen    candidate generation for every job at once: workgroup b
 //                  belongs to the job whose block range holds b and appends
 //                  into that job's candidate region (bounded)
+// then, once per round (one round unless the n-ranges are cut into chunks):
 //   k_bmake_tiles  one workgroup turns the per-job candidate counts into a tile
 //                  list (job, first candidate, n0, n1) in cost order
 //   k_bsieve       persistent workgroups pull tiles from an atomic queue; for
 //                  each candidate of a tile: setup (nmin!), iterate over the
 //                  tile's n-range with +-1 capture, and the checksum fold, all
-//                  in registers -- no residue goes back to memory in between
-// The serving loop (fs_server.cpp) drives the same kernels with n-chunked tiles
-// and keeps residues in the candidate regions between chunks.
+//                  in registers; between rounds the residue stays in the
+//                  candidate region
+// A job may also be split over several lanes per prime: its n-range is cut
+// into subranges that run as independent entries, each with the setup program
+// of its own first n.  The serving loop (fs_server.cpp) drives the same kernels
+// with n-chunked tiles, one round per quantum.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -98,10 +102,11 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bmake_tiles(const fs_bjob* __restr
                                                           const uint32_t* jcnt, uint32_t tile, uint4* tiles,
                                                           uint32_t tiles_cap, uint32_t* pref, uint32_t* glob) {
   if (threadIdx.x == 0) {
+    glob[FS_BG_HEAD] = 0;  // the previous round's persistent kernel has finished (stream order)
     uint32_t acc = 0;
     for (uint32_t k = 0; k < nj; ++k) {
       const fs_bjob& J = jobs[order[k]];
-      const uint32_t live = min(jcnt[order[k]], J.cand_cap);
+      const uint32_t live = min(jcnt[J.parent], J.cand_cap);
       pref[k] = acc;
       acc += (live + tile - 1) / tile;
     }
@@ -119,7 +124,12 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bmake_tiles(const fs_bjob* __restr
 
 // Persistent sieve: each workgroup repeatedly takes the next tile; every queue
 // index is taken once, and a workgroup leaves when the index passes the tile
-// count, so all workgroups exit.
+// count, so all workgroups exit.  A job split over several lanes per prime
+// appears as n-subrange entries that share the candidates, counters, sums and
+// factor tag of their parent; each entry's setup program starts at its own
+// nmin, and only the entry that ends at the job's nmax folds the checksum.
+// U: steps per +-1 branch of the iterate loops.
+template <int U>
 __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__ jobs,
                                                      const uint4* __restrict__ tiles, fs_cands c,
                                                      const uint64_t* __restrict__ prog,
@@ -137,8 +147,9 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__
     const uint4 T = tiles[t];
     const uint32_t j = T.x, begin = T.y, n0 = T.z, n1 = T.w;
     const fs_bjob J = jobs[j];
-    const uint32_t live = min(jcnt[j], J.cand_cap);
-    const bool first = (n0 == J.nmin), last = (n1 == J.nmax);
+    const uint32_t jo = J.parent;  // owner of counters, sums and factor tags
+    const uint32_t live = min(jcnt[jo], J.cand_cap);
+    const bool first = (n0 == J.nmin), last = (n1 == J.nmax), fold = last && !(J.flags & FS_BJ_NOFOLD);
     const uint64_t* jp = prog + J.prog_off;
     const double* nt = ntab + n0 + 1;
     for (uint32_t k = threadIdx.x; k < tile; k += FS_BLOCK) {
@@ -148,32 +159,34 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__
         const uint32_t g = J.cand_off + i;
         const uint64_t p = c.p[g], q = c.q[g], r2 = c.r2[g];
         auto rec = [&](int32_t nc) {
-          fs_append_factor(&glob[FS_BG_NF], &glob[FS_BG_FOVF], fac, fcap, p, nc, (int32_t)j);
+          fs_append_factor(&glob[FS_BG_NF], &glob[FS_BG_FOVF], fac, fcap, p, nc, (int32_t)jo);
         };
         uint64_t s;
-        if (!first && c.nnext[g] != n0) atomicOr(&jflag[j], FS_BJF_VALIDATE);
+        if (!first && c.nnext[g] != n0) atomicOr(&jflag[jo], FS_BJF_VALIDATE);
         if (J.path == FS_PATH_FP64) {
           s = first ? fs_cand_setup_fp64(p, q, r2, 0, (const double*)jp, 0, J.prog_terms) : c.res[g];
-          s = fs_cand_iter_fp64(p, s, n0, n1, nt, rec);
+          s = fs_cand_iter_fp64<U>(p, s, n0, n1, nt, rec);
         } else if (J.path == FS_PATH_HYBRID) {
           s = first ? fs_cand_setup_hybrid(p, q, r2, 0, jp, J.prog_terms, 0, J.prog_terms) : c.res[g];
-          s = fs_cand_iter_hybrid(p, s, n0, n1, nt, rec);
+          s = fs_cand_iter_hybrid<U>(p, s, n0, n1, nt, rec);
         } else {
           s = first ? fs_cand_setup_mont(p, q, r2, 0, jp, J.prog_terms, 0, J.prog_terms) : c.res[g];
           s = fs_cand_iter_mont(p, q, r2, s, n0, n1, rec);
         }
-        if (last) {
+        if (fold) {
           bool ok;
           v = fs_cand_fold(p, q, r2, s, J.nmax, ok);
-          if (!ok) atomicOr(&jflag[j], FS_BJF_VALIDATE);
-        } else {
+          if (!ok) atomicOr(&jflag[jo], FS_BJF_VALIDATE);
+        } else if (!last) {
           c.res[g] = s;
           c.nnext[g] = n1;
+        } else if (s >= p) {  // a subrange's end residue must be canonical
+          atomicOr(&jflag[jo], FS_BJF_VALIDATE);
         }
       }
-      if (last) {
+      if (fold) {
         v = fs_wave_sum(v);
-        if ((threadIdx.x & 63) == 0 && v != 0) atomicAdd((unsigned long long*)&jsum[j], (unsigned long long)v);
+        if ((threadIdx.x & 63) == 0 && v != 0) atomicAdd((unsigned long long*)&jsum[jo], (unsigned long long)v);
       }
     }
   }
@@ -186,20 +199,30 @@ __global__ void k_bfill_ntab(double* t, uint32_t count) {
 
 // ============================================================ launch helpers
 
-int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, uint32_t norder, uint32_t pg_blocks, uint32_t tile,
-                    uint32_t blocks, hipStream_t st) {
+int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, const std::vector<uint32_t>& norder,
+                    const std::vector<uint32_t>& off, uint32_t pg_blocks, uint32_t tile, uint32_t blocks,
+                    uint32_t unroll, hipStream_t st) {
   k_bclear<<<(nj + 255) / 256 + 1, 256, 0, st>>>(d.jobs, d.jcnt, d.jflag, d.jsum, nj, d.glob);
   BCHK(hipGetLastError(), "batch clear");
   if (pg_blocks) {
     k_bprimegen<<<pg_blocks, FS_BLOCK, 0, st>>>(d.jobs, nj, d.c, d.jcnt, d.jflag);
     BCHK(hipGetLastError(), "batch primegen");
   }
-  k_bmake_tiles<<<1, FS_BLOCK, 0, st>>>(d.jobs, d.order, d.chunk, norder, d.jcnt, tile, d.tiles, d.tiles_cap,
-                                        d.pref, d.glob);
-  BCHK(hipGetLastError(), "batch tiles");
-  k_bsieve<<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.fac, d.fcap, d.jcnt, d.jflag,
-                                        d.jsum, d.glob, tile);
-  BCHK(hipGetLastError(), "batch sieve");
+  for (size_t r = 0; r < norder.size(); ++r) {
+    k_bmake_tiles<<<1, FS_BLOCK, 0, st>>>(d.jobs, d.order + off[r], d.chunk + 2 * off[r], norder[r], d.jcnt, tile,
+                                          d.tiles, d.tiles_cap, d.pref, d.glob);
+    BCHK(hipGetLastError(), "batch tiles");
+    if (unroll == 8)
+      k_bsieve<8><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.fac, d.fcap, d.jcnt, d.jflag,
+                                               d.jsum, d.glob, tile);
+    else if (unroll == 2)
+      k_bsieve<2><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.fac, d.fcap, d.jcnt, d.jflag,
+                                               d.jsum, d.glob, tile);
+    else
+      k_bsieve<4><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.fac, d.fcap, d.jcnt, d.jflag,
+                                               d.jsum, d.glob, tile);
+    BCHK(hipGetLastError(), "batch sieve");
+  }
   return FS_OK;
 }
 

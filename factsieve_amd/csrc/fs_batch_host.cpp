@@ -111,15 +111,28 @@ void fs_batch_bufs::release() {
 
 int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const std::vector<uint32_t>& order,
                         const std::vector<uint32_t>& chunk, const std::vector<uint64_t>& words, uint32_t pg_blocks,
-                        uint32_t ncands, uint32_t tile, uint32_t blocks, uint32_t fcap, bool graph, float* dev_ms) {
+                        uint32_t ncands, uint32_t tile, uint32_t blocks, uint32_t fcap, bool graph, float* dev_ms,
+                        const std::vector<uint32_t>& rounds, uint32_t unroll) {
   const uint32_t nj = (uint32_t)bj.size();
-  uint32_t nmax_max = 0, ntiles = 0;
-  for (uint32_t k = 0; k < order.size(); ++k) {
-    const fs_bjob& b = bj[order[k]];
-    nmax_max = std::max(nmax_max, b.nmax);
-    ntiles += (b.cand_cap + tile - 1) / tile;
+  std::vector<uint32_t> norder = rounds, off;
+  if (norder.empty()) norder.push_back((uint32_t)order.size());
+  uint32_t nmax_max = 0, ntiles = 0, k = 0;
+  for (uint32_t r : norder) {  // the tile list holds one round at a time
+    off.push_back(k);
+    uint32_t t = 0;
+    for (uint32_t e = 0; e < r; ++e, ++k) {
+      const fs_bjob& b = bj[order[k]];
+      nmax_max = std::max(nmax_max, b.nmax);
+      t += (b.cand_cap + tile - 1) / tile;
+    }
+    ntiles = std::max(ntiles, t);
   }
-  int rc = E.ensure(nj, ncands, (uint32_t)words.size(), ntiles, nmax_max, fcap);
+  if (k != order.size() || 2 * order.size() != chunk.size()) {
+    fs_set_error_msg("batch: rounds do not cover the tile order");
+    return FS_E_ARG;
+  }
+  int rc = E.ensure(std::max<uint32_t>(nj, (uint32_t)order.size()), ncands, (uint32_t)words.size(), ntiles,
+                    nmax_max, fcap);
   if (rc) return rc;
   fs_batch_dev& d = E.d;
   HCHK(hipMemcpyAsync(d.jobs, bj.data(), nj * sizeof(fs_bjob), hipMemcpyHostToDevice, E.st), "batch upload");
@@ -135,7 +148,7 @@ int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const 
     hipGraph_t g;
     hipGraphExec_t ge;
     HCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal), "batch capture");
-    rc = fs_batch_launch(d, nj, (uint32_t)order.size(), pg_blocks, tile, blocks, E.st);
+    rc = fs_batch_launch(d, nj, norder, off, pg_blocks, tile, blocks, unroll, E.st);
     hipError_t ce = hipStreamEndCapture(E.st, &g);
     if (rc) return rc;
     HCHK(ce, "batch end capture");
@@ -148,7 +161,7 @@ int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const 
     HCHK(le, "batch graph launch");
     HCHK(se, "batch graph sync");
   } else {
-    if ((rc = fs_batch_launch(d, nj, (uint32_t)order.size(), pg_blocks, tile, blocks, E.st))) return rc;
+    if ((rc = fs_batch_launch(d, nj, norder, off, pg_blocks, tile, blocks, unroll, E.st))) return rc;
   }
   HCHK(hipEventRecord(E.ev1, E.st), "batch event");
   HCHK(hipStreamSynchronize(E.st), "batch sync");
@@ -205,19 +218,30 @@ extern "C" void fs_default_batch_options(fs_batch_options* o) {
   o->use_graph = 1;
   o->lpt_order = 1;
   o->verify_threads = 16;
+  o->lanes_per_prime = 1;
+  o->n_chunk = 0;
+  o->unroll = 4;
 }
 
 namespace {
 
+// One n-subrange of a job: [n0, n1) from the setup program of n0.
+struct Sub {
+  uint32_t n0, n1;
+  std::shared_ptr<const fs_program> prog;
+  double cost;
+};
+
 struct Plan {
   int status = FS_OK;
   int path = 0;
-  uint32_t terms = 0;
-  std::shared_ptr<const fs_program> prog;
+  std::vector<Sub> sub;  // one entry unless the job is split over several lanes per prime
   uint32_t cap = 0;
   uint32_t pg_blocks = 0;
-  double cost = 0;
 };
+
+// Shortest n-subrange a job is cut into when split over several lanes.
+constexpr uint32_t kMinSubrange = 1024;
 
 
 // One batch through the device; results per job (before the host filter).
@@ -232,38 +256,88 @@ int run_batch(fs_batch_bufs& E, const fs_job* jobs, uint32_t nj, std::vector<Pla
   const uint32_t tile = o.tile_candidates ? o.tile_candidates : 256;
   std::vector<fs_bjob> bj(nj);
   std::vector<uint64_t> words;
-  std::vector<uint32_t> order, chunk;
   uint32_t cand_off = 0, pg = 0;
+  // Entries: job j at index j (its first subrange), later subranges appended.
+  struct Ent {
+    uint32_t idx, job, sub;
+  };
+  std::vector<Ent> ents;
+  auto add_prog = [&](fs_bjob& b, const Sub& s) {
+    b.prog_off = (uint32_t)words.size();
+    b.prog_terms = s.prog->terms;
+    words.insert(words.end(), s.prog->words.begin(), s.prog->words.end());
+    while (words.size() % 8) words.push_back(0);  // 64-byte aligned programs
+  };
   for (uint32_t j = 0; j < nj; ++j) {
     fs_bjob& b = bj[j];
     memset(&b, 0, sizeof b);
     b.pg_first = pg;
+    b.parent = j;
     if (plan[j].status != FS_OK) continue;
+    const Plan& P = plan[j];
     b.pmin = jobs[j].pmin;
     b.pmax = jobs[j].pmax;
-    b.nmin = jobs[j].nmin;
-    b.nmax = jobs[j].nmax;
-    b.path = (uint32_t)plan[j].path;
-    b.prog_off = (uint32_t)words.size();
-    b.prog_terms = plan[j].terms;
-    words.insert(words.end(), plan[j].prog->words.begin(), plan[j].prog->words.end());
-    while (words.size() % 8) words.push_back(0);  // 64-byte aligned programs
+    b.nmin = P.sub[0].n0;
+    b.nmax = P.sub[0].n1;
+    b.path = (uint32_t)P.path;
+    add_prog(b, P.sub[0]);
     b.cand_off = cand_off;
-    b.cand_cap = plan[j].cap;
-    b.flags = FS_BJ_NEW;
-    cand_off += plan[j].cap;
-    pg += plan[j].pg_blocks;
-    order.push_back(j);
+    b.cand_cap = P.cap;
+    b.flags = FS_BJ_NEW | (P.sub.size() > 1 ? FS_BJ_NOFOLD : 0);
+    cand_off += P.cap;
+    pg += P.pg_blocks;
+    ents.push_back({j, j, 0});
   }
-  if (lpt)
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return plan[a].cost > plan[b].cost; });
-  for (uint32_t j : order) {
-    chunk.push_back(jobs[j].nmin);
-    chunk.push_back(jobs[j].nmax);
+  for (uint32_t j = 0; j < nj; ++j) {  // later subranges: no candidate generation of their own
+    if (plan[j].status != FS_OK) continue;
+    const Plan& P = plan[j];
+    for (uint32_t k = 1; k < P.sub.size(); ++k) {
+      fs_bjob b = bj[j];
+      b.pg_first = pg;
+      b.nmin = P.sub[k].n0;
+      b.nmax = P.sub[k].n1;
+      add_prog(b, P.sub[k]);
+      b.flags = (k + 1 < P.sub.size()) ? FS_BJ_NOFOLD : 0;
+      b.parent = j;
+      ents.push_back({(uint32_t)bj.size(), j, k});
+      bj.push_back(b);
+    }
   }
+  // Rounds: a whole subrange per entry, or (n_chunk, unsplit jobs) chunk r of
+  // each entry whose range reaches that far; each round in cost order.
+  std::vector<uint32_t> order, chunk, rounds;
+  const uint32_t C = o.n_chunk;
+  for (uint32_t r = 0;; ++r) {
+    std::vector<std::pair<double, uint32_t>> cur;  // (cost, entry)
+    std::vector<std::pair<uint32_t, uint32_t>> rng(ents.size());
+    for (uint32_t e = 0; e < ents.size(); ++e) {
+      const Plan& P = plan[ents[e].job];
+      const Sub& s = P.sub[ents[e].sub];
+      const bool chunked = C && P.sub.size() == 1;
+      if (!chunked && r > 0) continue;
+      const uint64_t n0 = chunked ? (uint64_t)s.n0 + (uint64_t)r * C : s.n0;
+      if (r > 0 && n0 >= s.n1) continue;
+      const uint32_t n1 = chunked ? (uint32_t)std::min<uint64_t>(n0 + C, s.n1) : s.n1;
+      rng[e] = {(uint32_t)n0, n1};
+      const double frac = s.n1 > s.n0 ? (double)(n1 - n0) / (double)(s.n1 - s.n0) : 1.0;
+      cur.push_back({s.cost * frac, e});
+    }
+    if (cur.empty()) break;
+    if (lpt)
+      std::stable_sort(cur.begin(), cur.end(), [](const std::pair<double, uint32_t>& a,
+                                                  const std::pair<double, uint32_t>& b) { return a.first > b.first; });
+    for (auto& c : cur) {
+      order.push_back(ents[c.second].idx);
+      chunk.push_back(rng[c.second].first);
+      chunk.push_back(rng[c.second].second);
+    }
+    rounds.push_back((uint32_t)cur.size());
+  }
+  if (order.empty()) rounds.clear();
   const uint32_t blocks = o.persistent_blocks ? o.persistent_blocks : fs_device_cus() * 8;
-  int rc = fs_batch_run_device(E, bj, order, chunk, words, pg, cand_off, tile, blocks, fcap, o.use_graph != 0,
-                               dev_ms);
+  const uint32_t unroll = (o.unroll == 2 || o.unroll == 8) ? o.unroll : 4;
+  int rc = fs_batch_run_device(E, bj, order, chunk, words, pg, cand_off, tile, blocks, fcap, o.use_graph != 0, dev_ms,
+                               rounds.empty() ? std::vector<uint32_t>{0} : rounds, unroll);
   if (rc) return rc;
   uint32_t glob[FS_BG_WORDS];
   if ((rc = fs_batch_read(E, nj, out.cnt, out.flag, out.sum, out.fac, glob))) return rc;
@@ -277,6 +351,7 @@ int run_batch(fs_batch_bufs& E, const fs_job* jobs, uint32_t nj, std::vector<Pla
 
 int plan_jobs(const fs_job* jobs, uint32_t nj, const fs_batch_options& o, std::vector<Plan>& plan) {
   const uint32_t tile = o.tile_candidates ? o.tile_candidates : 256;
+  const uint32_t lanes = std::max<uint32_t>(1, std::min<uint32_t>(o.lanes_per_prime, 8));
   plan.assign(nj, Plan());
   for (uint32_t j = 0; j < nj; ++j) {
     Plan& P = plan[j];
@@ -286,12 +361,21 @@ int plan_jobs(const fs_job* jobs, uint32_t nj, const fs_batch_options& o, std::v
       continue;
     }
     P.path = fs_select_path(J.pmax);
-    P.prog = fs_program_get(J.nmin, P.path);
-    if (!P.prog) {
+    const uint32_t len = J.nmax - J.nmin;
+    const uint32_t L = std::max<uint32_t>(1, std::min<uint32_t>(lanes, len / kMinSubrange));
+    for (uint32_t k = 0; k < L; ++k) {
+      Sub s;
+      s.n0 = J.nmin + (uint32_t)((uint64_t)len * k / L);
+      s.n1 = J.nmin + (uint32_t)((uint64_t)len * (k + 1) / L);
+      s.prog = fs_program_get(s.n0, P.path);
+      if (!s.prog) break;
+      s.cost = (double)(s.n1 - s.n0) * step_cost(P.path) + (double)s.prog->terms * setup_cost(P.path);
+      P.sub.push_back(s);
+    }
+    if (P.sub.size() != L) {
       P.status = FS_E_NOMEM;
       continue;
     }
-    P.terms = P.prog->terms;
     const uint64_t c = fs_cand_capacity(J.pmin, J.pmax - J.pmin);
     if (c > 0x7fffffffull) {
       fs_set_error_msg("batch: job p-range too large for one batch");
@@ -300,7 +384,6 @@ int plan_jobs(const fs_job* jobs, uint32_t nj, const fs_batch_options& o, std::v
     }
     P.cap = (uint32_t)((c + tile - 1) / tile * tile);
     P.pg_blocks = fs_pg_blocks(J.pmin, J.pmax);
-    P.cost = (double)(J.nmax - J.nmin) * step_cost(P.path) + (double)P.terms * setup_cost(P.path);
   }
   return FS_OK;
 }
@@ -422,28 +505,44 @@ extern "C" int fs_batch_autotune(const fs_job* jobs, uint32_t nj, fs_batch_optio
     j.pmax = j.pmin + cut;
   }
   const uint32_t cus = fs_device_cus();
-  double best = 1e30;
-  fs_batch_options bo = *opt;
+  uint32_t nlen_max = 0;
+  for (uint32_t k = 0; k < nj; ++k) nlen_max = std::max(nlen_max, jobs[k].nmax - jobs[k].nmin);
   std::vector<fs_job_result> res(nj);
-  for (uint32_t tile : {256u, 512u}) {
-    for (uint32_t mult : {2u, 4u, 8u}) {
-     for (uint32_t lpt : {0u, 1u}) {
-      fs_batch_options o = *opt;
-      o.tile_candidates = tile;
-      o.persistent_blocks = cus * mult;
-      o.lpt_order = lpt;
+  auto timed = [&](const fs_batch_options& o, double* sec) {
+    int rc = fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, sec);  // warm
+    if (rc) return rc;
+    return fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, sec);
+  };
+  fs_batch_options best = *opt;
+  if (best.tile_candidates == 0 || best.tile_candidates % FS_BLOCK) best.tile_candidates = 256;
+  double tbest = 1e30;
+  int rc = timed(best, &tbest);
+  if (rc) return rc;
+  // coordinate descent: each dimension in turn, keeping the best value found
+  auto dim = [&](auto set, std::initializer_list<uint32_t> vals) -> int {
+    for (uint32_t v : vals) {
+      fs_batch_options o = best;
+      set(o, v);
       double sec = 0;
-      int rc = fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, &sec);  // warm
-      if (rc) return rc;
-      rc = fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, &sec);
-      if (rc) return rc;
-      if (sec < best) {
-        best = sec;
-        bo = o;
+      const int rc2 = timed(o, &sec);
+      if (rc2) return rc2;
+      if (sec < tbest) {
+        tbest = sec;
+        best = o;
       }
-     }
     }
+    return FS_OK;
+  };
+  if ((rc = dim([](fs_batch_options& o, uint32_t v) { o.tile_candidates = v; }, {256u, 512u})) ||
+      (rc = dim([&](fs_batch_options& o, uint32_t v) { o.persistent_blocks = cus * v; }, {2u, 4u, 8u})) ||
+      (rc = dim([](fs_batch_options& o, uint32_t v) { o.lpt_order = v; }, {0u, 1u})) ||
+      (rc = dim([](fs_batch_options& o, uint32_t v) { o.unroll = v; }, {2u, 4u, 8u})) ||
+      (rc = dim([](fs_batch_options& o, uint32_t v) { o.lanes_per_prime = v; }, {1u, 2u, 4u})))
+    return rc;
+  if (nlen_max > 4096) {
+    const uint32_t c1 = std::max<uint32_t>(2048, nlen_max / 4), c2 = std::max<uint32_t>(1024, nlen_max / 16);
+    if ((rc = dim([](fs_batch_options& o, uint32_t v) { o.n_chunk = v; }, {0u, c1, c2}))) return rc;
   }
-  *opt = bo;
+  *opt = best;
   return FS_OK;
 }

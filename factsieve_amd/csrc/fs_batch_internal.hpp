@@ -21,9 +21,11 @@ struct fs_bjob {
   uint32_t cand_cap;
   uint32_t pg_first;       // first candidate-generation workgroup of this job
   uint32_t flags;          // FS_BJ_*
-  uint32_t pad;
+  uint32_t parent;         // job whose candidates, counters, sums and factor tag it uses (itself
+                           // unless it is a later n-subrange of a job split over several lanes)
 };
 #define FS_BJ_NEW 1u       // counters cleared and candidates generated in this launch
+#define FS_BJ_NOFOLD 2u    // an n-subrange that ends before the job's nmax: no checksum fold
 
 // per-job flag bits
 #define FS_BJF_CANDOVF 1u
@@ -56,10 +58,13 @@ struct fs_batch_dev {
   uint32_t fcap = 0;
 };
 
-// Enqueue clear -> candidate generation (jobs flagged FS_BJ_NEW) -> tile list
-// (the norder jobs of d.order with their d.chunk n-ranges) -> persistent sieve.
-int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, uint32_t norder, uint32_t pg_blocks, uint32_t tile,
-                    uint32_t blocks, hipStream_t st);
+// Enqueue clear -> candidate generation (jobs flagged FS_BJ_NEW) -> for each of
+// `rounds` rounds: tile list (the norder[r] entries of d.order / d.chunk from
+// offset off[r], their n-ranges) -> persistent sieve.  unroll: 2, 4 or 8 steps
+// per +-1 branch in the iterate loops.
+int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, const std::vector<uint32_t>& norder,
+                    const std::vector<uint32_t>& off, uint32_t pg_blocks, uint32_t tile, uint32_t blocks,
+                    uint32_t unroll, hipStream_t st);
 int fs_batch_fill_ntab(double* t, uint32_t count, hipStream_t st);
 extern "C" void fs_set_hip_error(const char* what, int err);
 
@@ -75,9 +80,12 @@ struct fs_batch_bufs {
 };
 uint32_t fs_device_cus();
 // Upload of a job table / tile order / chunks / programs and the launch.
+// rounds: entry counts of consecutive rounds of order/chunk (empty = one round
+// over all of order).
 int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const std::vector<uint32_t>& order,
                         const std::vector<uint32_t>& chunk, const std::vector<uint64_t>& words, uint32_t pg_blocks,
-                        uint32_t ncands, uint32_t tile, uint32_t blocks, uint32_t fcap, bool graph, float* dev_ms);
+                        uint32_t ncands, uint32_t tile, uint32_t blocks, uint32_t fcap, bool graph, float* dev_ms,
+                        const std::vector<uint32_t>& rounds = {}, uint32_t unroll = 4);
 // Read back per-job counters, flags, sums and the factor records.
 int fs_batch_read(fs_batch_bufs& E, uint32_t nj, std::vector<uint32_t>& cnt, std::vector<uint32_t>& flag,
                   std::vector<uint64_t>& sum, std::vector<fs_factor>& fac, uint32_t* glob_out);

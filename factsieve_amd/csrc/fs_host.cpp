@@ -238,6 +238,7 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
 std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path) {
   static std::mutex mu;
   static std::map<std::pair<uint32_t, int>, std::shared_ptr<const fs_program>> cache;
+  static size_t cached_words = 0;
   const auto key = std::make_pair(nmin, path);
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -252,8 +253,12 @@ std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path) {
     pr->words.assign((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 2, 0);
     fs_setup_program(nmin, path, pr->words.data());
     std::lock_guard<std::mutex> lk(mu);
-    if (cache.size() >= 256) cache.clear();  // holders keep their own reference
-    cache[key] = pr;
+    // bounded: 4096 programs or 2^25 words (256 MB); holders keep their own reference
+    if (cache.size() >= 4096 || cached_words + pr->words.size() > ((size_t)1 << 25)) {
+      cache.clear();
+      cached_words = 0;
+    }
+    if (cache.emplace(key, pr).second) cached_words += pr->words.size();
     return pr;
   } catch (const std::bad_alloc&) {
     fs_set_error_msg("setup program: host allocation failed");

@@ -323,35 +323,45 @@ __device__ __forceinline__ uint64_t fs_cand_setup_mont(uint64_t p, uint64_t q, u
 
 // ---------------------------------------------------------------- iterate
 
+// Unroll depths of the iterate loops: U steps share one +-1 branch and one
+// scalar load of U multipliers.
+#define FS_UNROLL_DEFAULT 4
+
 // FP64 path.  nt[j] = (double)(n0 + 1 + j), read with scalar loads, with at
-// least 4 entries past n1 - n0; six FP64 operations and one compare per step,
-// four steps share one branch on the OR of their +-1 tests.
-template <class Rec>
+// least U entries past n1 - n0; six FP64 operations and one compare per step,
+// U steps (2, 4 or 8) share one branch on the OR of their +-1 tests.
+template <int U = FS_UNROLL_DEFAULT, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                       const double* __restrict__ nt, Rec rec) {
+  static_assert(U == 2 || U == 4 || U == 8, "unroll depth");
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = fs_canon_to_fp(s_in, p);
   uint32_t n = n0;
-  const uint32_t n4 = n0 + ((n1 - n0) & ~3u);
-  double m0 = nt[0], m1 = nt[1], m2 = nt[2], m3 = nt[3];
-  for (; n < n4; n += 4) {
+  const uint32_t nu = n0 + ((n1 - n0) & ~(uint32_t)(U - 1));
+  double m[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) m[k] = nt[k];
+  for (; n < nu; n += U) {
     // next block, loaded one iteration ahead (left to the compiler's placement:
     // pinning it as in the hybrid loop measured slower here)
-    const double* mn = nt + (n - n0) + 4;
-    const double x0 = mn[0], x1 = mn[1], x2 = mn[2], x3 = mn[3];
-    const double a = s;
-    const double b = fs_fp_mulmod(a, m0, pd, pinv);
-    const double d = fs_fp_mulmod(b, m1, pd, pinv);
-    const double e = fs_fp_mulmod(d, m2, pd, pinv);
-    s = fs_fp_mulmod(e, m3, pd, pinv);
-    m0 = x0, m1 = x1, m2 = x2, m3 = x3;
-    const bool hit = (__builtin_fabs(a) == 1.0) | (__builtin_fabs(b) == 1.0) | (__builtin_fabs(d) == 1.0) |
-                     (__builtin_fabs(e) == 1.0);
+    const double* mn = nt + (n - n0) + U;
+    double x[U], v[U + 1];
+#pragma unroll
+    for (int k = 0; k < U; ++k) x[k] = mn[k];
+    v[0] = s;
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k + 1] = fs_fp_mulmod(v[k], m[k], pd, pinv);
+    s = v[U];
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      m[k] = x[k];
+      hit |= (__builtin_fabs(v[k]) == 1.0);
+    }
     if (hit) {
-      if (__builtin_fabs(a) == 1.0) rec(a > 0.0 ? -(int32_t)n : (int32_t)n);
-      if (__builtin_fabs(b) == 1.0) rec(b > 0.0 ? -(int32_t)(n + 1) : (int32_t)(n + 1));
-      if (__builtin_fabs(d) == 1.0) rec(d > 0.0 ? -(int32_t)(n + 2) : (int32_t)(n + 2));
-      if (__builtin_fabs(e) == 1.0) rec(e > 0.0 ? -(int32_t)(n + 3) : (int32_t)(n + 3));
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (__builtin_fabs(v[k]) == 1.0) rec(v[k] > 0.0 ? -(int32_t)(n + k) : (int32_t)(n + k));
     }
   }
   for (; n < n1; ++n) {
@@ -363,42 +373,49 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in,
 
 // Hybrid path (p <= FS_HYBRID_PMAX): signed residue |s| <= 0.51 p, exact
 // 64x32 products mod 2^64 with an FP64 quotient; nt as for the FP64 path.
-// n! = +-1 (mod p) iff s == +-1.  Four steps share one test: the quotient
+// n! = +-1 (mod p) iff s == +-1.  U steps share one test: the quotient
 // estimates y ~ s/p that the steps compute anyway pass |y| <= 1.5/p only for
 // s in {-1, 0, 1}, and the branch then tests s exactly.
-template <class Rec>
+template <int U = FS_UNROLL_DEFAULT, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                         const double* __restrict__ nt, Rec rec) {
+  static_assert(U == 2 || U == 4 || U == 8, "unroll depth");
   const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0, thr = 1.5 * pinv;
   const uint64_t pn = 0 - p, c31 = p << 31;
   int64_t s = fs_canon_to_signed(s_in, p);
   uint32_t n = n0;
-  const uint32_t n4 = n0 + ((n1 - n0) & ~3u);
-  double m0 = nt[0], m1 = nt[1], m2 = nt[2], m3 = nt[3];
-  for (; n < n4; n += 4) {
+  const uint32_t nu = n0 + ((n1 - n0) & ~(uint32_t)(U - 1));
+  double m[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) m[k] = nt[k];
+  for (; n < nu; n += U) {
     // Scalar loads complete out of order (a wait is for all of them): take
     // this block's multipliers first, then issue the next block's load, pinned
     // above this block's steps by the asm that claims to update s.
-    asm volatile("" ::"s"(m0), "s"(m1), "s"(m2), "s"(m3));
-    const double* mn = nt + (n - n0) + 4;
-    const double x0 = mn[0], x1 = mn[1], x2 = mn[2], x3 = mn[3];
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" ::"s"(m[k]));
+    const double* mn = nt + (n - n0) + U;
+    double x[U], y[U];
+    int64_t v[U + 1];
+#pragma unroll
+    for (int k = 0; k < U; ++k) x[k] = mn[k];
     asm volatile("" : "+v"(s)::"memory");
     uint32_t mm = n + 1;
     asm volatile("" : "+s"(mm));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
-    double ya, yb, yd, ye;
-    const int64_t a = s;
-    const int64_t b = fs_sh_mulmod(a, mm, m0, c31, pn, pinv, pinv32, &ya);
-    const int64_t d = fs_sh_mulmod(b, mm + 1, m1, c31, pn, pinv, pinv32, &yb);
-    const int64_t e = fs_sh_mulmod(d, mm + 2, m2, c31, pn, pinv, pinv32, &yd);
-    s = fs_sh_mulmod(e, mm + 3, m3, c31, pn, pinv, pinv32, &ye);
-    m0 = x0, m1 = x1, m2 = x2, m3 = x3;
-    const double ymin = __builtin_fmin(__builtin_fmin(__builtin_fabs(ya), __builtin_fabs(yb)),
-                                       __builtin_fmin(__builtin_fabs(yd), __builtin_fabs(ye)));
+    v[0] = s;
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k + 1] = fs_sh_mulmod(v[k], mm + k, m[k], c31, pn, pinv, pinv32, &y[k]);
+    s = v[U];
+    double ymin = __builtin_fabs(y[0]);
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      m[k] = x[k];
+      if (k) ymin = __builtin_fmin(ymin, __builtin_fabs(y[k]));
+    }
     if (ymin <= thr) {
-      if ((a == 1) | (a == -1)) rec(a == 1 ? -(int32_t)n : (int32_t)n);
-      if ((b == 1) | (b == -1)) rec(b == 1 ? -(int32_t)(n + 1) : (int32_t)(n + 1));
-      if ((d == 1) | (d == -1)) rec(d == 1 ? -(int32_t)(n + 2) : (int32_t)(n + 2));
-      if ((e == 1) | (e == -1)) rec(e == 1 ? -(int32_t)(n + 3) : (int32_t)(n + 3));
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if ((v[k] == 1) | (v[k] == -1)) rec(v[k] == 1 ? -(int32_t)(n + k) : (int32_t)(n + k));
     }
   }
   for (; n < n1; ++n) {

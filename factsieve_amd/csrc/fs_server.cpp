@@ -204,6 +204,7 @@ extern "C" int fs_server_step(fs_server h) {
     fs_bjob& b = bj[k];
     memset(&b, 0, sizeof b);
     b.pg_first = pg;
+    b.parent = k;
     if (s->slots[k] < 0) continue;
     WU& w = s->wus[s->slots[k]];
     b.pmin = w.job.pmin;

@@ -289,6 +289,12 @@ typedef struct {
   uint32_t use_graph;         /* 1: the launches of a batch run as one captured hipGraph */
   uint32_t lpt_order;         /* 1: tiles queued longest job first */
   int verify_threads;         /* CPU threads verifying factors */
+  uint32_t lanes_per_prime;   /* 1..8: each job's n-range cut into this many subranges (of at least
+                                 1024 n) that run on different lanes, each from its own setup (default 1) */
+  uint32_t n_chunk;           /* n values per round of the persistent kernel, residues kept in HBM
+                                 between rounds (0 = whole range in one round; jobs cut over several
+                                 lanes run whole) */
+  uint32_t unroll;            /* iterate steps per +-1 branch: 2, 4 or 8 (0 = 4) */
 } fs_batch_options;
 
 void fs_default_batch_options(fs_batch_options* o);
@@ -301,8 +307,10 @@ void fs_default_batch_options(fs_batch_options* o);
  * kept count per job; seconds: wall time of the call. */
 int fs_batch(const fs_job* jobs, uint32_t njobs, const fs_batch_options* opt, fs_job_result* out,
              fs_factor* factors, uint64_t cap, uint64_t* per_job_nf, double* seconds);
-/* Time the launch shapes (tile size, persistent workgroups, queue order) on a
- * 1/8 p-range sample of the jobs and write the fastest into *opt. */
+/* Time the launch shapes on a 1/8 p-range sample of the jobs and write the
+ * fastest into *opt: one dimension at a time from *opt, in the order tile size
+ * (primes per tile), persistent workgroups, queue order, unroll depth, lanes
+ * per prime and n-chunk length. */
 int fs_batch_autotune(const fs_job* jobs, uint32_t njobs, fs_batch_options* opt);
 
 /* ------------------------------------------------------ serving loop */

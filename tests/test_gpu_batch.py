@@ -56,12 +56,42 @@ def test_batch_small_jobs_match_oracle(gpu):
 
 
 @pytest.mark.parametrize("opts", [dict(tile_candidates=512), dict(persistent_blocks=64), dict(use_graph=0),
-                                  dict(lpt_order=0)])
+                                  dict(lpt_order=0), dict(unroll=2), dict(unroll=8), dict(n_chunk=700),
+                                  dict(n_chunk=1500, use_graph=0), dict(lanes_per_prime=2),
+                                  dict(lanes_per_prime=8, unroll=8, n_chunk=3000)])
 def test_batch_shape_invariance(gpu, opts):
     jobs = small_mix()
     base, _ = gpu.batch(jobs)
+    print("options", opts, flush=True)
     other, _ = gpu.batch(jobs, **opts)
     assert [outcome(o) for o in base] == [outcome(o) for o in other]
+
+
+def long_n_mix():
+    # n-ranges long enough to be cut over several lanes per prime (subranges of >= 1024 n)
+    return [
+        (10 ** 8, 10 ** 8 + 200000, 1000, 9000),        # FP64, factors present
+        (2 ** 40, 2 ** 40 + 400000, 50000, 60000),      # FP64, setup-heavy subranges
+        (2 ** 60, 2 ** 60 + 300000, 20000, 26000),      # hybrid below 2^64/3
+        (2 ** 62 + 2 ** 61, 2 ** 62 + 2 ** 61 + 200000, 4000, 9000),  # hybrid between 2^64/3 and 2^63
+        (2 ** 63 + 1000, 2 ** 63 + 100000, 3000, 9000), # hybrid above 2^63
+        (127, 20000, 127, 30000),                       # p <= n: subranges that start at n >= p
+        (2 ** 64 - 100000, 2 ** 64 - 1, 500, 4700),     # Montgomery
+        (10 ** 9, 10 ** 9 + 100000, 5000, 5000),        # empty n-range
+    ]
+
+
+@pytest.mark.parametrize("lanes,unroll", [(2, 4), (3, 2), (4, 8), (8, 4)])
+def test_batch_lanes_per_prime_match_plain(gpu, lanes, unroll):
+    """A job cut into n-subranges on several lanes per prime (each subrange
+    from its own setup program, only the last one folding the checksum) gives
+    the plain sieve's factors, candidate count and checksum."""
+    jobs = long_n_mix()
+    print("lanes", lanes, "unroll", unroll, flush=True)
+    res, _ = gpu.batch(jobs, lanes_per_prime=lanes, unroll=unroll)
+    for job, o in zip(jobs, res):
+        assert o.status == 0, job
+        assert outcome(o) == plain(gpu, job), job
 
 
 def test_batch_invalid_job_is_isolated(gpu):
@@ -75,8 +105,11 @@ def test_batch_autotune_returns_valid_shape(gpu):
     jobs = small_mix()[:8]
     o = gpu.batch_autotune(jobs)
     assert o["tile_candidates"] in (256, 512) and o["persistent_blocks"] > 0
+    assert o["unroll"] in (2, 4, 8) and o["lanes_per_prime"] in (1, 2, 4) and o["lpt_order"] in (0, 1)
     res, _ = gpu.batch(jobs, **o)
     assert all(r.status == 0 for r in res)
+    base, _ = gpu.batch(jobs)
+    assert [outcome(r) for r in res] == [outcome(r) for r in base]
 
 
 @pytest.mark.parametrize("budget,chunk", [(0, 700), (64 << 20, 5000)])
