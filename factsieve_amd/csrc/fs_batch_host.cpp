@@ -220,7 +220,7 @@ extern "C" void fs_default_batch_options(fs_batch_options* o) {
   o->verify_threads = 16;
   o->lanes_per_prime = 1;
   o->n_chunk = 0;
-  o->unroll = 4;
+  o->unroll = 8;
 }
 
 namespace {
@@ -335,7 +335,7 @@ int run_batch(fs_batch_bufs& E, const fs_job* jobs, uint32_t nj, std::vector<Pla
   }
   if (order.empty()) rounds.clear();
   const uint32_t blocks = o.persistent_blocks ? o.persistent_blocks : fs_device_cus() * 8;
-  const uint32_t unroll = (o.unroll == 2 || o.unroll == 8) ? o.unroll : 4;
+  const uint32_t unroll = (o.unroll == 2 || o.unroll == 4) ? o.unroll : 8;
   int rc = fs_batch_run_device(E, bj, order, chunk, words, pg, cand_off, tile, blocks, fcap, o.use_graph != 0, dev_ms,
                                rounds.empty() ? std::vector<uint32_t>{0} : rounds, unroll);
   if (rc) return rc;

@@ -85,7 +85,7 @@ uint32_t fs_device_cus();
 int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const std::vector<uint32_t>& order,
                         const std::vector<uint32_t>& chunk, const std::vector<uint64_t>& words, uint32_t pg_blocks,
                         uint32_t ncands, uint32_t tile, uint32_t blocks, uint32_t fcap, bool graph, float* dev_ms,
-                        const std::vector<uint32_t>& rounds = {}, uint32_t unroll = 4);
+                        const std::vector<uint32_t>& rounds = {}, uint32_t unroll = 8);
 // Read back per-job counters, flags, sums and the factor records.
 int fs_batch_read(fs_batch_bufs& E, uint32_t nj, std::vector<uint32_t>& cnt, std::vector<uint32_t>& flag,
                   std::vector<uint64_t>& sum, std::vector<fs_factor>& fac, uint32_t* glob_out);

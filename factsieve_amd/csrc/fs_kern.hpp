@@ -325,7 +325,7 @@ __device__ __forceinline__ uint64_t fs_cand_setup_mont(uint64_t p, uint64_t q, u
 
 // Unroll depths of the iterate loops: U steps share one +-1 branch and one
 // scalar load of U multipliers.
-#define FS_UNROLL_DEFAULT 4
+#define FS_UNROLL_DEFAULT 8
 
 // FP64 path.  nt[j] = (double)(n0 + 1 + j), read with scalar loads, with at
 // least U entries past n1 - n0; six FP64 operations and one compare per step,

@@ -294,7 +294,7 @@ typedef struct {
   uint32_t n_chunk;           /* n values per round of the persistent kernel, residues kept in HBM
                                  between rounds (0 = whole range in one round; jobs cut over several
                                  lanes run whole) */
-  uint32_t unroll;            /* iterate steps per +-1 branch: 2, 4 or 8 (0 = 4) */
+  uint32_t unroll;            /* iterate steps per +-1 branch: 2, 4 or 8 (default and 0: 8) */
 } fs_batch_options;
 
 void fs_default_batch_options(fs_batch_options* o);
