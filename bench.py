@@ -14,7 +14,7 @@ roofline: the dominant kernel is the residue iteration.  Its work is counted
 as 7 VALU lane-operations per p*n step (the FP64 formulation: 2 multiplies,
 2 FMAs, 1 add, 1 round, 1 compare), timed with HIP events on the launch
 stream (12 on the hybrid path of p > 2^53: 6 FP64 quotient ops, 2
-v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, 1 for the +-1 filter: 4 ops per 4 steps);
+v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, 1 for the +-1 filter: 8 ops per 8 steps);
 peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
 vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
 int32_mul_equiv: the same steps/s expressed in the 11 32-bit multiplies a
