@@ -14,7 +14,9 @@ roofline: the dominant kernel is the residue iteration.  Its work is counted
 as 7 VALU lane-operations per p*n step (the FP64 formulation: 2 multiplies,
 2 FMAs, 1 add, 1 round, 1 compare), timed with HIP events on the launch
 stream (12 on the hybrid path of p > 2^53: 6 FP64 quotient ops, 2
-v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, 1 for the +-1 filter: 8 ops per 8 steps);
+v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, 1 for the +-1 filter: 8 ops per 8 steps;
+10 when every modulus is at least nmax * 2^39 and the quotient comes from the
+high word of the residue alone: 4 FP64 ops instead of 6);
 peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
 vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
 int32_mul_equiv: the same steps/s expressed in the 11 32-bit multiplies a
@@ -44,6 +46,13 @@ WORKLOADS = {
 }
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine clock
 OPS_PER_STEP = {1: 7, 2: 12, 3: 45}             # VALU lane-ops per p*n step by path (from the ISA)
+OPS_PER_STEP_HYBRID_HI = 10                     # hybrid step with the high-word quotient
+
+
+def hybrid_hi(pmin, pmax, nmax):
+    """The hybrid iterate takes its quotient from the high word of the residue
+    when every modulus p satisfies p >= nmax * 2^39 and p < 2^64 - 2^58."""
+    return (pmin >> 39) >= nmax and pmax <= 2 ** 64 - 2 ** 58
 INT32_MUL_PER_STEP = 11                         # 64-bit Montgomery step: 4 + 3 + 4 partial products
 INT32_MUL_PEAK = VALU_PEAK_TOPS * 1e12          # full-rate v_mad_u64_u32 / v_mul_*_u32 (measured)
 # HBM bytes per candidate per iterate launch, measured with rocprofv3 --pmc FETCH_SIZE
@@ -198,6 +207,8 @@ def main():
     launches = sum(r.iterate_launches for r in results)
     avg_launch_ms = it_ms / max(launches, 1)
     ops = OPS_PER_STEP[ref.path]
+    if ref.path == 2 and hybrid_hi(pmin, pmax, nmax):
+        ops = OPS_PER_STEP_HYBRID_HI
     achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
     traffic = None
     if ref.path in TRAFFIC_BYTES_PER_CAND:

@@ -160,6 +160,37 @@ FS_HD int64_t fs_sh_mulmod(int64_t s, uint32_t m, double md, uint64_t c31, uint6
   return fs_sh_mulmod(s, m, md, c31, pn, pinv, pinv32, &y);
 }
 
+// The signed step with the quotient from the high word of s only, for moduli
+// far above the multiplier: p >= m * 2^FS_SH_HI_SHIFT and p < FS_SH_HI_PMAX.
+// y = s1 * 2^32/p misses s/p by s0/p < 2^32/p, so y*m misses X = s*m/p by
+// less than 2^32 m / p <= 2^-7 plus the rounding terms of fs_sh_mulmod
+// (< 2^-21 for m < 2^25), and r = s*m - (k - 2^31) p has
+// |r| <= p/2 + 2^32 m + 2^-21 p <= (1/2 + 2^-7 + 2^-21) p: below 0.51 p (the
+// step's input bound) and, for p < 2^64 - 2^58, below 2^63 -- the same
+// invariant with two FP64 operations fewer (no conversion of s0, no FMA).
+// *yq = y is a filter for s == +-1: s in {-1, 0, 1} has s1 in {-1, 0}, so
+// |y| <= 1.5 * pinv32 holds for it (and also for other |s| < 2^33, which the
+// exact test then rejects).
+#define FS_SH_HI_SHIFT 39
+#define FS_SH_HI_PMAX (0xFFFFFFFFFFFFFFFFull - ((uint64_t)1 << 58) + 1)
+FS_HD bool fs_sh_hi_ok(uint64_t p, uint32_t mmax) {
+  return p < FS_SH_HI_PMAX && (p >> FS_SH_HI_SHIFT) >= (uint64_t)mmax;
+}
+
+FS_HD int64_t fs_sh_mulmod_hi(int64_t s, uint32_t m, double md, uint64_t c31, uint64_t pn, double pinv32,
+                              double* yq) {
+#pragma clang fp contract(off)
+  const uint32_t s0 = (uint32_t)s;
+  const int32_t s1 = (int32_t)(s >> 32);
+  const double y = (double)s1 * pinv32;
+  *yq = y;
+  const uint32_t k = (uint32_t)__builtin_fma(y, md, 2147483648.5);
+  const uint64_t a = (uint64_t)s0 * m + c31;
+  const uint64_t b = (uint64_t)k * (uint32_t)pn + a;
+  const uint32_t hi = (uint32_t)((uint64_t)(uint32_t)s1 * m + (b >> 32)) + k * (uint32_t)(pn >> 32);
+  return (int64_t)(((uint64_t)hi << 32) | (uint32_t)b);
+}
+
 // Symmetric signed residue from a canonical one, and back.
 FS_HD int64_t fs_canon_to_signed(uint64_t r, uint64_t p) { return r > (p >> 1) ? (int64_t)(r - p) : (int64_t)r; }
 FS_HD uint64_t fs_signed_to_canon(int64_t s, uint64_t p) { return s < 0 ? (uint64_t)s + p : (uint64_t)s; }

@@ -37,6 +37,18 @@ extern "C" int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, in
   return FS_E_ARG;
 }
 
+extern "C" int fs_hostemu_mulmod_hi(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res) {
+  if ((p & 1) == 0 || p > FS_HYBRID_PMAX || m == 0 || !fs_sh_hi_ok(p, m) || !raw || !res) return FS_E_ARG;
+  const uint64_t as = s < 0 ? (uint64_t)0 - (uint64_t)s : (uint64_t)s;
+  if ((double)as > 0.51 * (double)p) return FS_E_ARG;
+  const double pinv32 = (1.0 / (double)p) * 4294967296.0;
+  double y;
+  const int64_t r = fs_sh_mulmod_hi(s, m, (double)m, p << 31, 0 - p, pinv32, &y);
+  *raw = r;
+  *res = fs_signed_to_canon(r, p);
+  return FS_OK;
+}
+
 extern "C" int fs_hostemu_setup_mul(uint64_t p, uint64_t s, uint64_t m, int square, uint64_t* raw, uint64_t* res,
                                     uint32_t* shift) {
   if ((p & 1) == 0 || p < 3 || p > FS_HYBRID_PMAX || m >= ((uint64_t)1 << 60) || !raw || !res || !shift)
@@ -165,13 +177,16 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     *res = fs_fp_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {
     if (p > FS_HYBRID_PMAX) return FS_E_ARG;
+    // the device loops' choice (per wave there; one modulus here) and filter
+    const bool hi = fs_sh_hi_ok(p, n1);
     const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
-    const double thr = 1.5 * pinv;  // the device loops' filter on the quotient estimate
+    const double thr = 1.5 * (hi ? pinv32 : pinv);
     int64_t s = fs_canon_to_signed(*res, p);
     for (uint32_t n = n0; n < n1; ++n) {
       const uint32_t m = n + 1;
       double y;
-      const int64_t t = fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32, &y);
+      const int64_t t = hi ? fs_sh_mulmod_hi(s, m, (double)m, p << 31, 0 - p, pinv32, &y)
+                           : fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32, &y);
       if ((s == 1) | (s == -1)) {
         if (!(__builtin_fabs(y) <= thr)) return FS_E_VALIDATE;  // the filter would have missed it
         rec(s == 1 ? -(int32_t)n : (int32_t)n);

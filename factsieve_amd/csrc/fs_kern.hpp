@@ -375,12 +375,14 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in,
 // 64x32 products mod 2^64 with an FP64 quotient; nt as for the FP64 path.
 // n! = +-1 (mod p) iff s == +-1.  U steps share one test: the quotient
 // estimates y ~ s/p that the steps compute anyway pass |y| <= 1.5/p only for
-// s in {-1, 0, 1}, and the branch then tests s exactly.
-template <int U = FS_UNROLL_DEFAULT, class Rec>
-__device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
-                                                        const double* __restrict__ nt, Rec rec) {
+// s in {-1, 0, 1}, and the branch then tests s exactly.  HI: the quotient from
+// the high word of s (fs_sh_mulmod_hi, for p >= n1 * 2^39), whose estimates
+// pass |y| <= 1.5 * 2^32/p for every |s| < 2^33.
+template <int U, bool HI, class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_hybrid_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
+                                                             const double* __restrict__ nt, Rec rec) {
   static_assert(U == 2 || U == 4 || U == 8, "unroll depth");
-  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0, thr = 1.5 * pinv;
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0, thr = 1.5 * (HI ? pinv32 : pinv);
   const uint64_t pn = 0 - p, c31 = p << 31;
   int64_t s = fs_canon_to_signed(s_in, p);
   uint32_t n = n0;
@@ -404,7 +406,9 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_i
     asm volatile("" : "+s"(mm));  // keep the multiplier a 32-bit scalar (no 64-bit induction variable)
     v[0] = s;
 #pragma unroll
-    for (int k = 0; k < U; ++k) v[k + 1] = fs_sh_mulmod(v[k], mm + k, m[k], c31, pn, pinv, pinv32, &y[k]);
+    for (int k = 0; k < U; ++k)
+      v[k + 1] = HI ? fs_sh_mulmod_hi(v[k], mm + k, m[k], c31, pn, pinv32, &y[k])
+                    : fs_sh_mulmod(v[k], mm + k, m[k], c31, pn, pinv, pinv32, &y[k]);
     s = v[U];
     double ymin = __builtin_fabs(y[0]);
 #pragma unroll
@@ -422,9 +426,19 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_i
     if ((s == 1) | (s == -1)) rec(s == 1 ? -(int32_t)n : (int32_t)n);
     uint32_t mm = n + 1;
     asm volatile("" : "+s"(mm));
-    s = fs_sh_mulmod(s, mm, nt[n - n0], c31, pn, pinv, pinv32);
+    double y;
+    s = HI ? fs_sh_mulmod_hi(s, mm, nt[n - n0], c31, pn, pinv32, &y)
+           : fs_sh_mulmod(s, mm, nt[n - n0], c31, pn, pinv, pinv32, &y);
   }
   return fs_signed_to_canon(s, p);
+}
+
+// Per wave: the high-word quotient when every active lane's modulus allows it.
+template <int U = FS_UNROLL_DEFAULT, class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
+                                                        const double* __restrict__ nt, Rec rec) {
+  if (__all(fs_sh_hi_ok(p, n1))) return fs_cand_iter_hybrid_loop<U, true>(p, s_in, n0, n1, nt, rec);
+  return fs_cand_iter_hybrid_loop<U, false>(p, s_in, n0, n1, nt, rec);
 }
 
 // Reference formulation (iterate.cl:54-80): Montgomery residue s6 and

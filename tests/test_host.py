@@ -283,6 +283,50 @@ def test_residue_step_at_error_bound_edges(fs):
             assert raw % p == res and abs(raw) <= (0.5 + 2 ** -19) * p, (p, s, m, raw)
 
 
+def test_hybrid_hi_step_at_error_bound_edges(fs):
+    """The hybrid step with the quotient from the high word of s only (used
+    when p >= n1 * 2^39): exact, and |s'| stays within 0.51p, at the largest
+    multiplier the bound allows, |s| up to 0.51p, and the largest dropped low
+    word s0 = 2^32 - 1."""
+    rng = random.Random(23)
+    top = 2 ** 64 - 2 ** 58          # first modulus the high-word step does not take
+    ps = [top - 1, top - 3, 2 ** 63 + 29, 2 ** 63 - 25, 2 ** 62 + 135, 2 ** 58 + 27, 2 ** 50 + 55,
+          (2 ** 19 << 39) + 1]
+    for p in ps:
+        mtop = min(p >> 39, 2 ** 31 - 1)
+        smax = min(int(0.51 * p) - 2 ** 12, 2 ** 63 - 1)
+        ms = [mtop, mtop - 1, max(1, mtop // 2), 1] + [rng.randrange(1, mtop + 1) for _ in range(150)]
+        s0max = (((smax >> 32) - 1) << 32) | 0xFFFFFFFF    # largest dropped low word, |s| <= smax
+        ss = [smax, -smax, 1, -1, 0, s0max, -s0max, -(((smax >> 32) - 1) << 32)] + \
+             [rng.randrange(-smax, smax + 1) for _ in range(150)]
+        for s in ss:
+            for m in ms[:4] + [ms[4 + (s % 150)]]:
+                raw, res = fs.hostemu_mulmod_hi(p, s, m)
+                assert res == (s * m) % p, (p, s, m)
+                assert raw % p == res and abs(raw) <= 0.51 * p, (p, s, m, raw)
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_mulmod_hi(2 ** 58 + 27, 5, (2 ** 58 + 27 >> 39) + 1)   # multiplier above p / 2^39
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_mulmod_hi(top + 1, 5, 3)                                # |s'| could reach 2^63
+
+
+def test_hybrid_hi_filter_sound(fs):
+    """With the high-word quotient the +-1 filter is |y| <= 1.5 * 2^32/p; a
+    residue of +-1 must never fail it (the host execution returns an error if
+    it does).  n-ranges chosen so that p >= n1 * 2^39 selects that loop."""
+    rng = random.Random(29)
+    ps = [2 ** 64 - 2 ** 58 - 1, 2 ** 63 + 29, 2 ** 62 + 135, 2 ** 58 + 27, 2 ** 45 + 59]
+    ps += [rng.randrange(2 ** 45, 2 ** 64 - 2 ** 58) | 1 for _ in range(200)]
+    for p in ps:
+        top = min(p >> 39, 2 ** 31 - 1)
+        for start in (1, p - 1):
+            n0 = rng.randrange(1, max(2, top - 4))
+            if n0 + 3 > top:
+                continue
+            _, facs, nf = fs.hostemu_iterate(fs.PATH_HYBRID, p, n0, n0 + 3, start)
+            assert nf >= 1 and facs[0] == (p, (-1 if start == 1 else 1) * n0), p
+
+
 def test_hybrid_quotient_filter_sound(fs):
     """The hybrid iterate tests s == +-1 only where the quotient estimate
     passes |y| <= 1.5/p; the host execution returns an error if a residue of
