@@ -16,7 +16,8 @@ as 7 VALU lane-operations per p*n step (the FP64 formulation: 2 multiplies,
 stream (12 on the hybrid path of p > 2^53: 6 FP64 quotient ops, 2
 v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, 1 for the +-1 filter: 8 ops per 8 steps;
 10 when every modulus is at least nmax * 2^39 and the quotient comes from the
-high word of the residue alone: 4 FP64 ops instead of 6);
+high word of the residue alone: 4 FP64 ops instead of 6; 5 on the FP64 path
+when p * nmax <= 2^52 and the product s*m is exact);
 peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
 vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
 int32_mul_equiv: the same steps/s expressed in the 11 32-bit multiplies a
@@ -47,6 +48,7 @@ WORKLOADS = {
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine clock
 OPS_PER_STEP = {1: 7, 2: 12, 3: 45}             # VALU lane-ops per p*n step by path (from the ISA)
 OPS_PER_STEP_HYBRID_HI = 10                     # hybrid step with the high-word quotient
+OPS_PER_STEP_FP64_EXACT = 5                     # FP64 step with an exact product (p * nmax <= 2^52)
 
 
 def hybrid_hi(pmin, pmax, nmax):
@@ -209,6 +211,8 @@ def main():
     ops = OPS_PER_STEP[ref.path]
     if ref.path == 2 and hybrid_hi(pmin, pmax, nmax):
         ops = OPS_PER_STEP_HYBRID_HI
+    if ref.path == 1 and pmax * nmax <= 2 ** 52:
+        ops = OPS_PER_STEP_FP64_EXACT
     achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
     traffic = None
     if ref.path in TRAFFIC_BYTES_PER_CAND:

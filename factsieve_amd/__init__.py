@@ -141,6 +141,7 @@ _SIGS = [
                                           ctypes.POINTER(Factor), u32, ctypes.POINTER(u32)]),
     ("fs_hostemu_candidates", i64, [u64, u64, vp, u64]),
     ("fs_hostemu_mulmod", ctypes.c_int, [ctypes.c_int, u64, i64, u64, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
+    ("fs_hostemu_fp_exact", ctypes.c_int, [u64, i64, u32, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
     ("fs_hostemu_mulmod_hi", ctypes.c_int, [u64, i64, u32, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
     ("fs_hostemu_setup_mul", ctypes.c_int, [u64, u64, u64, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(u64),
                                             ctypes.POINTER(u32)]),
@@ -281,6 +282,14 @@ def hostemu_mulmod(path, p, s, m):
     """One residue step s*m mod p of a path on the host: (raw signed value, canonical)."""
     raw, res = i64(), u64()
     check(lib().fs_hostemu_mulmod(path, p, s, m, ctypes.byref(raw), ctypes.byref(res)))
+    return raw.value, res.value
+
+
+def hostemu_fp_exact(p, s, m):
+    """One FP64 residue step without the product's error term (p * m <= 2^52)
+    on the host: (raw signed value, canonical)."""
+    raw, res = i64(), u64()
+    check(lib().fs_hostemu_fp_exact(p, s, m, ctypes.byref(raw), ctypes.byref(res)))
     return raw.value, res.value
 
 

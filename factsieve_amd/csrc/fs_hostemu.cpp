@@ -37,6 +37,17 @@ extern "C" int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, in
   return FS_E_ARG;
 }
 
+extern "C" int fs_hostemu_fp_exact(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res) {
+  if ((p & 1) == 0 || p < 7 || m == 0 || !fs_fp_exact_ok(p, m) || !raw || !res) return FS_E_ARG;
+  const uint64_t as = s < 0 ? (uint64_t)0 - (uint64_t)s : (uint64_t)s;
+  if ((double)as >= 0.8 * (double)p) return FS_E_ARG;
+  const double pd = (double)p;
+  const double r = fs_fp_mulmod_exact((double)s, (double)m, pd, 1.0 / pd);
+  *raw = (int64_t)r;
+  *res = fs_fp_to_canon(r, p);
+  return FS_OK;
+}
+
 extern "C" int fs_hostemu_mulmod_hi(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res) {
   if ((p & 1) == 0 || p > FS_HYBRID_PMAX || m == 0 || !fs_sh_hi_ok(p, m) || !raw || !res) return FS_E_ARG;
   const uint64_t as = s < 0 ? (uint64_t)0 - (uint64_t)s : (uint64_t)s;
@@ -168,11 +179,12 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
   };
   if (path == FS_PATH_FP64) {
     if (p >= ((uint64_t)1 << 53)) return FS_E_ARG;
+    const bool exact = fs_fp_exact_ok(p, n1);  // the device loops' choice (per wave there)
     const double pd = (double)p, pinv = 1.0 / pd;
     double s = fs_canon_to_fp(*res, p);
     for (uint32_t n = n0; n < n1; ++n) {
       if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
-      s = fs_fp_mulmod(s, (double)(n + 1), pd, pinv);
+      s = exact ? fs_fp_mulmod_exact(s, (double)(n + 1), pd, pinv) : fs_fp_mulmod(s, (double)(n + 1), pd, pinv);
     }
     *res = fs_fp_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {

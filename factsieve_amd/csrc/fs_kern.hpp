@@ -328,11 +328,12 @@ __device__ __forceinline__ uint64_t fs_cand_setup_mont(uint64_t p, uint64_t q, u
 #define FS_UNROLL_DEFAULT 8
 
 // FP64 path.  nt[j] = (double)(n0 + 1 + j), read with scalar loads, with at
-// least U entries past n1 - n0; six FP64 operations and one compare per step,
-// U steps (2, 4 or 8) share one branch on the OR of their +-1 tests.
-template <int U = FS_UNROLL_DEFAULT, class Rec>
-__device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
-                                                      const double* __restrict__ nt, Rec rec) {
+// least U entries past n1 - n0; six FP64 operations and one compare per step
+// (four when p * n1 <= 2^52 and the product s*m is exact: EXACT), U steps
+// (2, 4 or 8) share one branch on the OR of their +-1 tests.
+template <int U, bool EXACT, class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
+                                                           const double* __restrict__ nt, Rec rec) {
   static_assert(U == 2 || U == 4 || U == 8, "unroll depth");
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = fs_canon_to_fp(s_in, p);
@@ -350,7 +351,8 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in,
     for (int k = 0; k < U; ++k) x[k] = mn[k];
     v[0] = s;
 #pragma unroll
-    for (int k = 0; k < U; ++k) v[k + 1] = fs_fp_mulmod(v[k], m[k], pd, pinv);
+    for (int k = 0; k < U; ++k)
+      v[k + 1] = EXACT ? fs_fp_mulmod_exact(v[k], m[k], pd, pinv) : fs_fp_mulmod(v[k], m[k], pd, pinv);
     s = v[U];
     bool hit = false;
 #pragma unroll
@@ -366,9 +368,17 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in,
   }
   for (; n < n1; ++n) {
     if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
-    s = fs_fp_mulmod(s, nt[n - n0], pd, pinv);
+    s = EXACT ? fs_fp_mulmod_exact(s, nt[n - n0], pd, pinv) : fs_fp_mulmod(s, nt[n - n0], pd, pinv);
   }
   return fs_fp_to_canon(s, p);
+}
+
+// Per wave: the exact-product step when every active lane's modulus allows it.
+template <int U = FS_UNROLL_DEFAULT, class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
+                                                      const double* __restrict__ nt, Rec rec) {
+  if (__all(fs_fp_exact_ok(p, n1))) return fs_cand_iter_fp64_loop<U, true>(p, s_in, n0, n1, nt, rec);
+  return fs_cand_iter_fp64_loop<U, false>(p, s_in, n0, n1, nt, rec);
 }
 
 // Hybrid path (p <= FS_HYBRID_PMAX): signed residue |s| <= 0.51 p, exact

@@ -283,6 +283,27 @@ def test_residue_step_at_error_bound_edges(fs):
             assert raw % p == res and abs(raw) <= (0.5 + 2 ** -19) * p, (p, s, m, raw)
 
 
+def test_fp64_exact_step_at_error_bound_edges(fs):
+    """The FP64 step without the product's error term (used when p * n1 <=
+    2^52, so |s*m| < 0.8 * 2^52): exact, |s'| <= 0.51p, at p * m = 2^52 and
+    |s| just below 0.8p."""
+    rng = random.Random(31)
+    for p in [2 ** 26 + 15, 2 ** 32 - 5, 2 ** 40 + 15, 2 ** 45 + 59, 2 ** 51 - 129, 1000003, 127, 7]:
+        mtop = min(2 ** 52 // p, 2 ** 32 - 1)
+        smax = int(0.8 * p) - 1 if p > 100 else int(0.8 * p)
+        while smax >= 0.8 * p:
+            smax -= 1
+        ms = [mtop, mtop - 1 if mtop > 1 else 1, 1] + [rng.randrange(1, mtop + 1) for _ in range(100)]
+        ss = [smax, -smax, 1, -1, 0] + [rng.randrange(-smax, smax + 1) for _ in range(100)]
+        for s in ss:
+            for m in ms[:3] + [ms[3 + (s % 100)]]:
+                raw, res = fs.hostemu_fp_exact(p, s, m)
+                assert res == (s * m) % p, (p, s, m)
+                assert raw % p == res and abs(raw) <= 0.51 * p, (p, s, m, raw)
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_fp_exact(2 ** 40 + 15, 5, 2 ** 52 // (2 ** 40 + 15) + 1)   # p * m above 2^52
+
+
 def test_hybrid_hi_step_at_error_bound_edges(fs):
     """The hybrid step with the quotient from the high word of s only (used
     when p >= n1 * 2^39): exact, and |s'| stays within 0.51p, at the largest
