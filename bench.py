@@ -11,13 +11,14 @@ p*n residue steps per job).  `value` is p*n residue steps per second of whole
 jobs.  One JSON line is printed.
 
 roofline: the dominant kernel is the residue iteration.  Its work is counted
-as 7 VALU lane-operations per p*n step (the FP64 formulation: 2 multiplies,
-2 FMAs, 1 add, 1 round, 1 compare), timed with HIP events on the launch
-stream (12 on the hybrid path of p > 2^53: 6 FP64 quotient ops, 2
-v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, 1 for the +-1 filter: 8 ops per 8 steps;
-10 when every modulus is at least nmax * 2^39 and the quotient comes from the
-high word of the residue alone: 4 FP64 ops instead of 6; 5 on the FP64 path
-when p * nmax <= 2^52 and the product s*m is exact);
+in VALU lane-operations per p*n step of the compiled 8-step loop: 6.5 on the
+FP64 path (2 multiplies, 2 FMAs, 1 add, 1 round per step, and one compare per
+two steps: |v[k+1]| <= m[k] covers +-1 at k and k+1), timed with HIP events on
+the launch stream; 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
+2 v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, and 6 filter ops per 8 steps), 9.625
+when every modulus is at least nmax * 2^39 and the quotient comes from the
+high word of the residue alone (4 FP64 ops instead of 6), and 4.5 on the FP64
+path when p * nmax <= 2^52 and the product s*m is exact;
 peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
 vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
 int32_mul_equiv: the same steps/s expressed in the 11 32-bit multiplies a
@@ -46,9 +47,11 @@ WORKLOADS = {
            "full-width modulus stress: n in [5e5,5.1e5), p in [2^62,2^62+2^30)"),
 }
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine clock
-OPS_PER_STEP = {1: 7, 2: 12, 3: 45}             # VALU lane-ops per p*n step by path (from the ISA)
-OPS_PER_STEP_HYBRID_HI = 10                     # hybrid step with the high-word quotient
-OPS_PER_STEP_FP64_EXACT = 5                     # FP64 step with an exact product (p * nmax <= 2^52)
+# VALU lane-ops per p*n step by path, from the compiled 8-step loops: FP64 52
+# per 8 steps, hybrid 94, Montgomery ~45 per step
+OPS_PER_STEP = {1: 6.5, 2: 11.75, 3: 45}
+OPS_PER_STEP_HYBRID_HI = 9.625                  # hybrid with the high-word quotient: 77 per 8 steps
+OPS_PER_STEP_FP64_EXACT = 4.5                   # FP64 with an exact product (p * nmax <= 2^52): 36 per 8
 
 
 def hybrid_hi(pmin, pmax, nmax):

@@ -10,6 +10,7 @@
 #include <vector>
 #include "fs_arith.hpp"
 #include "fs_internal.hpp"
+#include "fs_kern.hpp"
 #include "../../include/factsieve.h"
 
 extern "C" int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, int64_t* raw, uint64_t* res) {
@@ -177,33 +178,67 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     }
     ++cnt;
   };
+  // The device loops' blocks of U = 8 steps from n0 with their +-1 filter (one
+  // test per two residues, see fs_kern.hpp), then single exact steps; a +-1
+  // residue inside a block whose filter does not fire is an error.
+  constexpr uint32_t U = FS_UNROLL_DEFAULT;
+  const uint32_t nu = n0 + ((n1 - n0) & ~(U - 1));
   if (path == FS_PATH_FP64) {
     if (p >= ((uint64_t)1 << 53)) return FS_E_ARG;
     const bool exact = fs_fp_exact_ok(p, n1);  // the device loops' choice (per wave there)
     const double pd = (double)p, pinv = 1.0 / pd;
+    auto step = [&](double s, uint32_t m) {
+      return exact ? fs_fp_mulmod_exact(s, (double)m, pd, pinv) : fs_fp_mulmod(s, (double)m, pd, pinv);
+    };
     double s = fs_canon_to_fp(*res, p);
-    for (uint32_t n = n0; n < n1; ++n) {
+    uint32_t n = n0;
+    for (; n < nu; n += U) {
+      double v[U + 1];
+      v[0] = s;
+      for (uint32_t k = 0; k < U; ++k) v[k + 1] = step(v[k], n + 1 + k);
+      bool hit = false;
+      for (uint32_t k = 0; k < U; k += 2) hit |= __builtin_fabs(v[k + 1]) <= (double)(n + 1 + k);
+      for (uint32_t k = 0; k < U; ++k) {
+        if (__builtin_fabs(v[k]) != 1.0) continue;
+        if (!hit) return FS_E_VALIDATE;  // the filter would have missed it
+        rec(v[k] > 0.0 ? -(int32_t)(n + k) : (int32_t)(n + k));
+      }
+      s = v[U];
+    }
+    for (; n < n1; ++n) {
       if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
-      s = exact ? fs_fp_mulmod_exact(s, (double)(n + 1), pd, pinv) : fs_fp_mulmod(s, (double)(n + 1), pd, pinv);
+      s = step(s, n + 1);
     }
     *res = fs_fp_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {
     if (p > FS_HYBRID_PMAX) return FS_E_ARG;
-    // the device loops' choice (per wave there; one modulus here) and filter
-    const bool hi = fs_sh_hi_ok(p, n1);
+    const bool hi = fs_sh_hi_ok(p, n1);  // the device loops' choice (per wave there)
     const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
-    const double thr = 1.5 * (hi ? pinv32 : pinv);
+    auto step = [&](int64_t s, uint32_t m, double* y) {
+      return hi ? fs_sh_mulmod_hi(s, m, (double)m, p << 31, 0 - p, pinv32, y)
+                : fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32, y);
+    };
     int64_t s = fs_canon_to_signed(*res, p);
-    for (uint32_t n = n0; n < n1; ++n) {
-      const uint32_t m = n + 1;
-      double y;
-      const int64_t t = hi ? fs_sh_mulmod_hi(s, m, (double)m, p << 31, 0 - p, pinv32, &y)
-                           : fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32, &y);
-      if ((s == 1) | (s == -1)) {
-        if (!(__builtin_fabs(y) <= thr)) return FS_E_VALIDATE;  // the filter would have missed it
-        rec(s == 1 ? -(int32_t)n : (int32_t)n);
+    uint32_t n = n0;
+    for (; n < nu; n += U) {
+      int64_t v[U + 1];
+      double y[U];
+      v[0] = s;
+      for (uint32_t k = 0; k < U; ++k) v[k + 1] = step(v[k], n + 1 + k, &y[k]);
+      double ymin = __builtin_fabs(y[1]);
+      for (uint32_t k = 3; k < U; k += 2) ymin = __builtin_fmin(ymin, __builtin_fabs(y[k]));
+      const double thr = hi ? 1.5 * pinv32 : (1.5 * pinv) * (double)(n + U - 1);
+      for (uint32_t k = 0; k < U; ++k) {
+        if ((v[k] != 1) & (v[k] != -1)) continue;
+        if (!(ymin <= thr)) return FS_E_VALIDATE;  // the filter would have missed it
+        rec(v[k] == 1 ? -(int32_t)(n + k) : (int32_t)(n + k));
       }
-      s = t;
+      s = v[U];
+    }
+    for (; n < n1; ++n) {
+      if ((s == 1) | (s == -1)) rec(s == 1 ? -(int32_t)n : (int32_t)n);
+      double y;
+      s = step(s, n + 1, &y);
     }
     *res = fs_signed_to_canon(s, p);
   } else if (path == FS_PATH_MONT) {

@@ -327,10 +327,17 @@ __device__ __forceinline__ uint64_t fs_cand_setup_mont(uint64_t p, uint64_t q, u
 // scalar load of U multipliers.
 #define FS_UNROLL_DEFAULT 8
 
+// The +-1 test of a block covers two residues per compare: if v[k] = +-1 then
+// the next residue v[k+1] = +-m[k] exactly (up to a multiple of p it is +-m[k];
+// the symmetric step leaves it at most m[k] in magnitude), so |v[k+1]| <= m[k]
+// holds whenever v[k] or v[k+1] is +-1.  It also holds for other residues of
+// magnitude <= m[k] (probability about 2 m / p per test); the taken branch then
+// tests every residue of the block exactly.
+
 // FP64 path.  nt[j] = (double)(n0 + 1 + j), read with scalar loads, with at
-// least U entries past n1 - n0; six FP64 operations and one compare per step
-// (four when p * n1 <= 2^52 and the product s*m is exact: EXACT), U steps
-// (2, 4 or 8) share one branch on the OR of their +-1 tests.
+// least U entries past n1 - n0; six FP64 operations per step (four when
+// p * n1 <= 2^52 and the product s*m is exact: EXACT) and one compare per two
+// steps; U steps (2, 4 or 8) share one branch.
 template <int U, bool EXACT, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                            const double* __restrict__ nt, Rec rec) {
@@ -356,10 +363,9 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t 
     s = v[U];
     bool hit = false;
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      m[k] = x[k];
-      hit |= (__builtin_fabs(v[k]) == 1.0);
-    }
+    for (int k = 0; k < U; k += 2) hit |= (__builtin_fabs(v[k + 1]) <= m[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k) m[k] = x[k];
     if (hit) {
 #pragma unroll
       for (int k = 0; k < U; ++k)
@@ -383,16 +389,18 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in,
 
 // Hybrid path (p <= FS_HYBRID_PMAX): signed residue |s| <= 0.51 p, exact
 // 64x32 products mod 2^64 with an FP64 quotient; nt as for the FP64 path.
-// n! = +-1 (mod p) iff s == +-1.  U steps share one test: the quotient
-// estimates y ~ s/p that the steps compute anyway pass |y| <= 1.5/p only for
-// s in {-1, 0, 1}, and the branch then tests s exactly.  HI: the quotient from
-// the high word of s (fs_sh_mulmod_hi, for p >= n1 * 2^39), whose estimates
-// pass |y| <= 1.5 * 2^32/p for every |s| < 2^33.
+// n! = +-1 (mod p) iff s == +-1.  U steps share one test on the quotient
+// estimates y[k] ~ v[k]/p that the steps compute anyway, taken at the odd
+// positions (two residues each, as above): |y| <= 1.5 * m[U-2] / p holds for
+// every |v| <= m[U-2] (the estimate's error is below 2^-50 |v|/p + 2^-21/p),
+// and the branch then tests every residue exactly.  HI: the quotient from the
+// high word of s (fs_sh_mulmod_hi, for p >= n1 * 2^39), whose estimates pass
+// |y| <= 1.5 * 2^32/p for every |v| < 2^33, so for every |v| <= m < 2^31.
 template <int U, bool HI, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                              const double* __restrict__ nt, Rec rec) {
   static_assert(U == 2 || U == 4 || U == 8, "unroll depth");
-  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0, thr = 1.5 * (HI ? pinv32 : pinv);
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0, thr_hi = 1.5 * pinv32, thr15 = 1.5 * pinv;
   const uint64_t pn = 0 - p, c31 = p << 31;
   int64_t s = fs_canon_to_signed(s_in, p);
   uint32_t n = n0;
@@ -420,12 +428,12 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_loop(uint64_t p, uint64_
       v[k + 1] = HI ? fs_sh_mulmod_hi(v[k], mm + k, m[k], c31, pn, pinv32, &y[k])
                     : fs_sh_mulmod(v[k], mm + k, m[k], c31, pn, pinv, pinv32, &y[k]);
     s = v[U];
-    double ymin = __builtin_fabs(y[0]);
+    double ymin = __builtin_fabs(y[1]);
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      m[k] = x[k];
-      if (k) ymin = __builtin_fmin(ymin, __builtin_fabs(y[k]));
-    }
+    for (int k = 3; k < U; k += 2) ymin = __builtin_fmin(ymin, __builtin_fabs(y[k]));
+    const double thr = HI ? thr_hi : thr15 * m[U - 2];  // m[U-2]: the block's largest even-position multiplier
+#pragma unroll
+    for (int k = 0; k < U; ++k) m[k] = x[k];
     if (ymin <= thr) {
 #pragma unroll
       for (int k = 0; k < U; ++k)

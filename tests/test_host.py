@@ -348,6 +348,40 @@ def test_hybrid_hi_filter_sound(fs):
             assert nf >= 1 and facs[0] == (p, (-1 if start == 1 else 1) * n0), p
 
 
+@pytest.mark.parametrize("path", [1, 2])
+def test_block_filter_catches_every_position(fs, path):
+    """The iterate loops test one residue of each pair (|v[k+1]| <= m[k], or
+    its quotient estimate) for +-1 at k and k+1.  Start residues are chosen so
+    that +-1 falls at each position 0..7 of an 8-step block, over moduli that
+    select each loop variant (FP64 with and without the exact product; hybrid
+    with and without the high-word quotient); the host execution returns an
+    error if a block's filter ever misses a +-1 residue."""
+    rng = random.Random(37 + path)
+    if path == 1:
+        ps = [1000003, 2 ** 26 + 15, 2 ** 40 + 15, 2 ** 50 + 55, 2 ** 53 - 111]
+        ps += [rng.randrange(10 ** 6, 2 ** 53) | 1 for _ in range(40)]
+    else:
+        ps = [2 ** 53 + 5, 2 ** 58 + 27, 2 ** 62 + 135, 2 ** 63 + 29, 2 ** 64 - 2 ** 58 - 1, 2 ** 64 - 2 ** 46 - 3]
+        ps += [rng.randrange(2 ** 53, 2 ** 64 - 2 ** 46) | 1 for _ in range(40)]
+    for p in ps:
+        for n0 in (1000, rng.randrange(2000, 400000)):
+            n1 = n0 + 8 * 3 + rng.randrange(0, 8)
+            if n1 >= p:
+                continue
+            for k in range(8):
+                for sign in (1, -1):
+                    blk = 8 * rng.randrange(0, 3)     # which block of the range
+                    prod = 1
+                    for j in range(1, blk + k + 1):
+                        prod = prod * (n0 + j) % p
+                    if math.gcd(prod, p) != 1:
+                        continue
+                    start = sign * pow(prod, -1, p) % p   # residue at n0 such that n0+blk+k gives +-1
+                    _, facs, nf = fs.hostemu_iterate(path, p, n0, n1, start)
+                    want = (p, -(n0 + blk + k) if sign == 1 else n0 + blk + k)
+                    assert want in facs[:nf], (p, n0, k, sign)
+
+
 def test_hybrid_quotient_filter_sound(fs):
     """The hybrid iterate tests s == +-1 only where the quotient estimate
     passes |y| <= 1.5/p; the host execution returns an error if a residue of
