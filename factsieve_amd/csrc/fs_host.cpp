@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
 #include <future>
 #include <map>
 #include <memory>
@@ -524,6 +526,73 @@ void fs_sort_factors(fs_factor* f, size_t n) { std::stable_sort(f, f + n, factor
 // into ranges of 2^15 multipliers, the ranges of all factors share the
 // threads, and each factor's partial products are then multiplied together:
 // a few large-n factors still use every thread.
+// Worker threads kept between calls (the role of the OpenMP thread team the
+// reference's verify loop runs on, cl_sieve.cpp:491-515): run(t, fn) executes
+// fn on the calling thread and t - 1 pool threads and returns when all are
+// done.  One parallel region at a time; the threads are joined at exit.
+namespace {
+class WorkerPool {
+ public:
+  void run(int t, const std::function<void()>& fn) {
+    std::lock_guard<std::mutex> region(region_mu_);
+    const int helpers = t - 1;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      while ((int)threads_.size() < helpers) {
+        const int idx = (int)threads_.size();
+        threads_.emplace_back([this, idx] { loop(idx); });
+      }
+      task_ = &fn;
+      want_ = helpers;
+      pending_ = helpers;
+      ++gen_;
+    }
+    cv_.notify_all();
+    fn();
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this] { return pending_ == 0; });
+    task_ = nullptr;
+  }
+  ~WorkerPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& th : threads_) th.join();
+  }
+
+ private:
+  void loop(int idx) {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      if (idx >= want_) continue;
+      const std::function<void()>* fn = task_;
+      lk.unlock();
+      (*fn)();
+      lk.lock();
+      if (--pending_ == 0) done_.notify_all();
+    }
+  }
+  std::mutex region_mu_, mu_;
+  std::condition_variable cv_, done_;
+  std::vector<std::thread> threads_;
+  const std::function<void()>* task_ = nullptr;
+  uint64_t gen_ = 0;
+  int want_ = 0, pending_ = 0;
+  bool stop_ = false;
+};
+
+WorkerPool& worker_pool() {
+  static WorkerPool pool;
+  return pool;
+}
+}  // namespace
+
 int64_t fs_verify_all(const fs_factor* f, size_t n, int threads) {
   if (n == 0) return -1;
   struct Range {
@@ -549,17 +618,15 @@ int64_t fs_verify_all(const fs_factor* f, size_t n, int threads) {
   if (threads < 1) threads = 1;
   if ((size_t)threads > rg.size()) threads = (int)std::max<size_t>(rg.size(), 1);
   std::atomic<size_t> next(0);
-  auto work = [&]() {
+  const std::function<void()> work = [&]() {
     for (;;) {
       const size_t t = next.fetch_add(1);
       if (t >= rg.size()) break;
       part[t] = range_product_mod(f[rg[t].i].p, rg[t].lo, rg[t].hi);
     }
   };
-  std::vector<std::thread> pool;
-  for (int t = 1; t < threads; ++t) pool.emplace_back(work);
-  work();
-  for (auto& th : pool) th.join();
+  if (threads > 1) worker_pool().run(std::min(threads, 256), work);
+  else work();
   for (size_t i = 0; i < n; ++i) {
     const uint64_t p = f[i].p;
     const int32_t fc = f[i].nc < 0 ? -1 : 1;
