@@ -308,11 +308,12 @@ typedef struct {
 void fs_default_batch_options(fs_batch_options* o);
 /* Many sieve jobs (no reference counterpart: the reference runs one job per
  * process, cl_sieve.cpp:918).  One candidate-generation launch for all jobs,
- * then one persistent launch whose workgroups pull (job, 256-candidate) tiles
- * from a device queue and run setup + iterate + checksum fold per candidate.
- * Results per job are bit-identical to fs_sieve on the job alone.  factors:
- * optional concatenated kept factors (job order, each sorted); per_job_nf:
- * kept count per job; seconds: wall time of the call. */
+ * then a persistent launch (one per round when n_chunk cuts the n-ranges) whose
+ * workgroups pull (job, tile of candidates, n-range) tiles from a device queue
+ * and run setup + iterate + checksum fold per candidate.  Results per job are
+ * bit-identical to fs_sieve on the job alone, for every option value.
+ * factors: optional concatenated kept factors (job order, each sorted);
+ * per_job_nf: kept count per job; seconds: wall time of the call. */
 int fs_batch(const fs_job* jobs, uint32_t njobs, const fs_batch_options* opt, fs_job_result* out,
              fs_factor* factors, uint64_t cap, uint64_t* per_job_nf, double* seconds);
 /* Time the launch shapes on a 1/8 p-range sample of the jobs and write the
