@@ -376,7 +376,7 @@ static_assert(fs_small_off(FS_NSMALL) == FS_NMASK, "mask table size");
 
 // Mask of residue class r of prime q: bit i set iff q divides B + 2i for odd
 // B = r (mod q), i.e. i = i0 (mod q) with i0 = (q - r) * 2^-1 mod q.
-FS_HD uint64_t fs_small_mask(uint32_t q, uint32_t r) {
+FS_HD constexpr uint64_t fs_small_mask(uint32_t q, uint32_t r) {
   const uint32_t i0 = ((q - r) % q) * ((q + 1) >> 1) % q;
   uint64_t m = 0;
   for (uint32_t i = i0; i < 64; i += q) m |= (uint64_t)1 << i;

@@ -37,13 +37,23 @@ __device__ __forceinline__ uint64_t fs_wave_sum(uint64_t v) {
 
 #define FS_BLOCK 256
 
-// LDS copy of the small-prime masks (1591 entries of 64 bits).
+// The small-prime masks (1591 entries of 64 bits: entry off(j) + r is the mask
+// of prime kSmallQ[j] for B = r mod q), built at compile time into the code
+// object's constant data.
+struct fs_sieve_tab_t {
+  uint64_t m[FS_NMASK];
+};
+constexpr fs_sieve_tab_t fs_make_sieve_tab() {
+  fs_sieve_tab_t t{};
+  for (int j = 0; j < FS_NSMALL; ++j)
+    for (uint32_t r = 0; r < kSmallQ[j]; ++r) t.m[fs_small_off(j) + r] = fs_small_mask(kSmallQ[j], r);
+  return t;
+}
+static __constant__ const fs_sieve_tab_t c_sieve_tab = fs_make_sieve_tab();
+
+// LDS copy of the small-prime masks (coalesced loads of the constant table).
 __device__ __forceinline__ void fs_sieve_table_init(uint64_t* s_tab) {
-  for (uint32_t k = threadIdx.x; k < FS_NMASK; k += FS_BLOCK) {
-    int j = 0;
-    while (j + 1 < FS_NSMALL && fs_small_off(j + 1) <= k) ++j;
-    s_tab[k] = fs_small_mask(kSmallQ[j], k - fs_small_off(j));
-  }
+  for (uint32_t k = threadIdx.x; k < FS_NMASK; k += FS_BLOCK) s_tab[k] = c_sieve_tab.m[k];
   __syncthreads();
 }
 
