@@ -152,6 +152,10 @@ def test_segmentation_and_chunking_invariance(gpu):
     (2 ** 63 - 100000, 2 ** 63 + 100000, 3000, 3500),
     (2 ** 64 - 2 ** 46 - 100000, 2 ** 64 - 2 ** 46 + 1, 3000, 3500),   # top of the hybrid range
     (2 ** 64 - 2 ** 46 - 50000, 2 ** 64 - 2 ** 46 + 50000, 3000, 3500),  # straddles it: Montgomery
+    (2 ** 40 - 100000, 2 ** 40 + 100000, 3700, 4097),   # FP64: p * n1 crosses 2^52 (exact-product waves and not)
+    ((16385 << 39) - 100000, (16385 << 39) + 100000, 16000, 16385),   # hybrid: p >> 39 crosses n1
+    (2 ** 64 - 2 ** 58 - 100000, 2 ** 64 - 2 ** 58 + 100000, 3000, 3500),  # top of the high-word quotient
+    ((2 ** 64 - 1) // 3 - 100000, (2 ** 64 - 1) // 3 + 100000, 3000, 3500),  # setup: two-stage REDC / REDC64
 ])
 def test_edge_ranges_match_oracle(gpu, args):
     print("edge", args, flush=True)
