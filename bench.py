@@ -18,7 +18,7 @@ the launch stream; 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
 2 v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, and 6 filter ops per 8 steps), 9.625
 when every modulus is at least nmax * 2^39 and the quotient comes from the
 high word of the residue alone (4 FP64 ops instead of 6), and 4.5 on the FP64
-path when p * nmax <= 2^52 and the product s*m is exact;
+path when p * nmax <= 2^53 and the product s*m is exact;
 peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
 vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
 int32_mul_equiv: the same steps/s expressed in the 11 32-bit multiplies a
@@ -51,7 +51,7 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine
 # per 8 steps, hybrid 94, Montgomery ~45 per step
 OPS_PER_STEP = {1: 6.5, 2: 11.75, 3: 45}
 OPS_PER_STEP_HYBRID_HI = 9.625                  # hybrid with the high-word quotient: 77 per 8 steps
-OPS_PER_STEP_FP64_EXACT = 4.5                   # FP64 with an exact product (p * nmax <= 2^52): 36 per 8
+OPS_PER_STEP_FP64_EXACT = 4.5                   # FP64 with an exact product (p * nmax <= 2^53): 36 per 8
 
 
 def hybrid_hi(pmin, pmax, nmax):
@@ -214,7 +214,7 @@ def main():
     ops = OPS_PER_STEP[ref.path]
     if ref.path == 2 and hybrid_hi(pmin, pmax, nmax):
         ops = OPS_PER_STEP_HYBRID_HI
-    if ref.path == 1 and pmax * nmax <= 2 ** 52:
+    if ref.path == 1 and pmax * nmax <= 2 ** 53:
         ops = OPS_PER_STEP_FP64_EXACT
     achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
     traffic = None

@@ -110,13 +110,15 @@ FS_HD double fs_fp_mulmod(double s, double m, double p, double pinv) {
   return r + l;
 }
 
-// The FP64 step when the product is exact: |s| < 0.8p, integer m, and
-// p * m <= 2^52 (fs_fp_exact_ok), so |s*m| < 2^53 and h = s*m needs no error
-// term.  k = rint(h * fl(1/p)) is within 1/2 + 2^-51 X of X = s*m/p, and
+// The FP64 step when the product is exact: |s| <= 0.51p (the iterate loops'
+// invariant: they start from the symmetric residue, |s| <= p/2, and every step
+// leaves |s'| <= (1/2 + 2^-21) p), integer m, and p * m <= 2^53
+// (fs_fp_exact_ok), so |s*m| <= 0.51 * 2^53 and h = s*m needs no error term.
+// k = rint(h * fl(1/p)) is within 1/2 + 2^-51 X of X = s*m/p, and
 // fma(-k, p, h) = s*m - k*p exactly (an integer of magnitude < 0.51 p).  Three
 // FP64 operations and the round instead of six.
 FS_HD bool fs_fp_exact_ok(uint64_t p, uint32_t mmax) {
-  return (unsigned __int128)p * mmax <= ((unsigned __int128)1 << 52);
+  return (unsigned __int128)p * mmax <= ((unsigned __int128)1 << 53);
 }
 FS_HD double fs_fp_mulmod_exact(double s, double m, double p, double pinv) {
 #pragma clang fp contract(off)

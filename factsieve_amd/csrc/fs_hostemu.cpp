@@ -41,7 +41,7 @@ extern "C" int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, in
 extern "C" int fs_hostemu_fp_exact(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res) {
   if ((p & 1) == 0 || p < 7 || m == 0 || !fs_fp_exact_ok(p, m) || !raw || !res) return FS_E_ARG;
   const uint64_t as = s < 0 ? (uint64_t)0 - (uint64_t)s : (uint64_t)s;
-  if ((double)as >= 0.8 * (double)p) return FS_E_ARG;
+  if ((double)as > 0.51 * (double)p) return FS_E_ARG;
   const double pd = (double)p;
   const double r = fs_fp_mulmod_exact((double)s, (double)m, pd, 1.0 / pd);
   *raw = (int64_t)r;

@@ -346,7 +346,7 @@ __device__ __forceinline__ uint64_t fs_cand_setup_mont(uint64_t p, uint64_t q, u
 
 // FP64 path.  nt[j] = (double)(n0 + 1 + j), read with scalar loads, with at
 // least U entries past n1 - n0; six FP64 operations per step (four when
-// p * n1 <= 2^52 and the product s*m is exact: EXACT) and one compare per two
+// p * n1 <= 2^53 and the product s*m is exact: EXACT) and one compare per two
 // steps; U steps (2, 4 or 8) share one branch.
 template <int U, bool EXACT, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,

@@ -203,8 +203,8 @@ int64_t fs_hostemu_candidates(uint64_t low, uint64_t high, uint64_t* out, uint64
  * p <= 2^64 - 2^46 (FS_E_ARG otherwise).  *raw = the step's own signed
  * representation of the result, *res = its canonical value in [0, p). */
 int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, int64_t* raw, uint64_t* res);
-/* The FP64 step without the product's error term, used when p * n1 <= 2^52
- * (s*m exact): |s| < 0.8p, 1 <= m, p * m <= 2^52, odd p >= 7 (FS_E_ARG
+/* The FP64 step without the product's error term, used when p * n1 <= 2^53
+ * (s*m exact): |s| <= 0.51p, 1 <= m, p * m <= 2^53, odd p >= 7 (FS_E_ARG
  * otherwise); |raw| <= 0.51p. */
 int fs_hostemu_fp_exact(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res);
 /* The HYBRID step with the quotient from the high word of s, used when

@@ -152,7 +152,7 @@ def test_segmentation_and_chunking_invariance(gpu):
     (2 ** 63 - 100000, 2 ** 63 + 100000, 3000, 3500),
     (2 ** 64 - 2 ** 46 - 100000, 2 ** 64 - 2 ** 46 + 1, 3000, 3500),   # top of the hybrid range
     (2 ** 64 - 2 ** 46 - 50000, 2 ** 64 - 2 ** 46 + 50000, 3000, 3500),  # straddles it: Montgomery
-    (2 ** 40 - 100000, 2 ** 40 + 100000, 3700, 4097),   # FP64: p * n1 crosses 2^52 (exact-product waves and not)
+    (2 ** 40 - 100000, 2 ** 40 + 100000, 8000, 8193),   # FP64: p * n1 crosses 2^53 (exact-product waves and not)
     ((16385 << 39) - 100000, (16385 << 39) + 100000, 16000, 16385),   # hybrid: p >> 39 crosses n1
     (2 ** 64 - 2 ** 58 - 100000, 2 ** 64 - 2 ** 58 + 100000, 3000, 3500),  # top of the high-word quotient
     ((2 ** 64 - 1) // 3 - 100000, (2 ** 64 - 1) // 3 + 100000, 3000, 3500),  # setup: two-stage REDC / REDC64
@@ -202,6 +202,17 @@ def test_cli_writes_reference_factors_file(gpu, tmp_path):
     r2 = subprocess.run([exe, "-n", "1000", "-N", "1e4", "-p", "1e8", "-P", "102e6"], cwd=tmp_path,
                         capture_output=True, text=True, timeout=120)
     assert r2.returncode == 0 and "Workunit complete." in r2.stdout
+
+
+def test_cli_self_test_passes(gpu, tmp_path):
+    """`factsieve -s` is the reference's run_test (cl_sieve.cpp:1273-1391): the five
+    self-test ranges against their expected factor counts, candidate counts and
+    checksums; exit status 0 only when all pass."""
+    print("cli -s", flush=True)
+    r = subprocess.run([gpu.EXE_PATH, "-s", "-v", "16"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "All test cases completed successfully!" in r.stdout
+    assert r.stderr.count("passed.") == 5 and "failed" not in r.stderr
 
 
 def test_checkpoint_resume_matches_uninterrupted(gpu, tmp_path):

@@ -285,13 +285,13 @@ def test_residue_step_at_error_bound_edges(fs):
 
 def test_fp64_exact_step_at_error_bound_edges(fs):
     """The FP64 step without the product's error term (used when p * n1 <=
-    2^52, so |s*m| < 0.8 * 2^52): exact, |s'| <= 0.51p, at p * m = 2^52 and
-    |s| just below 0.8p."""
+    2^53; the iterate loops keep |s| <= 0.51p, so |s*m| <= 0.51 * 2^53): exact,
+    |s'| <= 0.51p, at p * m = 2^53 and |s| = 0.51p."""
     rng = random.Random(31)
-    for p in [2 ** 26 + 15, 2 ** 32 - 5, 2 ** 40 + 15, 2 ** 45 + 59, 2 ** 51 - 129, 1000003, 127, 7]:
-        mtop = min(2 ** 52 // p, 2 ** 32 - 1)
-        smax = int(0.8 * p) - 1 if p > 100 else int(0.8 * p)
-        while smax >= 0.8 * p:
+    for p in [2 ** 26 + 15, 2 ** 32 - 5, 2 ** 40 + 15, 2 ** 45 + 59, 2 ** 51 - 129, 2 ** 52 + 1, 1000003, 127, 7]:
+        mtop = min(2 ** 53 // p, 2 ** 32 - 1)
+        smax = int(0.51 * p)
+        while smax > 0.51 * p:
             smax -= 1
         ms = [mtop, mtop - 1 if mtop > 1 else 1, 1] + [rng.randrange(1, mtop + 1) for _ in range(100)]
         ss = [smax, -smax, 1, -1, 0] + [rng.randrange(-smax, smax + 1) for _ in range(100)]
@@ -301,7 +301,7 @@ def test_fp64_exact_step_at_error_bound_edges(fs):
                 assert res == (s * m) % p, (p, s, m)
                 assert raw % p == res and abs(raw) <= 0.51 * p, (p, s, m, raw)
     with pytest.raises(fs.FactSieveError):
-        fs.hostemu_fp_exact(2 ** 40 + 15, 5, 2 ** 52 // (2 ** 40 + 15) + 1)   # p * m above 2^52
+        fs.hostemu_fp_exact(2 ** 40 + 15, 5, 2 ** 53 // (2 ** 40 + 15) + 1)   # p * m above 2^53
 
 
 def test_hybrid_hi_step_at_error_bound_edges(fs):
