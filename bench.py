@@ -11,14 +11,18 @@ p*n residue steps per job).  `value` is p*n residue steps per second of whole
 jobs.  One JSON line is printed.
 
 roofline: the dominant kernel is the residue iteration.  Its work is counted
-in VALU lane-operations per p*n step of the compiled 8-step loop: 6.5 on the
-FP64 path (2 multiplies, 2 FMAs, 1 add, 1 round per step, and one compare per
-two steps: |v[k+1]| <= m[k] covers +-1 at k and k+1), timed with HIP events on
-the launch stream; 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
+in VALU lane-operations per p*n step of the compiled 8-step loop, timed with
+HIP events on the launch stream.  On the FP64 path (p < 2^53): 4.0 with paired
+steps, used when 4 nmax^2 <= p (one 6-operation product by (n+1)(n+2) per two
+steps -- 2 multiplies, 2 FMAs, 1 add, 1 round -- and two exact compares,
+|v| == (n+1)(n+2) and |v| == n+2, which find +-1 at n and n+1), 3.0 when also
+p nmax^2 <= 2^53 (exact product: 2 multiplies, 1 FMA, 1 round); otherwise 6.5
+with single steps (one compare per two steps: |v[k+1]| <= m[k] covers +-1 at k
+and k+1); 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
 2 v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, and 6 filter ops per 8 steps), 9.625
 when every modulus is at least nmax * 2^39 and the quotient comes from the
 high word of the residue alone (4 FP64 ops instead of 6), and 4.5 on the FP64
-path when p * nmax <= 2^53 and the product s*m is exact;
+path with single steps when p * nmax <= 2^53 and the product s*m is exact;
 peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
 vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
 int32_mul_equiv: the same steps/s expressed in the 11 32-bit multiplies a
@@ -52,6 +56,16 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine
 OPS_PER_STEP = {1: 6.5, 2: 11.75, 3: 45}
 OPS_PER_STEP_HYBRID_HI = 9.625                  # hybrid with the high-word quotient: 77 per 8 steps
 OPS_PER_STEP_FP64_EXACT = 4.5                   # FP64 with an exact product (p * nmax <= 2^53): 36 per 8
+OPS_PER_STEP_FP64_PAIR = 4.0                    # FP64 paired steps (4 nmax^2 <= p): 4 x 6 + 8 compares per 8
+OPS_PER_STEP_FP64_PAIR_EXACT = 3.0              # paired steps with an exact product (p nmax^2 <= 2^53): 4 x 4 + 8
+
+
+def fp64_ops(pmin, pmax, nmax):
+    """VALU operations per step of the FP64 iterate loop every wave of a job
+    takes (the per-wave choice of fs_cand_iter_fp64, with n1 = nmax)."""
+    if nmax <= 2 ** 22 and 4 * nmax * nmax <= pmin:
+        return OPS_PER_STEP_FP64_PAIR_EXACT if pmax * nmax * nmax <= 2 ** 53 else OPS_PER_STEP_FP64_PAIR
+    return OPS_PER_STEP_FP64_EXACT if pmax * nmax <= 2 ** 53 else OPS_PER_STEP[1]
 
 
 def hybrid_hi(pmin, pmax, nmax):
@@ -214,8 +228,8 @@ def main():
     ops = OPS_PER_STEP[ref.path]
     if ref.path == 2 and hybrid_hi(pmin, pmax, nmax):
         ops = OPS_PER_STEP_HYBRID_HI
-    if ref.path == 1 and pmax * nmax <= 2 ** 53:
-        ops = OPS_PER_STEP_FP64_EXACT
+    if ref.path == 1:
+        ops = fp64_ops(pmin, pmax, nmax)
     achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
     traffic = None
     if ref.path in TRAFFIC_BYTES_PER_CAND:

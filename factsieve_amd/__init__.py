@@ -143,6 +143,8 @@ _SIGS = [
     ("fs_hostemu_mulmod", ctypes.c_int, [ctypes.c_int, u64, i64, u64, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
     ("fs_hostemu_fp_exact", ctypes.c_int, [u64, i64, u32, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
     ("fs_hostemu_mulmod_hi", ctypes.c_int, [u64, i64, u32, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
+    ("fs_hostemu_fp_pair", ctypes.c_int, [u64, i64, u32, ctypes.POINTER(i64), ctypes.POINTER(u64),
+                                          ctypes.POINTER(ctypes.c_int)]),
     ("fs_hostemu_setup_mul", ctypes.c_int, [u64, u64, u64, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(u64),
                                             ctypes.POINTER(u32)]),
     ("fs_default_options", None, [ctypes.POINTER(Options)]),
@@ -299,6 +301,14 @@ def hostemu_mulmod_hi(p, s, m):
     raw, res = i64(), u64()
     check(lib().fs_hostemu_mulmod_hi(p, s, m, ctypes.byref(raw), ctypes.byref(res)))
     return raw.value, res.value
+
+
+def hostemu_fp_pair(p, s, a):
+    """One FP64 paired step s*a*(a+1) mod p (4 (a+1)^2 <= p) on the host:
+    (raw signed value, canonical, whether the exact-product form ran)."""
+    raw, res, ex = i64(), u64(), ctypes.c_int()
+    check(lib().fs_hostemu_fp_pair(p, s, a, ctypes.byref(raw), ctypes.byref(res), ctypes.byref(ex)))
+    return raw.value, res.value, bool(ex.value)
 
 
 def hostemu_setup_mul(p, s, m, square=False):

@@ -127,6 +127,21 @@ FS_HD double fs_fp_mulmod_exact(double s, double m, double p, double pinv) {
   return __builtin_fma(-k, p, h);
 }
 
+// Paired FP64 step: one product by M = a*(a+1) advances the residue over two
+// consecutive multipliers a, a+1 <= n1.  fs_fp_pair_ok(p, n1): n1 <= 2^22 and
+// 4 n1^2 <= p.  Then M < n1^2 <= 2^44, so k = rint(fl(h * fl(1/p))) is within
+// 1/2 + 3 * 2^-53 * 0.51 * 2^44 < 0.503 of X = s*M/p and every step keeps
+// |s'| <= 0.503 p (fs_fp_mulmod, whose bounds need only M < 2^50 and |s| < 0.8p);
+// and M < p/4, so a residue of magnitude <= 0.51 p that is congruent to +-M
+// (or to +-(a+1)) is exactly +-M (or +-(a+1)).  fs_fp_pair_exact_ok adds
+// p * n1^2 <= 2^53: |s*M| < 0.51 * 2^53 and fs_fp_mulmod_exact applies.
+FS_HD bool fs_fp_pair_ok(uint64_t p, uint32_t n1) {
+  return n1 <= (1u << 22) && 4 * (uint64_t)n1 * n1 <= p;
+}
+FS_HD bool fs_fp_pair_exact_ok(uint64_t p, uint32_t n1) {
+  return fs_fp_pair_ok(p, n1) && (unsigned __int128)p * ((uint64_t)n1 * n1) <= ((unsigned __int128)1 << 53);
+}
+
 // Canonical [0, p) integer from a symmetric FP64 residue.
 FS_HD uint64_t fs_fp_to_canon(double s, uint64_t p) {
   const int64_t v = (int64_t)s;

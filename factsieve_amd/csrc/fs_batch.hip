@@ -133,7 +133,8 @@ template <int U>
 __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__ jobs,
                                                      const uint4* __restrict__ tiles, fs_cands c,
                                                      const uint64_t* __restrict__ prog,
-                                                     const double* __restrict__ ntab, fs_factor* fac, uint32_t fcap,
+                                                     const double* __restrict__ ntab, uint32_t pair_len,
+                                                     const double* __restrict__ ptab, fs_factor* fac, uint32_t fcap,
                                                      const uint32_t* jcnt, uint32_t* jflag, uint64_t* jsum,
                                                      uint32_t* glob, uint32_t tile) {
   __shared__ uint32_t s_t;
@@ -152,6 +153,10 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__
     const bool first = (n0 == J.nmin), last = (n1 == J.nmax), fold = last && !(J.flags & FS_BJ_NOFOLD);
     const uint64_t* jp = prog + J.prog_off;
     const double* nt = ntab + n0 + 1;
+    // pairs (n0+1, n0+2), (n0+3, n0+4), ... while the table covers n1 + 16
+    const uint32_t a0 = n0 + 1;
+    const bool pairs = (n1 + 16 <= pair_len);
+    const double* pt = ptab + (pairs ? (size_t)(a0 & 1) * 2 * (pair_len / 2 + 1) + 2 * (size_t)(a0 >> 1) : 0);
     for (uint32_t k = threadIdx.x; k < tile; k += FS_BLOCK) {
       const uint32_t i = begin + k;
       uint64_t v = 0;
@@ -165,7 +170,7 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__
         if (!first && c.nnext[g] != n0) atomicOr(&jflag[jo], FS_BJF_VALIDATE);
         if (J.path == FS_PATH_FP64) {
           s = first ? fs_cand_setup_fp64(p, q, r2, 0, (const double*)jp, 0, J.prog_terms) : c.res[g];
-          s = fs_cand_iter_fp64<U>(p, s, n0, n1, nt, rec);
+          s = fs_cand_iter_fp64<U>(p, s, n0, n1, nt, pt, pairs, rec);
         } else if (J.path == FS_PATH_HYBRID) {
           s = first ? fs_cand_setup_hybrid(p, q, r2, 0, jp, J.prog_terms, 0, J.prog_terms) : c.res[g];
           s = fs_cand_iter_hybrid<U>(p, s, n0, n1, nt, rec);
@@ -192,9 +197,15 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__
   }
 }
 
-__global__ void k_bfill_ntab(double* t, uint32_t count) {
+__global__ void k_bfill_ntab(double* t, uint32_t count, double* pt, uint32_t pair_len) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < count) t[j] = (double)j;
+  if (j < pair_len) {
+    const uint64_t a = j;
+    double* e = pt + (size_t)(j & 1) * 2 * (pair_len / 2 + 1) + 2 * (size_t)(j >> 1);
+    e[0] = (double)(a * (a + 1));
+    e[1] = (double)(a + 1);
+  }
 }
 
 // ============================================================ launch helpers
@@ -213,21 +224,21 @@ int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, const std::vector<uint32
                                           d.tiles, d.tiles_cap, d.pref, d.glob);
     BCHK(hipGetLastError(), "batch tiles");
     if (unroll == 8)
-      k_bsieve<8><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.fac, d.fcap, d.jcnt, d.jflag,
-                                               d.jsum, d.glob, tile);
+      k_bsieve<8><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.pair_len, d.ntab + d.ntab_len,
+                                               d.fac, d.fcap, d.jcnt, d.jflag, d.jsum, d.glob, tile);
     else if (unroll == 2)
-      k_bsieve<2><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.fac, d.fcap, d.jcnt, d.jflag,
-                                               d.jsum, d.glob, tile);
+      k_bsieve<2><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.pair_len, d.ntab + d.ntab_len,
+                                               d.fac, d.fcap, d.jcnt, d.jflag, d.jsum, d.glob, tile);
     else
-      k_bsieve<4><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.fac, d.fcap, d.jcnt, d.jflag,
-                                               d.jsum, d.glob, tile);
+      k_bsieve<4><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.pair_len, d.ntab + d.ntab_len,
+                                               d.fac, d.fcap, d.jcnt, d.jflag, d.jsum, d.glob, tile);
     BCHK(hipGetLastError(), "batch sieve");
   }
   return FS_OK;
 }
 
 int fs_batch_fill_ntab(double* t, uint32_t count, hipStream_t st) {
-  k_bfill_ntab<<<(count + 255) / 256, 256, 0, st>>>(t, count);
+  k_bfill_ntab<<<(count + 255) / 256, 256, 0, st>>>(t, count, t + count, fs_bpair_len(count));
   BCHK(hipGetLastError(), "batch ntab");
   return FS_OK;
 }

@@ -85,8 +85,10 @@ int fs_batch_bufs::ensure(uint32_t nj, uint32_t ncands, uint32_t nprog, uint32_t
   const uint32_t nt_need = nmax_max + 64;
   if (nt_need > d.ntab_len || !d.ntab) {
     const uint32_t n = std::max<uint32_t>(nt_need, 1u << 16);
-    if ((rc = grow(&d.ntab, n, "batch ntab")) || (rc = fs_batch_fill_ntab(d.ntab, n, st))) return rc;
+    if ((rc = grow(&d.ntab, fs_bntab_doubles(n), "batch ntab")) || (rc = fs_batch_fill_ntab(d.ntab, n, st)))
+      return rc;
     d.ntab_len = n;
+    d.pair_len = fs_bpair_len(n);
   }
   if (fcap > d.fcap || !d.fac) {
     if ((rc = grow(&d.fac, fcap, "batch factors"))) return rc;

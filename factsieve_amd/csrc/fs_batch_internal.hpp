@@ -52,8 +52,9 @@ struct fs_batch_dev {
   uint32_t tiles_cap = 0;
   fs_cands c{};
   uint64_t* prog = nullptr;
-  double* ntab = nullptr;      // ntab[k] = k for k < ntab_len
+  double* ntab = nullptr;      // ntab[k] = k for k < ntab_len, then the pair table
   uint32_t ntab_len = 0;
+  uint32_t pair_len = 0;       // the pair table covers multipliers a < pair_len (fs_bpair_len)
   fs_factor* fac = nullptr;
   uint32_t fcap = 0;
 };
@@ -65,6 +66,14 @@ struct fs_batch_dev {
 int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, const std::vector<uint32_t>& norder,
                     const std::vector<uint32_t>& off, uint32_t pg_blocks, uint32_t tile, uint32_t blocks,
                     uint32_t unroll, hipStream_t st);
+// Pair table of the paired FP64 step, after the ntab_len multipliers: for
+// a < pair_len, the entry (a*(a+1), a+1) at index a >> 1 of half a & 1, each
+// half holding pair_len / 2 + 1 entries of two doubles.  Only moduli with
+// n1 <= 2^22 take paired steps, so the table stops there.
+inline uint32_t fs_bpair_len(uint32_t ntab_len) { return ntab_len < (1u << 22) + 64 ? ntab_len : (1u << 22) + 64; }
+inline size_t fs_bntab_doubles(uint32_t ntab_len) {
+  return (size_t)ntab_len + 4 * ((size_t)fs_bpair_len(ntab_len) / 2 + 1);
+}
 int fs_batch_fill_ntab(double* t, uint32_t count, hipStream_t st);
 extern "C" void fs_set_hip_error(const char* what, int err);
 

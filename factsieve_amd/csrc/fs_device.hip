@@ -134,12 +134,13 @@ __global__ __launch_bounds__(FS_BLOCK) void k_setup_mont(fs_cands c, const uint3
 
 __global__ __launch_bounds__(FS_BLOCK) void k_iterate_fp64(fs_cands c, uint32_t* ctr, fs_factor* fac,
                                                            uint32_t fcap, uint32_t n0, uint32_t n1,
-                                                           const double* __restrict__ nt) {
+                                                           const double* __restrict__ nt,
+                                                           const double* __restrict__ pt) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
   const uint64_t p = c.p[i];
   if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
-  c.res[i] = fs_cand_iter_fp64(p, c.res[i], n0, n1, nt, FS_REC(p));
+  c.res[i] = fs_cand_iter_fp64(p, c.res[i], n0, n1, nt, pt, true, FS_REC(p));
   c.nnext[i] = n1;
 }
 
@@ -164,9 +165,17 @@ __global__ __launch_bounds__(FS_BLOCK) void k_iterate_mont(fs_cands c, uint32_t*
   c.nnext[i] = n1;
 }
 
-__global__ void k_fill_ntab(double* t, uint32_t first, uint32_t count) {
+// t[j] = first + j; the pair table holds, for a = first + j, the entry
+// (a*(a+1), a+1) at index j >> 1 of half (j & 1): pt[(j & 1) * 2 * half + 2 * (j >> 1)].
+__global__ void k_fill_ntab(double* t, double* pt, uint32_t half, uint32_t first, uint32_t count) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < count) t[j] = (double)(first + j);
+  if (j < count) {
+    const uint64_t a = (uint64_t)first + j;
+    t[j] = (double)a;
+    double* e = pt + (size_t)(j & 1) * 2 * half + 2 * (size_t)(j >> 1);
+    e[0] = (double)(a * (a + 1));  // exact while a < 2^26; only used for a < 2^22
+    e[1] = (double)(a + 1);
+  }
 }
 
 // ---- check: checksum fold + count + validation (check.cl:50-109)
@@ -587,19 +596,29 @@ extern "C" int fs_setup(const fs_cands* c, const uint32_t* d_ctr, const void* d_
 }
 
 // Multiplier tables of the FP64 iterate kernel, one per stream: nt[j] = base + j
-// as doubles, refilled on the stream (stream order protects earlier launches).
+// as doubles, and the pair table of the paired step (k_fill_ntab), refilled on
+// the stream (stream order protects earlier launches).
 struct NTab {
-  double* d = nullptr;
+  double* d = nullptr;   // cap multipliers, then the pair table (2 halves of cap/2 + 1 pairs)
   uint32_t base = 0, len = 0, cap = 0;
+  double* pairs() const { return d + cap; }
+  uint32_t half() const { return cap / 2 + 1; }
+  // pair table position of the pairs (a, a+1), (a+2, a+3), ... for a >= base
+  const double* pairs_at(uint32_t a) const {
+    const uint32_t j = a - base;
+    return pairs() + (size_t)(j & 1) * 2 * half() + 2 * (size_t)(j >> 1);
+  }
 };
 static std::mutex g_ntab_mu;
 static std::map<hipStream_t, NTab> g_ntab;
 
-static int ntab_for(hipStream_t st, uint32_t first, uint32_t count, const double** out) {
+static int ntab_for(hipStream_t st, uint32_t first, uint32_t count, const double** out,
+                    const double** pout = nullptr) {
   std::lock_guard<std::mutex> lk(g_ntab_mu);
   NTab& t = g_ntab[st];
   if (t.d && first >= t.base && (uint64_t)first + count + 8 <= (uint64_t)t.base + t.len) {
     *out = t.d + (first - t.base);
+    if (pout) *pout = t.pairs_at(first);
     return FS_OK;
   }
   // 8 entries of padding: the kernel reads one unrolled block ahead
@@ -607,7 +626,7 @@ static int ntab_for(hipStream_t st, uint32_t first, uint32_t count, const double
   if (want > t.cap) {
     if (t.d) HIPCHK(hipFree(t.d), "hipFree(ntab)");
     t.d = nullptr;
-    hipError_t e = hipMalloc(&t.d, (size_t)want * sizeof(double));
+    hipError_t e = hipMalloc(&t.d, ((size_t)want + 4 * ((size_t)want / 2 + 1)) * sizeof(double));
     if (e != hipSuccess) {
       set_error("hipMalloc(ntab)", e);
       t.cap = t.len = 0;
@@ -616,11 +635,12 @@ static int ntab_for(hipStream_t st, uint32_t first, uint32_t count, const double
     t.cap = want;
   }
   const uint32_t len = want;  // values past 2^31 are never used as multipliers
-  k_fill_ntab<<<(len + 255) / 256, 256, 0, st>>>(t.d, first, len);
+  k_fill_ntab<<<(len + 255) / 256, 256, 0, st>>>(t.d, t.pairs(), t.half(), first, len);
   HIPCHK(hipGetLastError(), "fill ntab");
   t.base = first;
   t.len = len;
   *out = t.d;
+  if (pout) *pout = t.pairs_at(first);
   return FS_OK;
 }
 
@@ -635,9 +655,9 @@ extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_facto
   const uint32_t g = cand_grid(c);
   switch (path) {
     case FS_PATH_FP64: {
-      const double* nt = nullptr;
-      if ((rc = ntab_for(S(s), n0 + 1, n1 - n0, &nt))) return rc;
-      k_iterate_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, nt);
+      const double *nt = nullptr, *pt = nullptr;
+      if ((rc = ntab_for(S(s), n0 + 1, n1 - n0, &nt, &pt))) return rc;
+      k_iterate_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, nt, pt);
       break;
     }
     case FS_PATH_HYBRID: {

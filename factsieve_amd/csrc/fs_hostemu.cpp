@@ -7,6 +7,7 @@
 // calls them: fs_sieve, fs_batch and the server run only the gfx950 kernels.
 #include <stdint.h>
 #include <string.h>
+#include <numeric>
 #include <vector>
 #include "fs_arith.hpp"
 #include "fs_internal.hpp"
@@ -58,6 +59,20 @@ extern "C" int fs_hostemu_mulmod_hi(uint64_t p, int64_t s, uint32_t m, int64_t* 
   const int64_t r = fs_sh_mulmod_hi(s, m, (double)m, p << 31, 0 - p, pinv32, &y);
   *raw = r;
   *res = fs_signed_to_canon(r, p);
+  return FS_OK;
+}
+
+extern "C" int fs_hostemu_fp_pair(uint64_t p, int64_t s, uint32_t a, int64_t* raw, uint64_t* res, int* exact) {
+  if ((p & 1) == 0 || p < 7 || p >= ((uint64_t)1 << 53) || a == 0 || a >= (1u << 22) || !fs_fp_pair_ok(p, a + 1) ||
+      !raw || !res || !exact)
+    return FS_E_ARG;
+  const uint64_t as = s < 0 ? (uint64_t)0 - (uint64_t)s : (uint64_t)s;
+  if ((double)as > 0.51 * (double)p) return FS_E_ARG;
+  const double pd = (double)p, M = (double)((uint64_t)a * (a + 1));
+  *exact = fs_fp_pair_exact_ok(p, a + 1);
+  const double r = *exact ? fs_fp_mulmod_exact((double)s, M, pd, 1.0 / pd) : fs_fp_mulmod((double)s, M, pd, 1.0 / pd);
+  *raw = (int64_t)r;
+  *res = fs_fp_to_canon(r, p);
   return FS_OK;
 }
 
@@ -192,6 +207,50 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     };
     double s = fs_canon_to_fp(*res, p);
     uint32_t n = n0;
+    if (fs_fp_pair_ok(p, n1)) {
+      // The paired loop: one product by M = a*(a+1) per two positions, the
+      // compares |v| == M and |v| == a+1 on its result.  The odd positions'
+      // residues are formed here by single steps only to check that the
+      // compares fire exactly for the blocks that hold a +-1 residue.
+      const bool pexact = fs_fp_pair_exact_ok(p, n1);
+      auto pstep = [&](double x, uint64_t M) {
+        return pexact ? fs_fp_mulmod_exact(x, (double)M, pd, pinv) : fs_fp_mulmod(x, (double)M, pd, pinv);
+      };
+      for (; n < nu; n += U) {
+        double v[U / 2 + 1];
+        v[0] = s;
+        for (uint32_t i = 0; i < U / 2; ++i) {
+          const uint64_t a = n + 1 + 2 * i;
+          v[i + 1] = pstep(v[i], a * (a + 1));
+          if (__builtin_fabs(v[i + 1]) > 0.51 * pd) return FS_E_VALIDATE;  // the loop's invariant
+          const bool hit =
+              (__builtin_fabs(v[i + 1]) == (double)(a * (a + 1))) | (__builtin_fabs(v[i + 1]) == (double)(a + 1));
+          const double u = fs_fp_mulmod(v[i], (double)a, pd, pinv);
+          const double odd = pstep(u, a + 1);
+          if (fs_fp_to_canon(odd, p) != fs_fp_to_canon(v[i + 1], p)) return FS_E_VALIDATE;
+          const double r[2] = {v[i], u};  // residues of n + 2i and n + 2i + 1
+          bool any = false;
+          for (uint32_t k = 0; k < 2; ++k) {
+            if (__builtin_fabs(r[k]) != 1.0) continue;
+            any = true;
+            const uint32_t nn = n + 2 * i + k;
+            rec(r[k] > 0.0 ? -(int32_t)nn : (int32_t)nn);
+          }
+          // A +-1 residue always makes a compare fire.  A compare without one
+          // needs a(a+1) not invertible mod p (a composite probable prime).
+          if (any && !hit) return FS_E_VALIDATE;
+          if (hit && !any && std::gcd(a * (a + 1), p) == 1) return FS_E_VALIDATE;
+        }
+        s = v[U / 2];
+      }
+      for (; n < n1; ++n) {
+        if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
+        s = pstep(s, n + 1);
+      }
+      *res = fs_fp_to_canon(s, p);
+      *nf = cnt;
+      return FS_OK;
+    }
     for (; n < nu; n += U) {
       double v[U + 1];
       v[0] = s;

@@ -389,10 +389,75 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t 
   return fs_fp_to_canon(s, p);
 }
 
-// Per wave: the exact-product step when every active lane's modulus allows it.
+// FP64 path, paired steps (fs_fp_pair_ok(p, n1)): one product by
+// M = a*(a+1) per two multipliers.  pt[2j] = (n0+1+2j)*(n0+2+2j) and
+// pt[2j+1] = n0+2+2j, read with scalar loads, with at least U entries past
+// n1 - n0.  With v[k] the residue of n0 + k and M, a+1 the pair's values,
+// v[k+2] = v[k] * M and v[k+2] = v[k+1] * (a+1), so, both values being below
+// p/4 and every residue of magnitude <= 0.51 p, |v[k+2]| == M exactly when
+// v[k] = +-1 and |v[k+2]| == a+1 exactly when v[k+1] = +-1.  Per two steps:
+// one six-operation product (four when EXACT) and two compares.  The taken
+// branch (a +-1 residue in the block) recomputes the odd positions with single
+// steps and records every +-1 in order.
+template <int U, bool EXACT, class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
+                                                                const double* __restrict__ nt,
+                                                                const double* __restrict__ pt, Rec rec) {
+  static_assert(U == 2 || U == 4 || U == 8, "unroll depth");
+  constexpr int H = U / 2;
+  const double pd = (double)p, pinv = 1.0 / pd;
+  double s = fs_canon_to_fp(s_in, p);
+  uint32_t n = n0;
+  const uint32_t nu = n0 + ((n1 - n0) & ~(uint32_t)(U - 1));
+  double w[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) w[k] = pt[k];
+  for (; n < nu; n += U) {
+    const double* wn = pt + (n - n0) + U;  // the next block's H pairs
+    double x[U], v[H + 1];
+#pragma unroll
+    for (int k = 0; k < U; ++k) x[k] = wn[k];
+    v[0] = s;
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+      v[i + 1] = EXACT ? fs_fp_mulmod_exact(v[i], w[2 * i], pd, pinv) : fs_fp_mulmod(v[i], w[2 * i], pd, pinv);
+    s = v[H];
+    bool hit = false;
+#pragma unroll
+    for (int i = 0; i < H; ++i)
+      hit |= (__builtin_fabs(v[i + 1]) == w[2 * i]) | (__builtin_fabs(v[i + 1]) == w[2 * i + 1]);
+#pragma unroll
+    for (int k = 0; k < U; ++k) w[k] = x[k];
+    if (hit) {
+#pragma unroll
+      for (int i = 0; i < H; ++i) {
+        const uint32_t nn = n + 2 * i;
+        if (__builtin_fabs(v[i]) == 1.0) rec(v[i] > 0.0 ? -(int32_t)nn : (int32_t)nn);
+        const double u = fs_fp_mulmod(v[i], nt[nn - n0], pd, pinv);  // residue of nn + 1
+        if (__builtin_fabs(u) == 1.0) rec(u > 0.0 ? -(int32_t)(nn + 1) : (int32_t)(nn + 1));
+      }
+    }
+  }
+  for (; n < n1; ++n) {
+    if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
+    s = EXACT ? fs_fp_mulmod_exact(s, nt[n - n0], pd, pinv) : fs_fp_mulmod(s, nt[n - n0], pd, pinv);
+  }
+  return fs_fp_to_canon(s, p);
+}
+
+// Per wave: paired steps when the pair table covers the launch (pairs) and
+// every active lane's modulus allows them, else single steps; within each, the
+// exact-product step when every active lane's modulus allows it.  pt is always
+// a valid table pointer (never selected against a null pointer), which keeps
+// its loads scalar.
 template <int U = FS_UNROLL_DEFAULT, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
-                                                      const double* __restrict__ nt, Rec rec) {
+                                                      const double* __restrict__ nt, const double* __restrict__ pt,
+                                                      bool pairs, Rec rec) {
+  if (pairs && __all(fs_fp_pair_ok(p, n1))) {
+    if (__all(fs_fp_pair_exact_ok(p, n1))) return fs_cand_iter_fp64_pair_loop<U, true>(p, s_in, n0, n1, nt, pt, rec);
+    return fs_cand_iter_fp64_pair_loop<U, false>(p, s_in, n0, n1, nt, pt, rec);
+  }
   if (__all(fs_fp_exact_ok(p, n1))) return fs_cand_iter_fp64_loop<U, true>(p, s_in, n0, n1, nt, rec);
   return fs_cand_iter_fp64_loop<U, false>(p, s_in, n0, n1, nt, rec);
 }

@@ -211,6 +211,12 @@ int fs_hostemu_fp_exact(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_
  * p >= n1 * 2^39 (for every multiplier m <= n1 of a launch): |s| <= 0.51p,
  * 1 <= m, p >> 39 >= m, odd p <= 2^64 - 2^46 (FS_E_ARG otherwise); |raw| <= 0.51p. */
 int fs_hostemu_mulmod_hi(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res);
+/* The FP64 paired step, one product by a*(a+1) for the multipliers a and a+1,
+ * used when 4 * n1^2 <= p and n1 <= 2^22 (for every pair a+1 <= n1 of a launch;
+ * without the error term when also p * n1^2 <= 2^53): |s| <= 0.51p, 1 <= a,
+ * 4 (a+1)^2 <= p, a + 1 <= 2^22, odd p < 2^53 (FS_E_ARG otherwise); *exact
+ * tells which form ran; |raw| <= 0.503p. */
+int fs_hostemu_fp_pair(uint64_t p, int64_t s, uint32_t a, int64_t* raw, uint64_t* res, int* exact);
 /* One multiply (square = 0) or square (square = 1, m ignored) of the HYBRID
  * setup program, s*m*2^-k mod p: below 2^64/3 a multiply reduces by k = 62 and
  * a square by k = 64, both lazily (s < 2p, result < 2p); below 2^63 both by

@@ -152,7 +152,11 @@ def test_segmentation_and_chunking_invariance(gpu):
     (2 ** 63 - 100000, 2 ** 63 + 100000, 3000, 3500),
     (2 ** 64 - 2 ** 46 - 100000, 2 ** 64 - 2 ** 46 + 1, 3000, 3500),   # top of the hybrid range
     (2 ** 64 - 2 ** 46 - 50000, 2 ** 64 - 2 ** 46 + 50000, 3000, 3500),  # straddles it: Montgomery
-    (2 ** 40 - 100000, 2 ** 40 + 100000, 8000, 8193),   # FP64: p * n1 crosses 2^53 (exact-product waves and not)
+    (2 ** 40 - 100000, 2 ** 40 + 100000, 8000, 8193),   # FP64 paired steps (4 n1^2 <= p)
+    (51539509248 - 100000, 51539509248 + 100000, 174270, 174763),  # single steps: p * n1 crosses 2^53
+    (4 * 10 ** 6, 6 * 10 ** 6, 10, 1000),                # paired steps with the exact product, ~30 factors
+    (1600000000 - 100000, 1600000000 + 100000, 19500, 20000),  # 4 n1^2 = 1.6e9 inside: paired and single waves
+    (360287970 - 100000, 360287970 + 100000, 4000, 5000),  # p * n1^2 crosses 2^53: exact and inexact pairs
     ((16385 << 39) - 100000, (16385 << 39) + 100000, 16000, 16385),   # hybrid: p >> 39 crosses n1
     (2 ** 64 - 2 ** 58 - 100000, 2 ** 64 - 2 ** 58 + 100000, 3000, 3500),  # top of the high-word quotient
     ((2 ** 64 - 1) // 3 - 100000, (2 ** 64 - 1) // 3 + 100000, 3000, 3500),  # setup: two-stage REDC / REDC64

@@ -304,6 +304,78 @@ def test_fp64_exact_step_at_error_bound_edges(fs):
         fs.hostemu_fp_exact(2 ** 40 + 15, 5, 2 ** 53 // (2 ** 40 + 15) + 1)   # p * m above 2^53
 
 
+def test_fp64_pair_step_at_error_bound_edges(fs):
+    """The FP64 paired step (one product by a*(a+1), used when 4 n1^2 <= p and
+    n1 <= 2^22): exact, and |s'| <= 0.503p, at the largest pair the bounds
+    allow (a + 1 = min(2^22, sqrt(p/4))), |s| up to 0.51p, moduli up to
+    2^53; the exact-product form exactly when p (a+1)^2 <= 2^53."""
+    rng = random.Random(41)
+    ps = [2 ** 53 - 111, 2 ** 53 - 1, 2 ** 46 + 15, 2 ** 46 - 1, 2 ** 40 + 15, 2 ** 33 + 17, 2 ** 26 + 15, 4 * 9 + 1,
+          4 * 2 ** 44 + 1, 1000003, 10 ** 12 + 39]
+    ps += [rng.randrange(2 ** 20, 2 ** 53) | 1 for _ in range(30)]
+    for p in ps:
+        atop = min(2 ** 22, math.isqrt(p // 4)) - 1          # largest a with 4 (a+1)^2 <= p, a+1 <= 2^22
+        assert 4 * (atop + 1) ** 2 <= p
+        smax = int(0.51 * p)
+        while smax > 0.51 * p:
+            smax -= 1
+        As = [atop, max(1, atop - 1), 1] + [rng.randrange(1, atop + 1) for _ in range(40)]
+        ss = [smax, -smax, 1, -1, 0] + [rng.randrange(-smax, smax + 1) for _ in range(40)]
+        for s in ss:
+            for a in As[:3] + [As[3 + (s % 40)]]:
+                raw, res, exact = fs.hostemu_fp_pair(p, s, a)
+                M = a * (a + 1)
+                assert res == (s * M) % p, (p, s, a)
+                assert raw % p == res and abs(raw) <= 0.503 * p, (p, s, a, raw)
+                assert exact == (p * (a + 1) ** 2 <= 2 ** 53), (p, a)
+    p = 2 ** 40 + 15
+    a = math.isqrt(p // 4)                                   # 4 (a+1)^2 > p: no paired step
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_fp_pair(p, 5, a)
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_fp_pair(2 ** 52 + 1, 5, 2 ** 22)         # a + 1 above 2^22
+
+
+def oracle_iterate(p, n0, n1, start, cap=64):
+    """The oracle's iterate from a canonical residue: (end residue, factors)."""
+    O = ol.oracle()
+    M = ol.mont(p)
+    s6 = ol.u64(O.ora_mmul(start, M.r2, p, M.q))
+    s7 = ol.u64(O.ora_mmul(n0 % p, M.r2, p, M.q))
+    buf = (ol.OFactor * cap)()
+    nf = O.ora_iterate(ctypes.byref(M), n0, n1, ctypes.byref(s6), ctypes.byref(s7), buf, cap)
+    return O.ora_mmul(s6.value, 1, p, M.q), [(buf[i].p, buf[i].nc) for i in range(min(nf, cap))]
+
+
+def test_fp64_pair_loop_matches_oracle(fs):
+    """The FP64 iterate with paired steps against the oracle: moduli on both
+    sides of 4 n1^2 = p (paired loop just inside, single-step loop just
+    outside), with and without the exact product, from random start residues
+    and from residues that put +-1 at every position of an 8-step block.  The
+    host execution also returns an error if one paired step ever differs from
+    two single steps, leaves |s| > 0.51p, or if the block's compares
+    (|v| == a*(a+1), |v| == a+1) disagree with an exact test of its residues."""
+    rng = random.Random(43)
+    cases = []
+    for n1 in (1040, 5003, 20000, 300000, 2 ** 22):
+        edge = 4 * n1 * n1
+        cases += [(edge + 1, n1), (edge + 3, n1), (edge - 1, n1)]   # pair, pair, single
+    cases += [(2 ** 26 + 15, 1030), (2 ** 33 + 17, 20001), (10 ** 12 + 39, 200000), (2 ** 53 - 111, 2 ** 22)]
+    for p, n1 in cases:
+        for n0 in (max(1, n1 - 8 * 4 - 3), max(1, n1 - 17)):
+            starts = [rng.randrange(p) for _ in range(2)]
+            for pos in range(8):
+                prod = 1
+                for j in range(1, pos + 1):
+                    prod = prod * (n0 + j) % p
+                if math.gcd(prod, p) == 1:
+                    starts.append(pow(prod, -1, p) * (1 if pos % 2 else -1) % p)
+            for start in starts:
+                r, facs, nf = fs.hostemu_iterate(fs.PATH_FP64, p, n0, n1, start)
+                want_r, want_f = oracle_iterate(p, n0, n1, start)
+                assert (r, facs[:nf]) == (want_r, want_f), (p, n0, n1, start)
+
+
 def test_hybrid_hi_step_at_error_bound_edges(fs):
     """The hybrid step with the quotient from the high word of s only (used
     when p >= n1 * 2^39): exact, and |s'| stays within 0.51p, at the largest
