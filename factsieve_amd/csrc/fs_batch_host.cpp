@@ -510,13 +510,22 @@ extern "C" int fs_batch_autotune(const fs_job* jobs, uint32_t nj, fs_batch_optio
   uint32_t nlen_max = 0;
   for (uint32_t k = 0; k < nj; ++k) nlen_max = std::max(nlen_max, jobs[k].nmax - jobs[k].nmin);
   std::vector<fs_job_result> res(nj);
+  // one warm run, then the faster of two timed runs (single runs differ by a
+  // few per cent, as much as some of the choices below)
   auto timed = [&](const fs_batch_options& o, double* sec) {
-    int rc = fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, sec);  // warm
+    int rc = fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, sec);
     if (rc) return rc;
-    return fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, sec);
+    double best_s = 1e30;
+    for (int k = 0; k < 2; ++k) {
+      if ((rc = fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, sec))) return rc;
+      best_s = std::min(best_s, *sec);
+    }
+    *sec = best_s;
+    return FS_OK;
   };
   fs_batch_options best = *opt;
   if (best.tile_candidates == 0 || best.tile_candidates % FS_BLOCK) best.tile_candidates = 256;
+  if (best.persistent_blocks == 0) best.persistent_blocks = cus * 8;  // the launch's default, made explicit
   double tbest = 1e30;
   int rc = timed(best, &tbest);
   if (rc) return rc;

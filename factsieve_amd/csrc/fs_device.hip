@@ -140,7 +140,7 @@ __global__ __launch_bounds__(FS_BLOCK) void k_iterate_fp64(fs_cands c, uint32_t*
   if (i >= fs_live(ctr, c.cap)) return;
   const uint64_t p = c.p[i];
   if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
-  c.res[i] = fs_cand_iter_fp64(p, c.res[i], n0, n1, nt, pt, true, FS_REC(p));
+  c.res[i] = fs_cand_iter_fp64<FS_UNROLL_DEFAULT, 16>(p, c.res[i], n0, n1, nt, pt, true, FS_REC(p));
   c.nnext[i] = n1;
 }
 
@@ -616,13 +616,14 @@ static int ntab_for(hipStream_t st, uint32_t first, uint32_t count, const double
                     const double** pout = nullptr) {
   std::lock_guard<std::mutex> lk(g_ntab_mu);
   NTab& t = g_ntab[st];
-  if (t.d && first >= t.base && (uint64_t)first + count + 8 <= (uint64_t)t.base + t.len) {
+  if (t.d && first >= t.base && (uint64_t)first + count + 16 <= (uint64_t)t.base + t.len) {
     *out = t.d + (first - t.base);
     if (pout) *pout = t.pairs_at(first);
     return FS_OK;
   }
-  // 8 entries of padding: the kernel reads one unrolled block ahead
-  const uint32_t want = (count < 65536 ? 65536 : count) + 8;
+  // 16 entries of padding: the kernels read one unrolled block ahead (8
+  // multipliers, or 8 pairs at 16 steps per block)
+  const uint32_t want = (count < 65536 ? 65536 : count) + 16;
   if (want > t.cap) {
     if (t.d) HIPCHK(hipFree(t.d), "hipFree(ntab)");
     t.d = nullptr;

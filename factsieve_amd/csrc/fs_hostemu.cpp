@@ -209,7 +209,8 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     uint32_t n = n0;
     if (fs_fp_pair_ok(p, n1)) {
       // The paired loop: one product by M = a*(a+1) per two positions, the
-      // compares |v| == M and |v| == a+1 on its result.  The odd positions'
+      // compares |v| == M and |v| == a+1 on its result (each pair's compares are
+      // exact on their own, so the device's block length does not matter here).  The odd positions'
       // residues are formed here by single steps only to check that the
       // compares fire exactly for the blocks that hold a +-1 residue.
       const bool pexact = fs_fp_pair_exact_ok(p, n1);

@@ -341,8 +341,10 @@ __device__ __forceinline__ uint64_t fs_cand_setup_mont(uint64_t p, uint64_t q, u
 // the next residue v[k+1] = +-m[k] exactly (up to a multiple of p it is +-m[k];
 // the symmetric step leaves it at most m[k] in magnitude), so |v[k+1]| <= m[k]
 // holds whenever v[k] or v[k+1] is +-1.  It also holds for other residues of
-// magnitude <= m[k] (probability about 2 m / p per test); the taken branch then
-// tests every residue of the block exactly.
+// magnitude <= m[k] (probability about 2 m / p per test).  The tests' wave masks
+// (ballots: each compare writes a scalar register pair) are or-ed on the scalar
+// unit, and a block whose mask is not zero takes a uniform branch in which
+// every lane tests every residue of the block exactly.
 
 // FP64 path.  nt[j] = (double)(n0 + 1 + j), read with scalar loads, with at
 // least U entries past n1 - n0; six FP64 operations per step (four when
@@ -371,9 +373,9 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t 
     for (int k = 0; k < U; ++k)
       v[k + 1] = EXACT ? fs_fp_mulmod_exact(v[k], m[k], pd, pinv) : fs_fp_mulmod(v[k], m[k], pd, pinv);
     s = v[U];
-    bool hit = false;
+    uint64_t hit = 0;  // wave masks of the tests, or-ed on the scalar unit
 #pragma unroll
-    for (int k = 0; k < U; k += 2) hit |= (__builtin_fabs(v[k + 1]) <= m[k]);
+    for (int k = 0; k < U; k += 2) hit |= __ballot(__builtin_fabs(v[k + 1]) <= m[k]);
 #pragma unroll
     for (int k = 0; k < U; ++k) m[k] = x[k];
     if (hit) {
@@ -403,40 +405,62 @@ template <int U, bool EXACT, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                                 const double* __restrict__ nt,
                                                                 const double* __restrict__ pt, Rec rec) {
-  static_assert(U == 2 || U == 4 || U == 8, "unroll depth");
+  static_assert(U == 2 || U == 4 || U == 8 || U == 16, "unroll depth");
   constexpr int H = U / 2;
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = fs_canon_to_fp(s_in, p);
   uint32_t n = n0;
   const uint32_t nu = n0 + ((n1 - n0) & ~(uint32_t)(U - 1));
-  double w[U];
-#pragma unroll
-  for (int k = 0; k < U; ++k) w[k] = pt[k];
-  for (; n < nu; n += U) {
-    const double* wn = pt + (n - n0) + U;  // the next block's H pairs
-    double x[U], v[H + 1];
-#pragma unroll
-    for (int k = 0; k < U; ++k) x[k] = wn[k];
+  // One block of H paired steps from position nb with the pairs in w; false
+  // when it holds no +-1 residue.
+  auto block = [&](const double (&w)[U], uint32_t nb) {
+    double v[H + 1];
     v[0] = s;
 #pragma unroll
     for (int i = 0; i < H; ++i)
       v[i + 1] = EXACT ? fs_fp_mulmod_exact(v[i], w[2 * i], pd, pinv) : fs_fp_mulmod(v[i], w[2 * i], pd, pinv);
     s = v[H];
-    bool hit = false;
+    // wave masks of the compares (each one v_cmp into a scalar register
+    // pair), or-ed on the scalar unit; the exact tests below run for the whole
+    // wave when any lane's compare fired
+    uint64_t hit = 0;
 #pragma unroll
     for (int i = 0; i < H; ++i)
-      hit |= (__builtin_fabs(v[i + 1]) == w[2 * i]) | (__builtin_fabs(v[i + 1]) == w[2 * i + 1]);
-#pragma unroll
-    for (int k = 0; k < U; ++k) w[k] = x[k];
+      hit |= __ballot(__builtin_fabs(v[i + 1]) == w[2 * i]) | __ballot(__builtin_fabs(v[i + 1]) == w[2 * i + 1]);
     if (hit) {
 #pragma unroll
       for (int i = 0; i < H; ++i) {
-        const uint32_t nn = n + 2 * i;
+        const uint32_t nn = nb + 2 * i;
         if (__builtin_fabs(v[i]) == 1.0) rec(v[i] > 0.0 ? -(int32_t)nn : (int32_t)nn);
         const double u = fs_fp_mulmod(v[i], nt[nn - n0], pd, pinv);  // residue of nn + 1
         if (__builtin_fabs(u) == 1.0) rec(u > 0.0 ? -(int32_t)(nn + 1) : (int32_t)(nn + 1));
       }
     }
+  };
+  // Two register sets for the pairs, alternating by block.  Scalar loads
+  // complete out of order (a wait is for all of them), so each block first
+  // takes its own set (the empty asm that reads it waits for its load), then
+  // issues the next block's load into the other set, pinned above this block's
+  // steps by the asm that claims to update s; no set is copied.
+  double wa[U], wb[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) wa[k] = pt[k];
+  while (n < nu) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" ::"s"(wa[k]));
+#pragma unroll
+    for (int k = 0; k < U; ++k) wb[k] = pt[(n - n0) + U + k];
+    asm volatile("" : "+v"(s)::"memory");
+    block(wa, n);
+    n += U;
+    if (n >= nu) break;
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" ::"s"(wb[k]));
+#pragma unroll
+    for (int k = 0; k < U; ++k) wa[k] = pt[(n - n0) + U + k];
+    asm volatile("" : "+v"(s)::"memory");
+    block(wb, n);
+    n += U;
   }
   for (; n < n1; ++n) {
     if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
@@ -450,13 +474,13 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint
 // exact-product step when every active lane's modulus allows it.  pt is always
 // a valid table pointer (never selected against a null pointer), which keeps
 // its loads scalar.
-template <int U = FS_UNROLL_DEFAULT, class Rec>
+template <int U = FS_UNROLL_DEFAULT, int UP = U, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                       const double* __restrict__ nt, const double* __restrict__ pt,
                                                       bool pairs, Rec rec) {
   if (pairs && __all(fs_fp_pair_ok(p, n1))) {
-    if (__all(fs_fp_pair_exact_ok(p, n1))) return fs_cand_iter_fp64_pair_loop<U, true>(p, s_in, n0, n1, nt, pt, rec);
-    return fs_cand_iter_fp64_pair_loop<U, false>(p, s_in, n0, n1, nt, pt, rec);
+    if (__all(fs_fp_pair_exact_ok(p, n1))) return fs_cand_iter_fp64_pair_loop<UP, true>(p, s_in, n0, n1, nt, pt, rec);
+    return fs_cand_iter_fp64_pair_loop<UP, false>(p, s_in, n0, n1, nt, pt, rec);
   }
   if (__all(fs_fp_exact_ok(p, n1))) return fs_cand_iter_fp64_loop<U, true>(p, s_in, n0, n1, nt, rec);
   return fs_cand_iter_fp64_loop<U, false>(p, s_in, n0, n1, nt, rec);
@@ -509,7 +533,7 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_loop(uint64_t p, uint64_
     const double thr = HI ? thr_hi : thr15 * m[U - 2];  // m[U-2]: the block's largest even-position multiplier
 #pragma unroll
     for (int k = 0; k < U; ++k) m[k] = x[k];
-    if (ymin <= thr) {
+    if (__ballot(ymin <= thr)) {  // whole wave: exact tests where any lane's filter passed
 #pragma unroll
       for (int k = 0; k < U; ++k)
         if ((v[k] == 1) | (v[k] == -1)) rec(v[k] == 1 ? -(int32_t)(n + k) : (int32_t)(n + k));
