@@ -170,7 +170,7 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__
         if (!first && c.nnext[g] != n0) atomicOr(&jflag[jo], FS_BJF_VALIDATE);
         if (J.path == FS_PATH_FP64) {
           s = first ? fs_cand_setup_fp64(p, q, r2, 0, (const double*)jp, 0, J.prog_terms) : c.res[g];
-          s = fs_cand_iter_fp64<U>(p, s, n0, n1, nt, pt, pairs, rec);
+          s = fs_cand_iter_fp64<U, (U == 8 ? 16 : U)>(p, s, n0, n1, nt, pt, pairs, rec);
         } else if (J.path == FS_PATH_HYBRID) {
           s = first ? fs_cand_setup_hybrid(p, q, r2, 0, jp, J.prog_terms, 0, J.prog_terms) : c.res[g];
           s = fs_cand_iter_hybrid<U>(p, s, n0, n1, nt, rec);
