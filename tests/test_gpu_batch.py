@@ -67,6 +67,34 @@ def test_batch_shape_invariance(gpu, opts):
     assert [outcome(o) for o in base] == [outcome(o) for o in other]
 
 
+def pair_mix():
+    # FP64 jobs with 4 nmax^2 <= p (paired steps): with and without the exact
+    # product, odd and even first multipliers, one straddling 4 nmax^2 = p
+    return [
+        (4 * 10 ** 6, 4 * 10 ** 6 + 400000, 10, 1000),           # exact pairs, factors present
+        (4 * 10 ** 6 + 400000, 4 * 10 ** 6 + 800000, 11, 997),   # odd nmin and nmax
+        (10 ** 10, 10 ** 10 + 200000, 20001, 49999),             # inexact pairs
+        (1600000000 - 100000, 1600000000 + 100000, 19500, 20000),  # paired and single waves
+    ]
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(n_chunk=777), dict(n_chunk=1001, unroll=4), dict(unroll=2),
+                                  dict(lanes_per_prime=2), dict(lanes_per_prime=4, n_chunk=1333)])
+def test_batch_paired_steps_match_oracle(gpu, opts):
+    """Paired FP64 steps in the batched kernel read an absolute pair table split
+    by the parity of the first multiplier; n-chunks of odd length and lanes
+    per prime start tiles at both parities.  Every job against the oracle."""
+    jobs = pair_mix()
+    print("options", opts, flush=True)
+    res, _ = gpu.batch(jobs, **opts)
+    for job, o in zip(jobs, res):
+        want = ol.sieve(*job, threads=8)
+        assert o.status == 0
+        assert (o.factorcount, o.primecount, o.checksum) == (want["factorcount"], want["primecount"],
+                                                             want["checksum"]), job
+        assert o.factors == want["factors"]
+
+
 def long_n_mix():
     # n-ranges long enough to be cut over several lanes per prime (subranges of >= 1024 n)
     return [

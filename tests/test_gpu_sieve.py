@@ -141,6 +141,19 @@ def test_segmentation_and_chunking_invariance(gpu):
     assert [list(f) for f in r.factors] == C1["factors"]
 
 
+@pytest.mark.parametrize("n_chunk", [777, 1000, 4096])
+def test_paired_steps_chunked_match_oracle(gpu, n_chunk):
+    """Paired FP64 steps over n-chunks of odd and even length (launches whose
+    first multiplier has either parity, each reading its pairs at its own
+    offset of the stream's pair table), against the oracle."""
+    args = (4 * 10 ** 6, 5 * 10 ** 6, 11, 997)
+    r = gpu.sieve(*args, n_chunk=n_chunk)
+    o = ol.sieve(*args, threads=8)
+    assert r.iterate_launches > 1 or n_chunk > 986
+    assert (r.factorcount, r.primecount, r.checksum) == (o["factorcount"], o["primecount"], o["checksum"])
+    assert r.factors == o["factors"]
+
+
 @pytest.mark.parametrize("args", [
     (10 ** 8, 10 ** 8, 1000, 2000),          # empty p-range
     (10 ** 8, 10 ** 8 + 100000, 5000, 5000),  # empty n-range
