@@ -21,7 +21,9 @@
 //    gives s' == s*m - k*p exactly with |s'| < 0.8 p (iterate's m < 2^31 gives
 //    |s'| <= 0.5 p + 2^-21 p).  Every integer below 2^53 (p, residues, n) is an
 //    exact double.  Because |s| < p - 1 (p > 5),
-//    s == +-1 (mod p) iff |s| == 1.  Seven full-rate FP64 VALU operations.
+//    s == +-1 (mod p) iff |s| == 1.  Six full-rate FP64 VALU operations (four
+//    when s*m is exact, fs_fp_mulmod_exact); the iterate loops multiply by
+//    (n+1)(n+2) at once when 4 n1^2 <= p (fs_fp_pair_ok).
 //
 //  * Hybrid (p <= 2^64 - 2^46, used above 2^53): signed residue |s| <= 0.51p (and
 //    below 2^63: the step keeps |s| <= (1/2 + 2^-19) p), multiplier
