@@ -161,6 +161,13 @@ __device__ __forceinline__ void fs_primegen_block(uint64_t base, uint64_t high, 
 // multiples of 4 (block boundaries) and the program words are 32-byte aligned.
 struct alignas(32) fs_prog_block {
   uint64_t w[4];
+  // the square flag (bit 63), extracted by a scalar shift of the high half:
+  // the compiler otherwise tests it with a 64-bit vector compare per block
+  __device__ __forceinline__ bool square() const {
+    uint32_t f;
+    asm("s_lshr_b32 %0, %1, 31" : "=s"(f) : "s"((uint32_t)(w[0] >> 32)));
+    return f != 0;
+  }
 };
 
 // FP64 program (p < 2^53): multipliers < 2^50 as double bits.
@@ -171,7 +178,7 @@ __device__ __forceinline__ uint64_t fs_cand_setup_fp64(uint64_t p, uint64_t q, u
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = (start == 0) ? 1.0 : fs_canon_to_fp(s_in, p);
   auto square = [&](const fs_prog_block& blk) {  // first use of a block: waits for its load
-    if (blk.w[0] & FS_PROG_SQUARE) {
+    if (blk.square()) {
       const uint64_t a = fs_fp_to_canon(s, p);
       s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
     }
@@ -225,7 +232,7 @@ template <class Mul, class Sq>
 __device__ __forceinline__ uint64_t fs_setup_blocks(uint64_t s, const fs_prog_block* __restrict__ prog, uint32_t b0,
                                                     uint32_t b1, Mul mul, Sq sq) {
   auto square = [&](const fs_prog_block& blk) {
-    if (blk.w[0] & FS_PROG_SQUARE) s = sq(s);
+    if (blk.square()) s = sq(s);
   };
   auto terms4 = [&](const fs_prog_block& blk) {
     s = mul(s, blk.w[0] & ~FS_PROG_SQUARE);
