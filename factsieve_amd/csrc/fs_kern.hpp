@@ -400,8 +400,8 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t 
 
 // FP64 path, paired steps (fs_fp_pair_ok(p, n1)): one product by
 // M = a*(a+1) per two multipliers.  pt[2j] = (n0+1+2j)*(n0+2+2j) and
-// pt[2j+1] = n0+2+2j, read with scalar loads, with at least U entries past
-// n1 - n0.  With v[k] the residue of n0 + k and M, a+1 the pair's values,
+// pt[2j+1] = n0+2+2j, read with scalar loads, with at least U entries (U/2
+// pairs) past index n1 - n0.  With v[k] the residue of n0 + k and M, a+1 the pair's values,
 // v[k+2] = v[k] * M and v[k+2] = v[k+1] * (a+1), so, both values being below
 // p/4 and every residue of magnitude <= 0.51 p, |v[k+2]| == M exactly when
 // v[k] = +-1 and |v[k+2]| == a+1 exactly when v[k+1] = +-1.  Per two steps:
@@ -418,8 +418,8 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint
   double s = fs_canon_to_fp(s_in, p);
   uint32_t n = n0;
   const uint32_t nu = n0 + ((n1 - n0) & ~(uint32_t)(U - 1));
-  // One block of H paired steps from position nb with the pairs in w; false
-  // when it holds no +-1 residue.
+  // One block of H paired steps from position nb with the pairs in w, and its
+  // +-1 records.
   auto block = [&](const double (&w)[U], uint32_t nb) {
     double v[H + 1];
     v[0] = s;
