@@ -507,6 +507,10 @@ class Segment:
         h = (u64 * max(len(values), 1))(*values)
         check(lib().fs_memcpy_htod(getattr(self.c, field), h, len(values) * 8, self.stream, 1))
 
+    def write_u32(self, field, values):
+        h = (u32 * max(len(values), 1))(*values)
+        check(lib().fs_memcpy_htod(getattr(self.c, field), h, len(values) * 4, self.stream, 1))
+
     def factors(self):
         n = min(self.counters()[CTR_NFACTORS], self.factor_cap)
         h = (Factor * max(n, 1))()

@@ -2,6 +2,7 @@
 self-test vectors, the oracle's golden fixtures and full-size invariants.
 Every call goes through the C ABI of libfactsieve.so."""
 import json
+import math
 import os
 import subprocess
 
@@ -23,6 +24,18 @@ def paths_for(fs, pmax):
     if pmax <= 2 ** 64 - 2 ** 46 + 1:
         return [fs.PATH_HYBRID, fs.PATH_MONT]
     return [fs.PATH_MONT]
+
+
+def oracle_iterate(p, n0, n1, start, cap=64):
+    """The oracle's iterate from a canonical residue: (end residue, factors)."""
+    import ctypes
+    O = ol.oracle()
+    M = ol.mont(p)
+    s6 = ol.u64(O.ora_mmul(start, M.r2, p, M.q))
+    s7 = ol.u64(O.ora_mmul(n0 % p, M.r2, p, M.q))
+    buf = (ol.OFactor * cap)()
+    nf = O.ora_iterate(ctypes.byref(M), n0, n1, ctypes.byref(s6), ctypes.byref(s7), buf, cap)
+    return O.ora_mmul(s6.value, 1, p, M.q), [(buf[i].p, buf[i].nc) for i in range(min(nf, cap))]
 
 
 @pytest.mark.parametrize("k", range(5))
@@ -92,6 +105,54 @@ def test_kernels_per_candidate(gpu, ci):
             ctr, sums = seg.counters(), seg.sums()
             assert ctr[gpu.CTR_VALIDATE] == 0 and ctr[gpu.CTR_MAXCOUNT] == n
             assert sums[0] == n and f"{sums[1]:016X}" == case["device_checksum"]
+        finally:
+            seg.close()
+
+
+@pytest.mark.parametrize("lo,hi", [(2 ** 40, 2 ** 40 + 60000), (2 ** 62, 2 ** 62 + 60000),
+                                   (2 ** 64 - 2 ** 46 - 60000, 2 ** 64 - 2 ** 46 + 1), (2 ** 64 - 60000, 2 ** 64 - 1)])
+def test_iterate_at_largest_n(gpu, lo, hi):
+    """The iterate kernel of every path at the top of the n range (n up to
+    2^31 - 1, the largest the factor records hold), from random start residues
+    and from residues that put +-1 at chosen n, against the oracle."""
+    import random
+    n0, n1 = 2 ** 31 - 2900, 2 ** 31 - 1
+    rng = random.Random(lo)
+    for path in paths_for(gpu, hi):
+        print(f"largest n: p=[{lo},{hi}) path {path}", flush=True)
+        seg = gpu.Segment(cap=8192)
+        try:
+            seg.clearresult()
+            seg.clearn()
+            seg.getsegprimes(lo, hi)
+            n = seg.counters()[gpu.CTR_COUNT]
+            assert 0 < n <= 8192
+            ps = seg.read_u64("p", n)
+            starts, planted = [], {}
+            for i, p in enumerate(ps):
+                k, sign = rng.randrange(0, n1 - n0), rng.choice((1, -1))
+                prod = 1
+                if i % 7 == 0:      # +-1 at n0 + k: start = +-((n0+1)...(n0+k))^-1
+                    for j in range(1, k + 1):
+                        prod = prod * (n0 + j) % p
+                if i % 7 == 0 and math.gcd(prod, p) == 1:
+                    starts.append(sign * pow(prod, -1, p) % p)
+                    planted[p] = (p, -(n0 + k) if sign == 1 else n0 + k)
+                else:
+                    starts.append(rng.randrange(p))
+            seg.write_u64("res", starts)
+            seg.write_u32("nnext", [n0] * n)
+            seg.iterate(n0, n1, path)
+            res = seg.read_u64("res", n)
+            facs = sorted(seg.factors())
+            assert seg.counters()[gpu.CTR_VALIDATE] == 0
+            want_f = []
+            for p, s0, r in zip(ps, starts, res):
+                want_r, f = oracle_iterate(p, n0, n1, s0)
+                assert r == want_r, (path, p)
+                want_f += f
+            assert facs == sorted(want_f)
+            assert set(planted.values()) <= set(facs)
         finally:
             seg.close()
 
