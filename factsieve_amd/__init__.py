@@ -26,6 +26,7 @@ FS_OK = 0
 FS_E_ARG, FS_E_HIP, FS_E_NOMEM, FS_E_OVERFLOW, FS_E_VALIDATE = -1, -2, -3, -4, -5
 FS_E_POWERTABLE, FS_E_VERIFY, FS_E_IO, FS_E_NODEV, FS_E_BUSY = -6, -7, -8, -9, -10
 PATH_AUTO, PATH_FP64, PATH_HYBRID, PATH_MONT = 0, 1, 2, 3
+VERIFY_AUTO, VERIFY_CPU, VERIFY_GPU = 0, 1, 2
 CTR_COUNT, CTR_MAXCOUNT, CTR_NFACTORS, CTR_POWERTABLE = 0, 1, 2, 3
 CTR_PRIMEOVF, CTR_VALIDATE, CTR_FACTOROVF, CTR_WORDS = 4, 5, 6, 8
 SUM_WORDS = 2
@@ -47,7 +48,8 @@ class Options(ctypes.Structure):
     _fields_ = [("path", ctypes.c_int), ("verify_threads", ctypes.c_int), ("write_files", ctypes.c_int),
                 ("resume", ctypes.c_int), ("quiet", ctypes.c_int), ("test_mode", ctypes.c_int),
                 ("seg_candidates", u32), ("n_chunk", u32),
-                ("checkpoint_secs", u32), ("workdir", ctypes.c_char_p), ("stop_after_segments", u64)]
+                ("checkpoint_secs", u32), ("workdir", ctypes.c_char_p), ("stop_after_segments", u64),
+                ("verify_device", ctypes.c_int)]
 
 
 class WorkStatus(ctypes.Structure):
@@ -133,6 +135,7 @@ _SIGS = [
     ("fs_is_prime", ctypes.c_int, [u64]),
     ("fs_verify_factor", ctypes.c_int, [u64, u32, i32]),
     ("fs_verify_factors", i64, [ctypes.POINTER(Factor), u64, ctypes.c_int]),
+    ("fs_verify_factors_gpu", i64, [ctypes.POINTER(Factor), u64, vp]),
     ("fs_read_checkpoint", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(WorkStatus), ctypes.POINTER(ctypes.c_int)]),
     ("fs_write_checkpoint", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(WorkStatus), ctypes.c_int]),
     ("fs_parse_uint64", ctypes.c_int, [ctypes.POINTER(u64), ctypes.c_char_p, u64, u64]),
@@ -226,6 +229,18 @@ def verify_factors(factors, threads=16):
     for i, (p, nc) in enumerate(factors):
         arr[i].p, arr[i].nc, arr[i].pad = p, nc, 0
     r = lib().fs_verify_factors(arr, len(factors), threads)
+    if r < -1:
+        raise FactSieveError(r, last_error())
+    return r
+
+
+def verify_factors_gpu(factors, stream=None):
+    """The same verification on the GPU (fs_verify_factors_gpu): index of the
+    first factor that fails, or -1."""
+    arr = (Factor * max(len(factors), 1))()
+    for i, (p, nc) in enumerate(factors):
+        arr[i].p, arr[i].nc, arr[i].pad = p, nc, 0
+    r = lib().fs_verify_factors_gpu(arr, len(factors), stream)
     if r < -1:
         raise FactSieveError(r, last_error())
     return r

@@ -392,26 +392,29 @@ int plan_jobs(const fs_job* jobs, uint32_t nj, const fs_batch_options& o, std::v
 
 }  // namespace
 
-// Verify, filter and fold the factors of one job; returns kept factors.
-// Sorts each listed job's factors and verifies all of them in one pass over
-// the CPU threads (factors of different jobs share the pool).  ok[j] = 0 for a
-// job with a factor that fails verification.
+// Sorts each listed job's factors and verifies all of them in one pass (the
+// CPU threads, or the GPU when the pass needs more than 2^24 multiplications;
+// factors of different jobs share it).  ok[j] = 0 for a job with a factor that
+// fails verification, or for every listed job when the verifier itself fails.
 static void verify_jobs(std::vector<std::vector<fs_factor>>& per, const std::vector<uint32_t>& list, int threads,
-                        std::vector<char>& ok) {
+                        hipStream_t st, std::vector<char>& ok) {
   std::vector<fs_factor> all;
   for (uint32_t j : list) {
     for (auto& x : per[j]) x.pad = 0;
     fs_sort_factors(per[j].data(), per[j].size());
     all.insert(all.end(), per[j].begin(), per[j].end());
   }
-  if (fs_verify_all(all.data(), all.size(), threads) < 0) return;
+  const int64_t r = fs_verify_select(all.data(), all.size(), threads, FS_VERIFY_AUTO, (fs_stream)st);
+  if (r == -1) return;
   for (uint32_t j : list) {  // a failure: find the job(s) it belongs to
-    const int64_t bad = fs_verify_all(per[j].data(), per[j].size(), threads);
+    const int64_t bad = r < -1 ? r : fs_verify_all(per[j].data(), per[j].size(), threads);
     if (bad >= 0) {
       char buf[160];
       snprintf(buf, sizeof buf, "CPU factor verification failed for p=%llu", (unsigned long long)per[j][bad].p);
       fs_set_error_msg(buf);
       ok[j] = 0;
+    } else if (bad < -1) {
+      ok[j] = 0;  // the verifier's own error stays in fs_last_error
     }
   }
 }
@@ -465,7 +468,7 @@ extern "C" int fs_batch(const fs_job* jobs, uint32_t nj, const fs_batch_options*
   for (uint32_t j = 0; j < nj; ++j)
     if (plan[j].status == FS_OK && !(dev.flag[j] & FS_BJF_VALIDATE)) finished.push_back(j);
   std::vector<char> verified(nj, 1);
-  verify_jobs(per, finished, o.verify_threads, verified);
+  verify_jobs(per, finished, o.verify_threads, E.st, verified);
   uint64_t written = 0;
   for (uint32_t j = 0; j < nj; ++j) {
     fs_job_result& r = out[j];

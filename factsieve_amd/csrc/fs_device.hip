@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <vector>
 #include "fs_arith.hpp"
 #include "fs_kern.hpp"
 #include "../../include/factsieve.h"
@@ -289,6 +290,64 @@ __global__ __launch_bounds__(FS_BLOCK) void k_verifyresult(fs_cands c, uint32_t*
     const uint64_t plain = fs_mont_mul(a, 1, p, q);
     if (a != b || plain != c.res[0]) atomicOr(&ctr[FS_CTR_POWERTABLE], 1u);
   }
+}
+
+// ---- factor verification on the device (verify, verifyprime.c:137-164)
+//
+// The reference re-checks every factor p | n! +- 1 on the CPU with an O(n)
+// product (an OpenMP loop, cl_sieve.cpp:491-515).  Here each factor's product
+// 2 * 3 * ... * n mod p is cut into ranges of FS_VRANGE consecutive multipliers,
+// one range per thread, with integer Montgomery products and a Montgomery
+// counter (not the FP64 or hybrid arithmetic of the sieve, so the check stays
+// independent of it); a second kernel multiplies each factor's range products.
+// first[i] is the index of factor i's first range (first[nf] = the range
+// count); factors with no range (n < 2, n >= p, even p, p < 3) are decided on
+// the host.
+
+#define FS_VRANGE 4096u
+
+__device__ __forceinline__ uint32_t fs_vrange_owner(const uint32_t* first, uint32_t nf, uint32_t t) {
+  uint32_t lo = 0, hi = nf - 1;  // last i with first[i] <= t
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_verify_ranges(const fs_factor* __restrict__ f,
+                                                            const uint32_t* __restrict__ first, uint32_t nf,
+                                                            uint32_t nrg, uint64_t* __restrict__ part) {
+  const uint32_t t = blockIdx.x * FS_BLOCK + threadIdx.x;
+  if (t >= nrg) return;
+  const uint32_t i = fs_vrange_owner(first, nf, t);
+  const uint64_t p = f[i].p;
+  const uint32_t n = (uint32_t)(f[i].nc < 0 ? -(int64_t)f[i].nc : f[i].nc);
+  const uint64_t lo = 2 + (uint64_t)(t - first[i]) * FS_VRANGE;
+  const uint64_t hi = min(lo + FS_VRANGE - 1, (uint64_t)n);
+  const uint64_t q = fs_invert(p), one = fs_r64_mod(p), r2 = fs_r2_from_one(one, p, q);
+  uint64_t rm = one, im = fs_mont_mul(lo, r2, p, q);  // Montgomery forms of 1 and lo (lo < p)
+  for (uint64_t m = lo; m <= hi; ++m) {
+    rm = fs_mont_mul(rm, im, p, q);
+    im = fs_add_mod(im, one, p);
+  }
+  part[t] = fs_mont_mul(rm, 1, p, q);
+}
+
+__global__ __launch_bounds__(FS_BLOCK) void k_verify_combine(const fs_factor* __restrict__ f,
+                                                             const uint32_t* __restrict__ first, uint32_t nf,
+                                                             const uint64_t* __restrict__ part,
+                                                             uint64_t* __restrict__ res) {
+  const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
+  if (i >= nf) return;
+  const uint32_t b = first[i], e = first[i + 1];
+  if (b == e) return;
+  const uint64_t p = f[i].p;
+  const uint64_t q = fs_invert(p), r2 = fs_r2_from_one(fs_r64_mod(p), p, q);
+  uint64_t r = part[b];
+  for (uint32_t t = b + 1; t < e; ++t) r = fs_mul_mod(r, part[t], p, q, r2);
+  res[i] = r;  // n! mod p
 }
 
 // ================================================================ C ABI: runtime
@@ -733,4 +792,86 @@ extern "C" int fs_verifyresult(const fs_cands* c, uint32_t* d_ctr, void* d_verif
     g_inject = 0;
   }
   return FS_OK;
+}
+
+// Factor verification on the device: the contract of fs_verify_factors.
+extern "C" int64_t fs_verify_factors_gpu(const fs_factor* f, uint64_t n, fs_stream s) {
+  if (n == 0) return -1;
+  if (!f) {
+    g_last_error = "fs_verify_factors_gpu: null factor list";
+    return FS_E_VERIFY;
+  }
+  if (n >= 0x7FFFFFFFull) {
+    g_last_error = "fs_verify_factors_gpu: too many factors";
+    return FS_E_VERIFY;
+  }
+  int rc = ensure_init();
+  if (rc) return rc;
+  const uint32_t nf = (uint32_t)n;
+  std::vector<uint32_t> first(nf + 1);
+  std::vector<char> direct(nf, 0);
+  uint64_t nrg = 0;
+  for (uint32_t i = 0; i < nf; ++i) {
+    first[i] = (uint32_t)nrg;
+    const uint64_t p = f[i].p;
+    const uint32_t fn = (uint32_t)(f[i].nc < 0 ? -(int64_t)f[i].nc : f[i].nc);
+    if (p < 3 || (p & 1) == 0 || fn < 2 || fn >= p) {
+      direct[i] = 1;
+      continue;
+    }
+    nrg += ((uint64_t)fn - 2) / FS_VRANGE + 1;
+    if (nrg >= 0xFFFFFFFFull) {
+      g_last_error = "fs_verify_factors_gpu: too many multiplier ranges";
+      return FS_E_VERIFY;
+    }
+  }
+  first[nf] = (uint32_t)nrg;
+  std::vector<uint64_t> res(nf, 0);
+  if (nrg > 0) {
+    fs_factor* d_f = nullptr;
+    uint32_t* d_first = nullptr;
+    uint64_t *d_part = nullptr, *d_res = nullptr;
+    auto release = [&]() {
+      (void)hipFree(d_f);
+      (void)hipFree(d_first);
+      (void)hipFree(d_part);
+      (void)hipFree(d_res);
+    };
+    hipError_t e = hipMalloc((void**)&d_f, (size_t)nf * sizeof(fs_factor));
+    if (e == hipSuccess) e = hipMalloc((void**)&d_first, ((size_t)nf + 1) * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_part, (size_t)nrg * 8);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_res, (size_t)nf * 8);
+    if (e != hipSuccess) {
+      release();
+      set_error("hipMalloc(verify)", e);
+      return FS_E_NOMEM;
+    }
+    e = hipMemcpyAsync(d_f, f, (size_t)nf * sizeof(fs_factor), hipMemcpyHostToDevice, S(s));
+    if (e == hipSuccess) e = hipMemcpyAsync(d_first, first.data(), ((size_t)nf + 1) * 4, hipMemcpyHostToDevice, S(s));
+    if (e == hipSuccess) {
+      k_verify_ranges<<<(uint32_t)((nrg + FS_BLOCK - 1) / FS_BLOCK), FS_BLOCK, 0, S(s)>>>(d_f, d_first, nf,
+                                                                                         (uint32_t)nrg, d_part);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+      k_verify_combine<<<(nf + FS_BLOCK - 1) / FS_BLOCK, FS_BLOCK, 0, S(s)>>>(d_f, d_first, nf, d_part, d_res);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(res.data(), d_res, (size_t)nf * 8, hipMemcpyDeviceToHost, S(s));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(s));
+    release();
+    if (e != hipSuccess) {
+      set_error("verify kernels", e);
+      return FS_E_HIP;
+    }
+  }
+  for (uint32_t i = 0; i < nf; ++i) {
+    const uint64_t p = f[i].p;
+    const int32_t c = f[i].nc < 0 ? -1 : 1;
+    const uint32_t fn = (uint32_t)(f[i].nc < 0 ? -(int64_t)f[i].nc : f[i].nc);
+    // p | n! + c: n! = -c (mod p), i.e. 1 for n!-1 and p-1 for n!+1
+    const bool ok = direct[i] ? fs_verify_factor(p, fn, c) != 0 : (c == -1 ? res[i] == 1 : res[i] == p - 1);
+    if (!ok) return (int64_t)i;
+  }
+  return -1;
 }

@@ -645,13 +645,31 @@ int64_t fs_verify_all(const fs_factor* f, size_t n, int threads) {
 }
 
 extern "C" int64_t fs_verify_factors(const fs_factor* f, uint64_t n, int threads) {
-  if (!f && n) return FS_E_ARG;
+  if (!f && n) {  // FS_E_ARG (-1) would read as "all pass"
+    fs_set_error_msg("fs_verify_factors: null factor list");
+    return FS_E_VERIFY;
+  }
   try {
     return fs_verify_all(f, (size_t)n, threads);
   } catch (const std::exception&) {
     fs_set_error_msg("fs_verify_factors: host allocation or thread creation failed");
     return FS_E_NOMEM;
   }
+}
+
+// The verifier a drain uses: the GPU (fs_verify_factors_gpu) when the mode asks
+// for it, or in auto mode when the factors need more than 2^24 multiplications
+// (the CPU loop then takes longer than the launch and transfers); else the CPU
+// threads.  Index of the first failure, -1, or an FS_E_* error below -1.
+int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s) {
+  uint64_t work = 0;
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t fn = (uint64_t)(f[i].nc < 0 ? -(int64_t)f[i].nc : f[i].nc);
+    work += fn < f[i].p ? fn : 0;
+  }
+  if (mode == FS_VERIFY_GPU || (mode == FS_VERIFY_AUTO && work > ((uint64_t)1 << 24)))
+    return fs_verify_factors_gpu(f, n, s);
+  return fs_verify_all(f, n, threads);
 }
 
 // Filter composites (2-PRP p) and fold kept factors into the checksum
@@ -764,7 +782,8 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
     if ((rc = fs_memcpy_dtoh(h.data(), d.fac, (size_t)nf * sizeof(fs_factor), d.s, 1))) return rc;
     fs_sort_factors(h.data(), nf);
     res->raw_factors += nf;
-    const int64_t bad = fs_verify_all(h.data(), nf, o.verify_threads);
+    const int64_t bad = fs_verify_select(h.data(), nf, o.verify_threads, o.verify_device, d.s);
+    if (bad < -1) return (int)bad;
     if (bad >= 0) {
       char buf[256];
       const fs_factor& f = h[bad];

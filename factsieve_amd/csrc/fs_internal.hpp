@@ -24,6 +24,9 @@ std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path);
 // drop composite p and fold kept factors into *checksum (returns kept count).
 void fs_sort_factors(fs_factor* f, size_t n);
 int64_t fs_verify_all(const fs_factor* f, size_t n, int threads);
+// CPU or GPU verifier by mode (FS_VERIFY_*) and the amount of work; the
+// contract of fs_verify_factors.
+int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s);
 size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose);
 uint32_t fs_auto_segment_candidates();
 // Range checks of setupSearch (cl_sieve.cpp:557-589); FS_OK or FS_E_ARG.

@@ -23,6 +23,8 @@ static void help() {
   printf("-v #\t\t\tOptional, specify the number of CPU threads used to verify factors.  Default is 2, max is 128.\n");
   printf("-s \t\t\tPerform self test to verify proper operation of the program with the current GPU.\n");
   printf("-d #\t\t\tOptional, GPU device index (default 0).\n");
+  printf("--verify-on cpu|gpu|auto\tOptional, where found factors are re-verified (default auto: on the GPU when the\n"
+         "\t\t\tfactors need more than 2^24 multiplications, else on the -v CPU threads).\n");
   printf("-h\t\t\tPrint this help\n");
   exit(EXIT_FAILURE);
 }
@@ -32,6 +34,7 @@ struct Args {
   uint64_t nmin = 0, nmax = 0;
   uint64_t threads = 2;
   uint64_t device = 0;
+  int verify_device = FS_VERIFY_AUTO;
   bool test = false;
 };
 
@@ -44,6 +47,12 @@ static int parse_option(int opt, const char* arg, Args& a) {
     case 'N': return fs_parse_uint64(&a.nmax, arg, 128, 0x7FFFFFFFull);
     case 'v': return fs_parse_uint64(&a.threads, arg, 1, 128);
     case 'd': return arg ? fs_parse_uint64(&a.device, arg, 0, 63) : 0;
+    case 'V':
+      if (!strcmp(arg, "cpu")) a.verify_device = FS_VERIFY_CPU;
+      else if (!strcmp(arg, "gpu")) a.verify_device = FS_VERIFY_GPU;
+      else if (!strcmp(arg, "auto")) a.verify_device = FS_VERIFY_AUTO;
+      else return -1;
+      return 0;
     case 's':
       a.test = true;
       fprintf(stderr, "Performing self test.\n");
@@ -64,7 +73,8 @@ static void die(const char* what) {
 
 int main(int argc, char** argv) {
   static const struct option long_opts[] = {
-      {"device", optional_argument, 0, 'd'}, {"test", no_argument, 0, 's'}, {0, 0, 0, 0}};
+      {"device", optional_argument, 0, 'd'}, {"test", no_argument, 0, 's'},
+      {"verify-on", required_argument, 0, 'V'}, {0, 0, 0, 0}};
   Args a;
   fprintf(stderr, "\nFactSieve v%s\n", FS_VERSION);
   printf("\nFactSieve v%s\n", FS_VERSION);
@@ -111,6 +121,7 @@ int main(int argc, char** argv) {
   fs_options o;
   fs_default_options(&o);
   o.verify_threads = (int)a.threads;
+  o.verify_device = a.verify_device;
   o.write_files = 1;
   o.resume = 1;
   o.quiet = 0;

@@ -277,7 +277,8 @@ extern "C" int fs_server_step(fs_server h) {
     if (r.status == FS_OK) {
       for (auto& x : w.raw) x.pad = 0;
       fs_sort_factors(w.raw.data(), w.raw.size());
-      if (fs_verify_all(w.raw.data(), w.raw.size(), s->o.verify_threads) >= 0) {
+      if (fs_verify_select(w.raw.data(), w.raw.size(), s->o.verify_threads, FS_VERIFY_AUTO, (fs_stream)s->E.st) !=
+          -1) {
         r.status = FS_E_VERIFY;
       } else {
         const size_t kept = fs_filter_factors(w.raw.data(), w.raw.size(), &ck, false);

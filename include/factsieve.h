@@ -188,6 +188,12 @@ int fs_verify_factor(uint64_t p, uint32_t n, int32_t c);  /* verify, verifyprime
  * CPU threads; the sign of nc gives c (negative: n!-1).  Returns the index of
  * the first factor that fails, -1 if all pass, or a negative FS_E_* below -1. */
 int64_t fs_verify_factors(const fs_factor* f, uint64_t n, int threads);
+/* The same verification on the GPU (verify, verifyprime.c:137-164, in place of
+ * the CPU loop of cl_sieve.cpp:491-515): each factor's product 2..n mod p is cut
+ * into ranges of 4096 multipliers, one per thread, with integer Montgomery
+ * products (independent of the sieve's FP64 and hybrid arithmetic).  Same
+ * return contract as fs_verify_factors; runs on stream s and waits for it. */
+int64_t fs_verify_factors_gpu(const fs_factor* f, uint64_t n, fs_stream s);
 int fs_parse_uint64(uint64_t* result, const char* str, uint64_t lo, uint64_t hi); /* putil.c:33-115 */
 
 /* ----------------------------- host execution of the kernel arithmetic */
@@ -239,7 +245,14 @@ typedef struct {
   uint32_t checkpoint_secs; /* drain + checkpoint period (reference: 60) */
   const char* workdir;      /* directory for factors.txt / checkpoints (NULL = cwd) */
   uint64_t stop_after_segments; /* test hook: stop (as if killed) after k segments (0 = off) */
+  int verify_device;        /* FS_VERIFY_AUTO (default): factors of a drain are re-verified on the
+                               GPU when they need more than 2^24 multiplications, else on the CPU;
+                               FS_VERIFY_CPU: always on verify_threads CPU threads (the reference);
+                               FS_VERIFY_GPU: always on the GPU (fs_verify_factors_gpu) */
 } fs_options;
+#define FS_VERIFY_AUTO 0
+#define FS_VERIFY_CPU 1
+#define FS_VERIFY_GPU 2
 
 typedef struct {
   uint64_t factorcount;     /* kept factors (prime p) */

@@ -260,6 +260,46 @@ def test_candidate_overflow_recovers(gpu):
     assert (r.factorcount, r.primecount, f"{r.checksum:016X}") == (25, 108529, C1["checksum"])
 
 
+def test_gpu_verifier_matches_cpu(gpu):
+    """fs_verify_factors_gpu against the CPU verifier: the factor lists of the
+    reference self-test ranges 1 and 3 (1,071 factors with n up to 2e6; 42,770
+    with p < 1e5), each alone and with planted wrong factors at several
+    positions, and the edge cases the device leaves to the host (n < 2, n >= p,
+    even p, p < 3) beside large p."""
+    import random
+    lists = []
+    for k in (0, 2):
+        pmin, pmax, nmin, nmax = KATS[k][:4]
+        lists.append(gpu.sieve(pmin, pmax, nmin, nmax, verify_threads=16).factors)
+    assert len(lists[0]) == 1071 and len(lists[1]) == 42770
+    rng = random.Random(11)
+    for fl in lists:
+        assert gpu.verify_factors_gpu(fl) == -1 == gpu.verify_factors(fl, threads=16)
+        for _ in range(4):
+            bad = list(fl)
+            i = rng.randrange(len(bad))
+            p, nc = bad[i]
+            bad[i] = (p, -nc) if rng.random() < 0.5 else (p, nc + (1 if nc > 0 else -1))
+            want = gpu.verify_factors(bad, threads=16)
+            assert want >= 0 and gpu.verify_factors_gpu(bad) == want, (i, bad[i])
+    edge = [(1000003, 2000000), (1000003, -5), (100, 7), (2, 1), (4294967311, 70000), (2 ** 61 - 1, -100000),
+            (2 ** 64 - 59, 3000), (2 ** 64 - 59, -1), (100003, 100002), (100003, -100001)]
+    for c in edge:
+        assert gpu.verify_factors_gpu([c]) == gpu.verify_factors([c], threads=2), c
+    assert gpu.verify_factors_gpu([]) == -1
+
+
+@pytest.mark.parametrize("mode", ["cpu", "gpu", "auto"])
+def test_sieve_verify_device_modes(gpu, mode):
+    """Reference self-test range 1 (1,071 factors, 1.6e9 verification
+    multiplications: auto verifies on the GPU) with each verifier: the same
+    factors, counts and checksum."""
+    pmin, pmax, nmin, nmax, nf, pc, ck = KATS[0]
+    m = {"cpu": gpu.VERIFY_CPU, "gpu": gpu.VERIFY_GPU, "auto": gpu.VERIFY_AUTO}[mode]
+    r = gpu.sieve(pmin, pmax, nmin, nmax, verify_threads=16, verify_device=m)
+    assert (r.factorcount, r.primecount, f"{r.checksum:016X}") == (nf, pc, ck)
+
+
 def test_invalid_ranges_rejected(gpu):
     for args in [(10 ** 8, 10 ** 7, 1000, 2000), (1000, 2000, 1500, 1000), (200, 300, 500, 600)]:
         with pytest.raises(gpu.FactSieveError) as e:

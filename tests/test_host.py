@@ -188,6 +188,12 @@ def test_verify_factor_list_matches_single(fs):
     assert fs.verify_factors([], threads=4) == -1
 
 
+def test_verify_factors_null_list_is_an_error(fs):
+    """A null list with n > 0 must not read as "all pass" (-1)."""
+    assert fs.lib().fs_verify_factors(None, 3, 2) == fs.FS_E_VERIFY
+    assert "null" in fs.last_error()
+
+
 def test_parse_matches_oracle(fs):
     O = ol.oracle()
     strs = ["127", "1e8", "4K", "4M", "1k", "1g", "3b62", "0x7f", "12x", "1e99", "99999999999999999999",
