@@ -39,15 +39,41 @@ static std::vector<uint32_t> primes_upto_uncached(uint32_t limit) {
   std::vector<uint32_t> out;
   if (limit < 2) return out;
   out.push_back(2);
-  const uint64_t nodd = ((uint64_t)limit - 1) / 2;  // odd numbers 3..limit -> index (x-3)/2
-  std::vector<uint64_t> comp((nodd + 64) / 64, 0);
-  for (uint64_t i = 0; i < nodd; ++i) {
-    if (comp[i >> 6] >> (i & 63) & 1) continue;
-    const uint64_t x = 2 * i + 3;
-    out.push_back((uint32_t)x);
-    for (uint64_t y = x * x; y <= limit; y += 2 * x) {
-      const uint64_t j = (y - 3) / 2;
-      comp[j >> 6] |= (uint64_t)1 << (j & 63);
+  if (limit < 3) return out;
+  // base primes up to sqrt(limit), then segments of 2^18 odd numbers (a
+  // 32 KiB bit set, cache resident) marked by each base prime in turn
+  uint32_t r = 1;
+  while ((uint64_t)(r + 1) * (r + 1) <= limit) ++r;
+  std::vector<uint32_t> base;
+  {
+    std::vector<char> c(r + 1, 0);
+    for (uint32_t x = 3; x <= r; x += 2) {
+      if (c[x]) continue;
+      base.push_back(x);
+      for (uint64_t y = (uint64_t)x * x; y <= r; y += 2 * x) c[y] = 1;
+    }
+  }
+  const uint64_t nodd = ((uint64_t)limit - 1) / 2;  // odd numbers 3..limit: index (x-3)/2
+  constexpr uint64_t kSeg = 1u << 18;
+  std::vector<uint64_t> comp(kSeg / 64);
+  std::vector<uint64_t> next(base.size());  // next odd multiple index of each base prime
+  for (size_t k = 0; k < base.size(); ++k) next[k] = ((uint64_t)base[k] * base[k] - 3) / 2;
+  for (uint64_t s0 = 0; s0 < nodd; s0 += kSeg) {
+    const uint64_t s1 = std::min(nodd, s0 + kSeg);
+    std::fill(comp.begin(), comp.end(), 0);
+    for (size_t k = 0; k < base.size(); ++k) {
+      uint64_t j = next[k];
+      for (; j < s1; j += base[k]) comp[(j - s0) >> 6] |= (uint64_t)1 << ((j - s0) & 63);
+      next[k] = j;
+    }
+    for (uint64_t w = 0; w < (s1 - s0 + 63) / 64; ++w) {
+      uint64_t m = ~comp[w];
+      while (m) {
+        const uint64_t i = s0 + w * 64 + __builtin_ctzll(m);
+        m &= m - 1;
+        if (i >= s1) break;
+        out.push_back((uint32_t)(2 * i + 3));
+      }
     }
   }
   return out;
@@ -119,6 +145,10 @@ std::vector<uint64_t> fs_horner_program(uint32_t nmin, uint64_t bound) {
     emax = std::max(emax, ex[i]);
   }
   const int J = 31 - __builtin_clz(emax);
+  // primes of each level j (bit j of the exponent set), ascending, in one pass
+  std::vector<std::vector<uint32_t>> lvl(J + 1);
+  for (size_t i = 0; i < pr.size(); ++i)
+    for (uint32_t e = ex[i]; e; e &= e - 1) lvl[__builtin_ctz(e)].push_back(pr[i]);
   // Each level's primes are packed first-fit-decreasing: a word starts with the
   // largest unused prime and takes the largest unused prime that still fits,
   // until none does.  "Largest unused index <= x" is a union-find over the
@@ -134,9 +164,7 @@ std::vector<uint64_t> fs_horner_program(uint32_t nmin, uint64_t bound) {
   };
   for (int j = J; j >= 0; --j) {
     if (j != J) prog.push_back(0);
-    g.clear();
-    for (size_t i = 0; i < pr.size(); ++i)
-      if ((ex[i] >> j) & 1) g.push_back(pr[i]);
+    g.swap(lvl[j]);
     const uint32_t n = (uint32_t)g.size();
     up.resize(n + 1);
     for (uint32_t k = 0; k <= n; ++k) up[k] = k;
@@ -169,10 +197,28 @@ extern "C" int fs_select_path(uint64_t pmax) {
   return FS_PATH_MONT;
 }
 
+// The setup program of (nmin, path) in one build: its term count, and its
+// words (the layout fs_setup_program writes) when `out` is given.
+static int64_t build_program(uint32_t nmin, int path, std::vector<uint64_t>* out);
+
 extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
+  if (!words) return build_program(nmin, path, nullptr);
+  std::vector<uint64_t> w;
+  const int64_t T = build_program(nmin, path, &w);
+  if (T < 0) return T;
+  // exactly the documented words: T (FP64), T + 2 (HYBRID), 2T (MONT)
+  const size_t len = path == FS_PATH_MONT ? 2 * (size_t)T : path == FS_PATH_HYBRID ? (size_t)T + 2 : (size_t)T;
+  std::copy(w.begin(), w.begin() + len, words);
+  return T;
+}
+
+static int64_t build_program(uint32_t nmin, int path, std::vector<uint64_t>* out) {
+  uint64_t* words = nullptr;
   if (path == FS_PATH_MONT) {
     const int64_t m = fs_power_table(nmin, nullptr, nullptr, nullptr);
-    if (words) {
+    if (out) {
+      out->assign((size_t)(2 * m) + 2, 0);
+      words = out->data();
       std::vector<uint64_t> b(m);
       std::vector<uint32_t> e(m), c(m);
       fs_power_table(nmin, b.data(), e.data(), c.data());
@@ -207,7 +253,9 @@ extern "C" int64_t fs_setup_program(uint32_t nmin, int path, uint64_t* words) {
     pending_square = false;
   }
   if (pending_square) prog.insert(prog.end(), {FS_PROG_SQUARE | 1, 1, 1, 1});
-  if (words) {
+  if (out) {
+    out->assign(prog.size() + 2, 0);
+    words = out->data();
     // HYBRID: powers of 2^-1 the Montgomery products accumulate; E62 with
     // multiplies by 2^-62 (below 2^64/3), E64 with multiplies by 2^-64 (above);
     // squares are by 2^-64 in both
@@ -249,11 +297,9 @@ std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path) {
   }
   try {
     auto pr = std::make_shared<fs_program>();
-    const int64_t T = fs_setup_program(nmin, path, nullptr);
+    const int64_t T = build_program(nmin, path, &pr->words);
     if (T < 0) return nullptr;
     pr->terms = (uint32_t)T;
-    pr->words.assign((size_t)(path == FS_PATH_MONT ? 2 * T : T) + 2, 0);
-    fs_setup_program(nmin, path, pr->words.data());
     std::lock_guard<std::mutex> lk(mu);
     // bounded: 4096 programs or 2^25 words (256 MB); holders keep their own reference
     if (cache.size() >= 4096 || cached_words + pr->words.size() > ((size_t)1 << 25)) {
