@@ -60,7 +60,9 @@ def test_select_path(fs):
     assert fs.select_path(2 ** 64 - 2 ** 46 + 2) == fs.PATH_MONT
 
 
-@pytest.mark.parametrize("nmin", [127, 1000, 10000, 100000, 500000])
+# 524291 = 2 * 2^18 + 3 and 1048579 = 4 * 2^18 + 3 start the second and third
+# 2^18-odd-number segments of the library's prime sieve
+@pytest.mark.parametrize("nmin", [127, 1000, 10000, 100000, 500000, 524289, 524291, 524293, 1048579, 3000017])
 def test_power_table_matches_oracle(fs, nmin):
     O = ol.oracle()
     B = ctypes.POINTER(ctypes.c_uint64)()
