@@ -34,6 +34,7 @@ bounded p-subrange of the same workload, on this host's cores.
 import argparse
 import ctypes
 import json
+import math
 import os
 import sys
 import time
@@ -60,12 +61,27 @@ OPS_PER_STEP_FP64_PAIR = 4.0                    # FP64 paired steps (4 nmax^2 <=
 OPS_PER_STEP_FP64_PAIR_EXACT = 3.0              # paired steps with an exact product (p nmax^2 <= 2^53): 4 x 4 + 8
 
 
-def fp64_ops(pmin, pmax, nmax):
-    """VALU operations per step of the FP64 iterate loop every wave of a job
-    takes (the per-wave choice of fs_cand_iter_fp64, with n1 = nmax)."""
-    if nmax <= 2 ** 22 and 4 * nmax * nmax <= pmin:
-        return OPS_PER_STEP_FP64_PAIR_EXACT if pmax * nmax * nmax <= 2 ** 53 else OPS_PER_STEP_FP64_PAIR
-    return OPS_PER_STEP_FP64_EXACT if pmax * nmax <= 2 ** 53 else OPS_PER_STEP[1]
+def fp64_ops_at(pmin, pmax, n1):
+    """VALU operations per step of the FP64 iterate loop every wave of a launch
+    ending at n1 takes (the per-wave choice of fs_cand_iter_fp64)."""
+    if n1 <= 2 ** 22 and 4 * n1 * n1 <= pmin:
+        return OPS_PER_STEP_FP64_PAIR_EXACT if pmax * n1 * n1 <= 2 ** 53 else OPS_PER_STEP_FP64_PAIR
+    return OPS_PER_STEP_FP64_EXACT if pmax * n1 <= 2 ** 53 else OPS_PER_STEP[1]
+
+
+def fp64_ops(pmin, pmax, nmin, nmax):
+    """Step-weighted VALU operations per step of a job's FP64 iterate launches:
+    the driver ends launches where a cheaper loop stops applying (paired steps
+    up to isqrt(pmin / 4), exact products up to 2^53 / pmax and
+    isqrt(2^53 / pmax)), as for a one-segment job."""
+    top = pmax - 1
+    pair = min(math.isqrt(pmin // 4), 2 ** 22)
+    cuts = sorted({c for c in (pair, min(pair, math.isqrt(2 ** 53 // top)), 2 ** 53 // top) if nmin < c < nmax})
+    total, a = 0.0, nmin
+    for b in cuts + [nmax]:
+        total += (b - a) * fp64_ops_at(pmin, pmax, b)
+        a = b
+    return total / max(nmax - nmin, 1)
 
 
 def hybrid_hi(pmin, pmax, nmax):
@@ -229,7 +245,7 @@ def main():
     if ref.path == 2 and hybrid_hi(pmin, pmax, nmax):
         ops = OPS_PER_STEP_HYBRID_HI
     if ref.path == 1:
-        ops = fp64_ops(pmin, pmax, nmax)
+        ops = fp64_ops(pmin, pmax, nmin, nmax)
     achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
     traffic = None
     if ref.path in TRAFFIC_BYTES_PER_CAND:

@@ -937,6 +937,38 @@ uint32_t fs_auto_segment_candidates() {
   return cus * 4u * 8u * 64u * 8u;
 }
 
+// n values where a cheaper iterate loop stops applying to every modulus of
+// [plo, phi) (the kernels choose their loop per wave by the launch's last n):
+// a launch that ends at such a point runs the n below it with that loop.
+// FP64: paired steps while 4 n1^2 <= p and n1 <= 2^22 (fs_fp_pair_ok, smallest
+// p), paired with the exact product while p n1^2 <= 2^53, single steps with the
+// exact product while p n1 <= 2^53 (largest p); hybrid: the high-word quotient
+// while p >> 39 >= n1 (fs_sh_hi_ok).
+static std::vector<uint32_t> loop_boundaries(int path, uint64_t plo, uint64_t phi, uint32_t nmin, uint32_t nmax) {
+  std::vector<uint32_t> b;
+  auto add = [&](uint64_t n) {
+    if (n > nmin && n < nmax) b.push_back((uint32_t)n);
+  };
+  auto isqrt = [](uint64_t x) {
+    uint64_t r = (uint64_t)sqrt((double)x);
+    while (r * r > x) --r;
+    while ((r + 1) * (r + 1) <= x) ++r;
+    return r;
+  };
+  const uint64_t ptop = phi - 1;  // largest modulus of the segment
+  if (path == FS_PATH_FP64 && ptop > 0) {
+    const uint64_t pair = std::min<uint64_t>(isqrt(plo / 4), (uint64_t)1 << 22);
+    add(pair);
+    add(std::min<uint64_t>(pair, isqrt(((uint64_t)1 << 53) / ptop)));
+    add(((uint64_t)1 << 53) / ptop);
+  } else if (path == FS_PATH_HYBRID && ptop < FS_SH_HI_PMAX) {
+    add(plo >> FS_SH_HI_SHIFT);
+  }
+  std::sort(b.begin(), b.end());
+  b.erase(std::unique(b.begin(), b.end()), b.end());
+  return b;
+}
+
 // Time the events of the last segment's kernels into the result counters.
 static void account(DevBufs& d, fs_result* r) {
   float ms = 0;
@@ -1131,8 +1163,12 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
 
     // iterate over n in chunks, then fold the checksum
     if ((rc = fs_event_record(d.ev[4], d.s))) return rc;
+    const std::vector<uint32_t> cuts = loop_boundaries(path, st.p, stop, nmin, nmax);
+    size_t ci = 0;
     for (uint32_t n0 = nmin; n0 < nmax;) {
-      const uint32_t n1 = (nmax - n0 > nchunk_max) ? n0 + nchunk_max : nmax;
+      uint32_t n1 = (nmax - n0 > nchunk_max) ? n0 + nchunk_max : nmax;
+      while (ci < cuts.size() && cuts[ci] <= n0) ++ci;
+      if (ci < cuts.size() && cuts[ci] < n1) n1 = cuts[ci];
       if ((rc = fs_iterate(&d.c, d.ctr, d.fac, d.fcap, n0, n1, path, d.s))) return rc;
       ++r.iterate_launches;
       n0 = n1;
