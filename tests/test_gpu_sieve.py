@@ -243,6 +243,34 @@ def test_edge_ranges_match_oracle(gpu, args):
     assert r.factors == o["factors"]
 
 
+def random_jobs(seed, count):
+    import random
+    rng = random.Random(seed)
+    jobs = []
+    for _ in range(count):
+        k = rng.randrange(17, 64)
+        lo = rng.randrange(2 ** k, min(2 ** (k + 1), 2 ** 64 - 300000))
+        n0 = rng.randrange(127, 20000)
+        n1 = n0 + rng.randrange(1, 3000)
+        if lo < n1:
+            lo = n1 + 1
+        jobs.append((lo, lo + rng.randrange(1000, 200000), n0, n1))
+    return jobs
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_random_jobs_match_oracle(gpu, seed):
+    """Seeded random jobs over every modulus size from 2^17 to 2^64 (every
+    arithmetic path and iterate loop, and the launch cuts between loops)
+    against the oracle."""
+    for args in random_jobs(seed, 12):
+        print("random", args, flush=True)
+        r = gpu.sieve(*args)
+        o = ol.sieve(*args, threads=8)
+        assert (r.factorcount, r.primecount, r.checksum) == (o["factorcount"], o["primecount"], o["checksum"]), args
+        assert r.factors == o["factors"]
+
+
 @pytest.mark.parametrize("flag,code", [(3, -6), (5, -5), (6, -4)])
 def test_fault_injection_is_reported(gpu, flag, code):
     gpu.check(gpu.lib().fs_inject_fault(flag))
