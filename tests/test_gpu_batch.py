@@ -5,6 +5,7 @@ graph capture or admission schedule."""
 import pytest
 
 import oracle_lib as ol
+from jobs_lib import random_jobs
 from factsieve_amd import workloads
 
 pytestmark = pytest.mark.gpu
@@ -87,6 +88,20 @@ def test_batch_paired_steps_match_oracle(gpu, opts):
     jobs = pair_mix()
     print("options", opts, flush=True)
     res, _ = gpu.batch(jobs, **opts)
+    for job, o in zip(jobs, res):
+        want = ol.sieve(*job, threads=8)
+        assert o.status == 0
+        assert (o.factorcount, o.primecount, o.checksum) == (want["factorcount"], want["primecount"],
+                                                             want["checksum"]), job
+        assert o.factors == want["factors"]
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_batch_random_jobs_match_oracle(gpu, seed):
+    """Seeded random jobs over every modulus size (every path and loop) in one
+    batch, each against the oracle."""
+    jobs = random_jobs(seed, 16)
+    res, _ = gpu.batch(jobs)
     for job, o in zip(jobs, res):
         want = ol.sieve(*job, threads=8)
         assert o.status == 0

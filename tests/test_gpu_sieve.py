@@ -9,6 +9,7 @@ import subprocess
 import pytest
 
 import oracle_lib as ol
+from jobs_lib import random_jobs
 
 pytestmark = pytest.mark.gpu
 
@@ -241,21 +242,6 @@ def test_edge_ranges_match_oracle(gpu, args):
     o = ol.sieve(*args, threads=8)
     assert (r.factorcount, r.primecount, r.checksum) == (o["factorcount"], o["primecount"], o["checksum"])
     assert r.factors == o["factors"]
-
-
-def random_jobs(seed, count):
-    import random
-    rng = random.Random(seed)
-    jobs = []
-    for _ in range(count):
-        k = rng.randrange(17, 64)
-        lo = rng.randrange(2 ** k, min(2 ** (k + 1), 2 ** 64 - 300000))
-        n0 = rng.randrange(127, 20000)
-        n1 = n0 + rng.randrange(1, 3000)
-        if lo < n1:
-            lo = n1 + 1
-        jobs.append((lo, lo + rng.randrange(1000, 200000), n0, n1))
-    return jobs
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
