@@ -497,7 +497,15 @@ def test_cli_rejects_bad_arguments(fs):
     r = subprocess.run([exe, "-n", "99"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "out of range argument -n 99" in r.stdout
     r = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=60)
-    assert r.returncode != 0 and "-N #" in r.stdout
+    assert r.returncode != 0 and "-N #" in r.stdout and "--verify-on" in r.stdout
+    r = subprocess.run([exe, "--verify-on", "fpga", "-n", "1000"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "invalid argument --verify-on fpga" in r.stdout
+
+
+def test_default_options(fs):
+    o = fs.default_options()
+    assert (o.path, o.verify_threads, o.verify_device, o.checkpoint_secs) == (fs.PATH_AUTO, 2, fs.VERIFY_AUTO, 60)
+    assert (o.write_files, o.resume, o.quiet, o.seg_candidates, o.n_chunk) == (0, 0, 1, 0, 0)
 
 
 # ---------------------------------------------------------------- checkpoints
