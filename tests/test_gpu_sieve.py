@@ -301,6 +301,11 @@ def test_gpu_verifier_matches_cpu(gpu):
     for c in edge:
         assert gpu.verify_factors_gpu([c]) == gpu.verify_factors([c], threads=2), c
     assert gpu.verify_factors_gpu([]) == -1
+    # Wilson's theorem at the largest n: (p-1)! = -1 and (p-2)! = +1 (mod p) for
+    # p = 2^31 - 1, 524,288 ranges of 4,096 multipliers each
+    w = 2 ** 31 - 1
+    assert gpu.verify_factors_gpu([(w, w - 1), (w, -(w - 2))]) == -1
+    assert gpu.verify_factors_gpu([(w, w - 1), (w, w - 2)]) == 1
 
 
 @pytest.mark.parametrize("mode", ["cpu", "gpu", "auto"])
