@@ -20,8 +20,9 @@ p nmax^2 <= 2^53 (exact product: 2 multiplies, 1 FMA, 1 round); otherwise 6.5
 with single steps (one compare per two steps: |v[k+1]| <= m[k] covers +-1 at k
 and k+1); 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
 2 v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, and 6 filter ops per 8 steps), 9.625
-when every modulus is at least nmax * 2^39 and the quotient comes from the
-high word of the residue alone (4 FP64 ops instead of 6), and 4.5 on the FP64
+when every modulus is at least nmax * 2^34 (below 2^63; nmax * 2^39 above)
+and the quotient comes from the high word of the residue alone (4 FP64 ops
+instead of 6), and 4.5 on the FP64
 path with single steps when p * nmax <= 2^53 and the product s*m is exact;
 peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 lane-ops/s (the FP64
 vector peak of 78.6 TFLOP/s counts each FMA lane-op as two flops).
@@ -86,8 +87,15 @@ def fp64_ops(pmin, pmax, nmin, nmax):
 
 def hybrid_hi(pmin, pmax, nmax):
     """The hybrid iterate takes its quotient from the high word of the residue
-    when every modulus p satisfies p >= nmax * 2^39 and p < 2^64 - 2^58."""
-    return (pmin >> 39) >= nmax and pmax <= 2 ** 64 - 2 ** 58
+    when every modulus p satisfies p >= nmax * 2^34 and p < 2^63, or
+    p >= nmax * 2^39 and p < 2^64 - 2^58."""
+    def mmax(p):
+        if p >= 2 ** 64 - 2 ** 58:
+            return 0
+        return p >> 34 if p < 2 ** 63 else p >> 39
+    top = pmax - 1
+    lowest = mmax(pmin) if top < 2 ** 63 else min(mmax(pmin), mmax(max(pmin, 2 ** 63)), mmax(top))
+    return lowest >= nmax
 INT32_MUL_PER_STEP = 11                         # 64-bit Montgomery step: 4 + 3 + 4 partial products
 INT32_MUL_PEAK = VALU_PEAK_TOPS * 1e12          # full-rate v_mad_u64_u32 / v_mul_*_u32 (measured)
 # HBM bytes per candidate per iterate launch, measured with rocprofv3 --pmc FETCH_SIZE

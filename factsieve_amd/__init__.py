@@ -311,8 +311,8 @@ def hostemu_fp_exact(p, s, m):
 
 
 def hostemu_mulmod_hi(p, s, m):
-    """One hybrid residue step with the high-word quotient (p >= m * 2^39) on the
-    host: (raw signed value, canonical)."""
+    """One hybrid residue step with the high-word quotient (p >= m * 2^34 below
+    2^63, p >= m * 2^39 above) on the host: (raw signed value, canonical)."""
     raw, res = i64(), u64()
     check(lib().fs_hostemu_mulmod_hi(p, s, m, ctypes.byref(raw), ctypes.byref(res)))
     return raw.value, res.value

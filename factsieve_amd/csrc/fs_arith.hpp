@@ -195,21 +195,30 @@ FS_HD int64_t fs_sh_mulmod(int64_t s, uint32_t m, double md, uint64_t c31, uint6
 }
 
 // The signed step with the quotient from the high word of s only, for moduli
-// far above the multiplier: p >= m * 2^FS_SH_HI_SHIFT and p < FS_SH_HI_PMAX.
-// y = s1 * 2^32/p misses s/p by s0/p < 2^32/p, so y*m misses X = s*m/p by
-// less than 2^32 m / p <= 2^-7 plus the rounding terms of fs_sh_mulmod
-// (< 2^-21 for m < 2^25), and r = s*m - (k - 2^31) p has
-// |r| <= p/2 + 2^32 m + 2^-21 p <= (1/2 + 2^-7 + 2^-21) p: below 0.51 p (the
-// step's input bound) and, for p < 2^64 - 2^58, below 2^63 -- the same
-// invariant with two FP64 operations fewer (no conversion of s0, no FMA).
+// far above the multiplier (fs_sh_hi_ok).  y = s1 * 2^32/p misses s/p by
+// s0/p in [0, 2^32/p), so y*m falls short of X = s*m/p by at most
+// 2^32 m / p, and the rounding terms of fs_sh_mulmod add less than 2^-21
+// (m < 2^29): r = s*m - (k - 2^31) p has |r| <= (1/2 + 2^32 m/p + 2^-21) p.
+// Two regimes keep r an exact int64 with |r| < p - 1, whatever the input
+// |s| <= 0.76 p:
+//   p < 2^63 and p >= m * 2^34:  |r| <= (3/4 + 2^-21) p < 2^63;
+//   p < FS_SH_HI_PMAX and p >= m * 2^39:  |r| <= (1/2 + 2^-7 + 2^-21) p.
+// Two FP64 operations fewer than fs_sh_mulmod (no conversion of s0, no FMA).
 // *yq = y is a filter for s == +-1: s in {-1, 0, 1} has s1 in {-1, 0}, so
 // |y| <= 1.5 * pinv32 holds for it (and also for other |s| < 2^33, which the
-// exact test then rejects).
+// exact test then rejects).  A residue +-1 times m <= p/2^34 gets k = 2^31
+// (|y*m| <= 1/4), so the step leaves exactly +-m, as the paired test needs.
 #define FS_SH_HI_SHIFT 39
+#define FS_SH_HI_SHIFT_LOW 34
 #define FS_SH_HI_PMAX (0xFFFFFFFFFFFFFFFFull - ((uint64_t)1 << 58) + 1)
-FS_HD bool fs_sh_hi_ok(uint64_t p, uint32_t mmax) {
-  return p < FS_SH_HI_PMAX && (p >> FS_SH_HI_SHIFT) >= (uint64_t)mmax;
+// Largest multiplier the high-word step takes for p (0 when it takes none).
+FS_HD uint64_t fs_sh_hi_mmax(uint64_t p) {
+  if (p >= FS_SH_HI_PMAX) return 0;
+  return p < ((uint64_t)1 << 63) ? p >> FS_SH_HI_SHIFT_LOW : p >> FS_SH_HI_SHIFT;
 }
+// Bound on |r| / p of the high-word step, and so on the residues of its loop.
+#define FS_SH_HI_RBOUND 0.7500005
+FS_HD bool fs_sh_hi_ok(uint64_t p, uint32_t mmax) { return fs_sh_hi_mmax(p) >= (uint64_t)mmax; }
 
 FS_HD int64_t fs_sh_mulmod_hi(int64_t s, uint32_t m, double md, uint64_t c31, uint64_t pn, double pinv32,
                               double* yq) {

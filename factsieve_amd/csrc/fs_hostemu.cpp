@@ -53,7 +53,7 @@ extern "C" int fs_hostemu_fp_exact(uint64_t p, int64_t s, uint32_t m, int64_t* r
 extern "C" int fs_hostemu_mulmod_hi(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res) {
   if ((p & 1) == 0 || p > FS_HYBRID_PMAX || m == 0 || !fs_sh_hi_ok(p, m) || !raw || !res) return FS_E_ARG;
   const uint64_t as = s < 0 ? (uint64_t)0 - (uint64_t)s : (uint64_t)s;
-  if ((double)as > 0.51 * (double)p) return FS_E_ARG;
+  if ((double)as > FS_SH_HI_RBOUND * (double)p) return FS_E_ARG;
   const double pinv32 = (1.0 / (double)p) * 4294967296.0;
   double y;
   const int64_t r = fs_sh_mulmod_hi(s, m, (double)m, p << 31, 0 - p, pinv32, &y);
@@ -274,9 +274,14 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     if (p > FS_HYBRID_PMAX) return FS_E_ARG;
     const bool hi = fs_sh_hi_ok(p, n1);  // the device loops' choice (per wave there)
     const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+    const double lim = (hi ? FS_SH_HI_RBOUND : 0.51) * (double)p;  // the loop's residue bound
+    bool out = false;
     auto step = [&](int64_t s, uint32_t m, double* y) {
-      return hi ? fs_sh_mulmod_hi(s, m, (double)m, p << 31, 0 - p, pinv32, y)
-                : fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32, y);
+      const int64_t r = hi ? fs_sh_mulmod_hi(s, m, (double)m, p << 31, 0 - p, pinv32, y)
+                           : fs_sh_mulmod(s, m, (double)m, p << 31, 0 - p, pinv, pinv32, y);
+      // (the bound assumes m < 2p, which the sieve's launches meet: its residue is 0 for n >= p)
+      out |= (uint64_t)m < 2 * p && __builtin_fabs((double)r) > lim;
+      return r;
     };
     int64_t s = fs_canon_to_signed(*res, p);
     uint32_t n = n0;
@@ -300,6 +305,7 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
       double y;
       s = step(s, n + 1, &y);
     }
+    if (out) return FS_E_VALIDATE;  // a residue left the bound the step's next input needs
     *res = fs_signed_to_canon(s, p);
   } else if (path == FS_PATH_MONT) {
     const uint64_t nmo = p - one;

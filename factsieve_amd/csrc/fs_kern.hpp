@@ -493,14 +493,15 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64(uint64_t p, uint64_t s_in,
   return fs_cand_iter_fp64_loop<U, false>(p, s_in, n0, n1, nt, rec);
 }
 
-// Hybrid path (p <= FS_HYBRID_PMAX): signed residue |s| <= 0.51 p, exact
-// 64x32 products mod 2^64 with an FP64 quotient; nt as for the FP64 path.
+// Hybrid path (p <= FS_HYBRID_PMAX): signed residue |s| <= 0.51 p (0.76 p
+// with the high-word quotient), exact 64x32 products mod 2^64 with an FP64
+// quotient; nt as for the FP64 path.
 // n! = +-1 (mod p) iff s == +-1.  U steps share one test on the quotient
 // estimates y[k] ~ v[k]/p that the steps compute anyway, taken at the odd
 // positions (two residues each, as above): |y| <= 1.5 * m[U-2] / p holds for
 // every |v| <= m[U-2] (the estimate's error is below 2^-50 |v|/p + 2^-21/p),
 // and the branch then tests every residue exactly.  HI: the quotient from the
-// high word of s (fs_sh_mulmod_hi, for p >= n1 * 2^39), whose estimates pass
+// high word of s (fs_sh_mulmod_hi, when fs_sh_hi_ok(p, n1)), whose estimates pass
 // |y| <= 1.5 * 2^32/p for every |v| < 2^33, so for every |v| <= m < 2^31.
 template <int U, bool HI, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,

@@ -213,9 +213,10 @@ int fs_hostemu_mulmod(int path, uint64_t p, int64_t s, uint64_t m, int64_t* raw,
  * (s*m exact): |s| <= 0.51p, 1 <= m, p * m <= 2^53, odd p >= 7 (FS_E_ARG
  * otherwise); |raw| <= 0.51p. */
 int fs_hostemu_fp_exact(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res);
-/* The HYBRID step with the quotient from the high word of s, used when
- * p >= n1 * 2^39 (for every multiplier m <= n1 of a launch): |s| <= 0.51p,
- * 1 <= m, p >> 39 >= m, odd p <= 2^64 - 2^46 (FS_E_ARG otherwise); |raw| <= 0.51p. */
+/* The HYBRID step with the quotient from the high word of s, used when every
+ * multiplier m <= n1 of a launch has p >= m * 2^34 and p < 2^63, or
+ * p >= m * 2^39 and p < 2^64 - 2^58: |s| <= 0.7500005p, 1 <= m, odd p (FS_E_ARG
+ * otherwise); |raw| <= 0.7500005p, and <= 0.51p in the second case. */
 int fs_hostemu_mulmod_hi(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res);
 /* The FP64 paired step, one product by a*(a+1) for the multipliers a and a+1,
  * used when 4 * n1^2 <= p and n1 <= 2^22 (for every pair a+1 <= n1 of a launch;

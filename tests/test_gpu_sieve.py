@@ -232,7 +232,9 @@ def test_paired_steps_chunked_match_oracle(gpu, n_chunk):
     (4 * 10 ** 6, 6 * 10 ** 6, 10, 1000),                # paired steps with the exact product, ~30 factors
     (1600000000 - 100000, 1600000000 + 100000, 19500, 20000),  # 4 n1^2 = 1.6e9 inside: paired and single waves
     (360287970 - 100000, 360287970 + 100000, 4000, 5000),  # p * n1^2 crosses 2^53: exact and inexact pairs
-    ((16385 << 39) - 100000, (16385 << 39) + 100000, 16000, 16385),   # hybrid: p >> 39 crosses n1
+    ((16385 << 39) - 100000, (16385 << 39) + 100000, 16000, 16385),   # hybrid just above 2^53: high-word loop
+    ((524300 << 34) - 100000, (524300 << 34) + 100000, 523800, 524300),  # p >> 34 crosses n1
+    (2 ** 63 - 3000, 2 ** 63 + 3000, 2 ** 24 - 40, 2 ** 24 + 40),  # high-word limit drops from p >> 34 to p >> 39
     (2 ** 64 - 2 ** 58 - 100000, 2 ** 64 - 2 ** 58 + 100000, 3000, 3500),  # top of the high-word quotient
     ((2 ** 64 - 1) // 3 - 100000, (2 ** 64 - 1) // 3 + 100000, 3000, 3500),  # setup: two-stage REDC / REDC64
 ])

@@ -384,18 +384,27 @@ def test_fp64_pair_loop_matches_oracle(fs):
                 assert (r, facs[:nf]) == (want_r, want_f), (p, n0, n1, start)
 
 
+def hi_mmax(p):
+    """Largest multiplier the hybrid high-word step takes for p: p >> 34 below
+    2^63 (|s'| <= 3/4 p + 2^-21 p), p >> 39 up to 2^64 - 2^58 (|s'| <= 0.51p)."""
+    if p >= 2 ** 64 - 2 ** 58:
+        return 0
+    return p >> 34 if p < 2 ** 63 else p >> 39
+
+
 def test_hybrid_hi_step_at_error_bound_edges(fs):
-    """The hybrid step with the quotient from the high word of s only (used
-    when p >= n1 * 2^39): exact, and |s'| stays within 0.51p, at the largest
-    multiplier the bound allows, |s| up to 0.51p, and the largest dropped low
-    word s0 = 2^32 - 1."""
+    """The hybrid step with the quotient from the high word of s only: exact,
+    and |s'| stays within its bound (3/4 p + 2^-21 p below 2^63, 0.51p above),
+    at the largest multiplier each regime allows, |s| up to that bound, and
+    the largest dropped low word s0 = 2^32 - 1."""
     rng = random.Random(23)
     top = 2 ** 64 - 2 ** 58          # first modulus the high-word step does not take
-    ps = [top - 1, top - 3, 2 ** 63 + 29, 2 ** 63 - 25, 2 ** 62 + 135, 2 ** 58 + 27, 2 ** 50 + 55,
-          (2 ** 19 << 39) + 1]
+    ps = [top - 1, top - 3, 2 ** 63 + 29, 2 ** 63 - 25, 2 ** 62 + 135, 2 ** 58 + 27, 2 ** 53 + 5, 2 ** 50 + 55,
+          (2 ** 19 << 34) + 1, (2 ** 19 << 39) + 1, (524300 << 34) + 7]
     for p in ps:
-        mtop = min(p >> 39, 2 ** 31 - 1)
-        smax = min(int(0.51 * p) - 2 ** 12, 2 ** 63 - 1)
+        bound = 0.75 + 2 ** -21 if p < 2 ** 63 else 0.51
+        mtop = min(hi_mmax(p), 2 ** 31 - 1)
+        smax = min(int(bound * p) - 2 ** 12, 2 ** 63 - 1)
         ms = [mtop, mtop - 1, max(1, mtop // 2), 1] + [rng.randrange(1, mtop + 1) for _ in range(150)]
         s0max = (((smax >> 32) - 1) << 32) | 0xFFFFFFFF    # largest dropped low word, |s| <= smax
         ss = [smax, -smax, 1, -1, 0, s0max, -s0max, -(((smax >> 32) - 1) << 32)] + \
@@ -404,9 +413,13 @@ def test_hybrid_hi_step_at_error_bound_edges(fs):
             for m in ms[:4] + [ms[4 + (s % 150)]]:
                 raw, res = fs.hostemu_mulmod_hi(p, s, m)
                 assert res == (s * m) % p, (p, s, m)
-                assert raw % p == res and abs(raw) <= 0.51 * p, (p, s, m, raw)
+                assert raw % p == res and abs(raw) <= bound * p, (p, s, m, raw)
+                if abs(s) == 1:
+                    assert raw == s * m, (p, s, m, raw)   # +-1 times m stays +-m (paired test)
     with pytest.raises(fs.FactSieveError):
-        fs.hostemu_mulmod_hi(2 ** 58 + 27, 5, (2 ** 58 + 27 >> 39) + 1)   # multiplier above p / 2^39
+        fs.hostemu_mulmod_hi(2 ** 58 + 27, 5, ((2 ** 58 + 27) >> 34) + 1)   # multiplier above p / 2^34
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_mulmod_hi(2 ** 63 + 29, 5, ((2 ** 63 + 29) >> 39) + 1)   # above 2^63: p / 2^39
     with pytest.raises(fs.FactSieveError):
         fs.hostemu_mulmod_hi(top + 1, 5, 3)                                # |s'| could reach 2^63
 
@@ -414,18 +427,19 @@ def test_hybrid_hi_step_at_error_bound_edges(fs):
 def test_hybrid_hi_filter_sound(fs):
     """With the high-word quotient the +-1 filter is |y| <= 1.5 * 2^32/p; a
     residue of +-1 must never fail it (the host execution returns an error if
-    it does).  n-ranges chosen so that p >= n1 * 2^39 selects that loop."""
+    it does, or if a residue leaves the loop's bound).  n-ranges chosen so that
+    the high-word loop is selected."""
     rng = random.Random(29)
-    ps = [2 ** 64 - 2 ** 58 - 1, 2 ** 63 + 29, 2 ** 62 + 135, 2 ** 58 + 27, 2 ** 45 + 59]
+    ps = [2 ** 64 - 2 ** 58 - 1, 2 ** 63 + 29, 2 ** 63 - 25, 2 ** 62 + 135, 2 ** 58 + 27, 2 ** 53 + 5, 2 ** 45 + 59]
     ps += [rng.randrange(2 ** 45, 2 ** 64 - 2 ** 58) | 1 for _ in range(200)]
     for p in ps:
-        top = min(p >> 39, 2 ** 31 - 1)
+        top = min(hi_mmax(p), 2 ** 31 - 1)
         for start in (1, p - 1):
-            n0 = rng.randrange(1, max(2, top - 4))
-            if n0 + 3 > top:
-                continue
-            _, facs, nf = fs.hostemu_iterate(fs.PATH_HYBRID, p, n0, n0 + 3, start)
-            assert nf >= 1 and facs[0] == (p, (-1 if start == 1 else 1) * n0), p
+            for n0 in (rng.randrange(1, max(2, top - 4)), top - 3):
+                if n0 < 1 or n0 + 3 > top:
+                    continue
+                _, facs, nf = fs.hostemu_iterate(fs.PATH_HYBRID, p, n0, n0 + 3, start)
+                assert nf >= 1 and facs[0] == (p, (-1 if start == 1 else 1) * n0), p
 
 
 @pytest.mark.parametrize("path", [1, 2])
@@ -443,8 +457,11 @@ def test_block_filter_catches_every_position(fs, path):
     else:
         ps = [2 ** 53 + 5, 2 ** 58 + 27, 2 ** 62 + 135, 2 ** 63 + 29, 2 ** 64 - 2 ** 58 - 1, 2 ** 64 - 2 ** 46 - 3]
         ps += [rng.randrange(2 ** 53, 2 ** 64 - 2 ** 46) | 1 for _ in range(40)]
+    # path 2: n0 = 700000 and 2^24 + 100 keep the full quotient below
+    # p = 700000 * 2^34 and 2^58 (the high-word loop needs n1 <= p >> 34)
+    n0s = (1000, 700000, 2 ** 24 + 100) if path == 2 else (1000,)
     for p in ps:
-        for n0 in (1000, rng.randrange(2000, 400000)):
+        for n0 in n0s + (rng.randrange(2000, 400000),):
             n1 = n0 + 8 * 3 + rng.randrange(0, 8)
             if n1 >= p:
                 continue
