@@ -176,11 +176,26 @@ def test_validation_flags_iterate_out_of_order(gpu):
 
 
 def test_paths_agree_at_full_size_c2(gpu):
-    """Full C2 (3.6e12 p*n steps): the FP64 and hybrid formulations are
-    independent arithmetic; their checksums and factor lists must agree."""
-    a = gpu.sieve(10 ** 12, 10 ** 12 + 10 ** 9, 10 ** 5, 2 * 10 ** 5, verify_threads=16, path=gpu.PATH_FP64)
-    b = gpu.sieve(10 ** 12, 10 ** 12 + 10 ** 9, 10 ** 5, 2 * 10 ** 5, verify_threads=16, path=gpu.PATH_HYBRID)
-    assert a.primecount == b.primecount == 36191004
+    """Full C2 (3.6e12 p*n steps): the FP64 (paired steps), hybrid and
+    Montgomery (the reference's own formulation) paths are independent
+    arithmetic; their checksums and factor lists must agree, and every factor
+    is re-verified by the sieve's own verifier."""
+    args = (10 ** 12, 10 ** 12 + 10 ** 9, 10 ** 5, 2 * 10 ** 5)
+    a = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_FP64)
+    b = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_HYBRID)
+    c = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_MONT)
+    assert a.primecount == b.primecount == c.primecount == 36191004
+    assert (a.checksum, a.factors) == (b.checksum, b.factors) == (c.checksum, c.factors)
+    assert a.checksum == 0xF67C312116703E89 and a.factorcount == 7
+
+
+def test_paths_agree_at_full_size_c3(gpu):
+    """Full C3 (p in [2^62, 2^62 + 2^30), n in [5e5, 5.1e5)): the hybrid path
+    (high-word quotient, two-stage REDC setup) against the Montgomery path."""
+    args = (2 ** 62, 2 ** 62 + 2 ** 30, 500000, 510000)
+    a = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_HYBRID)
+    b = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_MONT)
+    assert a.primecount == b.primecount == 24983780
     assert (a.checksum, a.factors) == (b.checksum, b.factors)
 
 
