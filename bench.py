@@ -13,7 +13,7 @@ jobs.  One JSON line is printed.
 roofline: the dominant kernel is the residue iteration.  Its work is counted
 in VALU lane-operations per p*n step of the compiled 8-step loop, timed with
 HIP events on the launch stream.  On the FP64 path (p < 2^53): 4.0 with paired
-steps, used when 4 nmax^2 <= p (one 6-operation product by (n+1)(n+2) per two
+steps, used when 128 nmax^2 <= 63 p (one 6-operation product by (n+1)(n+2) per two
 steps -- 2 multiplies, 2 FMAs, 1 add, 1 round -- and two exact compares,
 |v| == (n+1)(n+2) and |v| == n+2, which find +-1 at n and n+1), 3.0 when also
 p nmax^2 <= 2^53 (exact product: 2 multiplies, 1 FMA, 1 round); otherwise 6.5
@@ -58,14 +58,14 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine
 OPS_PER_STEP = {1: 6.5, 2: 11.75, 3: 45}
 OPS_PER_STEP_HYBRID_HI = 9.625                  # hybrid with the high-word quotient: 77 per 8 steps
 OPS_PER_STEP_FP64_EXACT = 4.5                   # FP64 with an exact product (p * nmax <= 2^53): 36 per 8
-OPS_PER_STEP_FP64_PAIR = 4.0                    # FP64 paired steps (4 nmax^2 <= p): 4 x 6 + 8 compares per 8
+OPS_PER_STEP_FP64_PAIR = 4.0                    # FP64 paired steps (128 nmax^2 <= 63 p): 4 x 6 + 8 compares per 8
 OPS_PER_STEP_FP64_PAIR_EXACT = 3.0              # paired steps with an exact product (p nmax^2 <= 2^53): 4 x 4 + 8
 
 
 def fp64_ops_at(pmin, pmax, n1):
     """VALU operations per step of the FP64 iterate loop every wave of a launch
     ending at n1 takes (the per-wave choice of fs_cand_iter_fp64)."""
-    if n1 <= 2 ** 22 and 4 * n1 * n1 <= pmin:
+    if n1 <= 2 ** 22 and 128 * n1 * n1 <= 63 * pmin:
         return OPS_PER_STEP_FP64_PAIR_EXACT if pmax * n1 * n1 <= 2 ** 53 else OPS_PER_STEP_FP64_PAIR
     return OPS_PER_STEP_FP64_EXACT if pmax * n1 <= 2 ** 53 else OPS_PER_STEP[1]
 
@@ -73,10 +73,10 @@ def fp64_ops_at(pmin, pmax, n1):
 def fp64_ops(pmin, pmax, nmin, nmax):
     """Step-weighted VALU operations per step of a job's FP64 iterate launches:
     the driver ends launches where a cheaper loop stops applying (paired steps
-    up to isqrt(pmin / 4), exact products up to 2^53 / pmax and
+    up to isqrt(63 pmin / 128), exact products up to 2^53 / pmax and
     isqrt(2^53 / pmax)), as for a one-segment job."""
     top = pmax - 1
-    pair = min(math.isqrt(pmin // 4), 2 ** 22)
+    pair = min(math.isqrt(63 * pmin // 128), 2 ** 22)
     cuts = sorted({c for c in (pair, min(pair, math.isqrt(2 ** 53 // top)), 2 ** 53 // top) if nmin < c < nmax})
     total, a = 0.0, nmin
     for b in cuts + [nmax]:

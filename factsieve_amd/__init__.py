@@ -319,7 +319,7 @@ def hostemu_mulmod_hi(p, s, m):
 
 
 def hostemu_fp_pair(p, s, a):
-    """One FP64 paired step s*a*(a+1) mod p (4 (a+1)^2 <= p) on the host:
+    """One FP64 paired step s*a*(a+1) mod p (128 (a+1)^2 <= 63 p) on the host:
     (raw signed value, canonical, whether the exact-product form ran)."""
     raw, res, ex = i64(), u64(), ctypes.c_int()
     check(lib().fs_hostemu_fp_pair(p, s, a, ctypes.byref(raw), ctypes.byref(res), ctypes.byref(ex)))

@@ -403,7 +403,7 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t 
 // pt[2j+1] = n0+2+2j, read with scalar loads, with at least U entries (U/2
 // pairs) past index n1 - n0.  With v[k] the residue of n0 + k and M, a+1 the pair's values,
 // v[k+2] = v[k] * M and v[k+2] = v[k+1] * (a+1), so, both values being below
-// p/4 and every residue of magnitude <= 0.51 p, |v[k+2]| == M exactly when
+// 0.4922 p and every residue of magnitude <= 0.503 p, |v[k+2]| == M exactly when
 // v[k] = +-1 and |v[k+2]| == a+1 exactly when v[k+1] = +-1.  Per two steps:
 // one six-operation product (four when EXACT) and two compares.  The taken
 // branch (a +-1 residue in the block) recomputes the odd positions with single

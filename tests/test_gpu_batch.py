@@ -69,13 +69,15 @@ def test_batch_shape_invariance(gpu, opts):
 
 
 def pair_mix():
-    # FP64 jobs with 4 nmax^2 <= p (paired steps): with and without the exact
-    # product, odd and even first multipliers, one straddling 4 nmax^2 = p
+    # FP64 jobs with 128 nmax^2 <= 63 p (paired steps): with and without the
+    # exact product, odd and even first multipliers, one straddling
+    # 128 nmax^2 = 63 p
     return [
         (4 * 10 ** 6, 4 * 10 ** 6 + 400000, 10, 1000),           # exact pairs, factors present
         (4 * 10 ** 6 + 400000, 4 * 10 ** 6 + 800000, 11, 997),   # odd nmin and nmax
         (10 ** 10, 10 ** 10 + 200000, 20001, 49999),             # inexact pairs
-        (1600000000 - 100000, 1600000000 + 100000, 19500, 20000),  # paired and single waves
+        (1600000000 - 100000, 1600000000 + 100000, 19500, 20000),  # all paired
+        (812698413 - 100000, 812698413 + 100000, 19500, 20000),  # paired and single waves
     ]
 
 

@@ -242,10 +242,11 @@ def test_paired_steps_chunked_match_oracle(gpu, n_chunk):
     (2 ** 63 - 100000, 2 ** 63 + 100000, 3000, 3500),
     (2 ** 64 - 2 ** 46 - 100000, 2 ** 64 - 2 ** 46 + 1, 3000, 3500),   # top of the hybrid range
     (2 ** 64 - 2 ** 46 - 50000, 2 ** 64 - 2 ** 46 + 50000, 3000, 3500),  # straddles it: Montgomery
-    (2 ** 40 - 100000, 2 ** 40 + 100000, 8000, 8193),   # FP64 paired steps (4 n1^2 <= p)
+    (2 ** 40 - 100000, 2 ** 40 + 100000, 8000, 8193),   # FP64 paired steps (128 n1^2 <= 63 p)
     (51539509248 - 100000, 51539509248 + 100000, 174270, 174763),  # single steps: p * n1 crosses 2^53
     (4 * 10 ** 6, 6 * 10 ** 6, 10, 1000),                # paired steps with the exact product, ~30 factors
-    (1600000000 - 100000, 1600000000 + 100000, 19500, 20000),  # 4 n1^2 = 1.6e9 inside: paired and single waves
+    (1600000000 - 100000, 1600000000 + 100000, 19500, 20000),  # 4 n1^2 = 1.6e9 inside: all paired
+    (812698413 - 100000, 812698413 + 100000, 19500, 20000),  # 128 n1^2 = 63 p inside: paired and single waves
     (360287970 - 100000, 360287970 + 100000, 4000, 5000),  # p * n1^2 crosses 2^53: exact and inexact pairs
     ((16385 << 39) - 100000, (16385 << 39) + 100000, 16000, 16385),   # hybrid just above 2^53: high-word loop
     ((524300 << 34) - 100000, (524300 << 34) + 100000, 523800, 524300),  # p >> 34 crosses n1

@@ -313,17 +313,18 @@ def test_fp64_exact_step_at_error_bound_edges(fs):
 
 
 def test_fp64_pair_step_at_error_bound_edges(fs):
-    """The FP64 paired step (one product by a*(a+1), used when 4 n1^2 <= p and
-    n1 <= 2^22): exact, and |s'| <= 0.503p, at the largest pair the bounds
-    allow (a + 1 = min(2^22, sqrt(p/4))), |s| up to 0.51p, moduli up to
-    2^53; the exact-product form exactly when p (a+1)^2 <= 2^53."""
+    """The FP64 paired step (one product by a*(a+1), used when
+    128 n1^2 <= 63 p and n1 <= 2^22): exact, and |s'| <= 0.503p, at the
+    largest pair the bounds allow (a + 1 = min(2^22, sqrt(63 p / 128))), |s|
+    up to 0.51p, moduli up to 2^53; the exact-product form exactly when
+    p (a+1)^2 <= 2^53."""
     rng = random.Random(41)
     ps = [2 ** 53 - 111, 2 ** 53 - 1, 2 ** 46 + 15, 2 ** 46 - 1, 2 ** 40 + 15, 2 ** 33 + 17, 2 ** 26 + 15, 4 * 9 + 1,
           4 * 2 ** 44 + 1, 1000003, 10 ** 12 + 39]
     ps += [rng.randrange(2 ** 20, 2 ** 53) | 1 for _ in range(30)]
     for p in ps:
-        atop = min(2 ** 22, math.isqrt(p // 4)) - 1          # largest a with 4 (a+1)^2 <= p, a+1 <= 2^22
-        assert 4 * (atop + 1) ** 2 <= p
+        atop = min(2 ** 22, math.isqrt(63 * p // 128)) - 1   # largest a with 128 (a+1)^2 <= 63 p, a+1 <= 2^22
+        assert 128 * (atop + 1) ** 2 <= 63 * p and atop * (atop + 1) < 0.4922 * p
         smax = int(0.51 * p)
         while smax > 0.51 * p:
             smax -= 1
@@ -337,7 +338,7 @@ def test_fp64_pair_step_at_error_bound_edges(fs):
                 assert raw % p == res and abs(raw) <= 0.503 * p, (p, s, a, raw)
                 assert exact == (p * (a + 1) ** 2 <= 2 ** 53), (p, a)
     p = 2 ** 40 + 15
-    a = math.isqrt(p // 4)                                   # 4 (a+1)^2 > p: no paired step
+    a = math.isqrt(63 * p // 128)                            # 128 (a+1)^2 > 63 p: no paired step
     with pytest.raises(fs.FactSieveError):
         fs.hostemu_fp_pair(p, 5, a)
     with pytest.raises(fs.FactSieveError):
@@ -357,7 +358,7 @@ def oracle_iterate(p, n0, n1, start, cap=64):
 
 def test_fp64_pair_loop_matches_oracle(fs):
     """The FP64 iterate with paired steps against the oracle: moduli on both
-    sides of 4 n1^2 = p (paired loop just inside, single-step loop just
+    sides of 128 n1^2 = 63 p (paired loop just inside, single-step loop just
     outside), with and without the exact product, from random start residues
     and from residues that put +-1 at every position of an 8-step block.  The
     host execution also returns an error if one paired step ever differs from
@@ -366,8 +367,8 @@ def test_fp64_pair_loop_matches_oracle(fs):
     rng = random.Random(43)
     cases = []
     for n1 in (1040, 5003, 20000, 300000, 2 ** 22):
-        edge = 4 * n1 * n1
-        cases += [(edge + 1, n1), (edge + 3, n1), (edge - 1, n1)]   # pair, pair, single
+        edge = -(-128 * n1 * n1 // 63)                          # smallest p with 128 n1^2 <= 63 p
+        cases += [(edge | 1, n1), ((edge | 1) + 2, n1), (edge - 1 - edge % 2, n1)]   # pair, pair, single
     cases += [(2 ** 26 + 15, 1030), (2 ** 33 + 17, 20001), (10 ** 12 + 39, 200000), (2 ** 53 - 111, 2 ** 22)]
     for p, n1 in cases:
         for n0 in (max(1, n1 - 8 * 4 - 3), max(1, n1 - 17)):
