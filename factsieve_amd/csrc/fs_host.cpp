@@ -707,15 +707,18 @@ extern "C" int64_t fs_verify_factors(const fs_factor* f, uint64_t n, int threads
 // for it, or in auto mode when the factors need more than 2^24 multiplications
 // (the CPU loop then takes longer than the launch and transfers); else the CPU
 // threads.  Index of the first failure, -1, or an FS_E_* error below -1.
-int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s) {
+bool fs_verify_uses_gpu(const fs_factor* f, size_t n, int mode) {
+  if (mode != FS_VERIFY_AUTO) return mode == FS_VERIFY_GPU;
   uint64_t work = 0;
   for (size_t i = 0; i < n; ++i) {
     const uint64_t fn = (uint64_t)(f[i].nc < 0 ? -(int64_t)f[i].nc : f[i].nc);
     work += fn < f[i].p ? fn : 0;
   }
-  if (mode == FS_VERIFY_GPU || (mode == FS_VERIFY_AUTO && work > ((uint64_t)1 << 24)))
-    return fs_verify_factors_gpu(f, n, s);
-  return fs_verify_all(f, n, threads);
+  return work > ((uint64_t)1 << 24);
+}
+
+int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s) {
+  return fs_verify_uses_gpu(f, n, mode) ? fs_verify_factors_gpu(f, n, s) : fs_verify_all(f, n, threads);
 }
 
 // Filter composites (2-PRP p) and fold kept factors into the checksum
@@ -823,17 +826,18 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
     return FS_E_OVERFLOW;
   }
   if (nf > 0) {
-    if (verbose) printf("processing %u factors on CPU\n", nf);
     std::vector<fs_factor> h(nf);
     if ((rc = fs_memcpy_dtoh(h.data(), d.fac, (size_t)nf * sizeof(fs_factor), d.s, 1))) return rc;
     fs_sort_factors(h.data(), nf);
     res->raw_factors += nf;
+    const char* where = fs_verify_uses_gpu(h.data(), nf, o.verify_device) ? "GPU" : "CPU";
+    if (verbose) printf("processing %u factors on %s\n", nf, where);
     const int64_t bad = fs_verify_select(h.data(), nf, o.verify_threads, o.verify_device, d.s);
     if (bad < -1) return (int)bad;
     if (bad >= 0) {
       char buf[256];
       const fs_factor& f = h[bad];
-      snprintf(buf, sizeof buf, "CPU factor verification failed!  %llu is not a factor of %u!%+d",
+      snprintf(buf, sizeof buf, "%s factor verification failed!  %llu is not a factor of %u!%+d", where,
                (unsigned long long)f.p, (unsigned)(f.nc < 0 ? -f.nc : f.nc), f.nc < 0 ? -1 : 1);
       fs_set_error_msg(buf);
       fprintf(stderr, "%s\n", buf);

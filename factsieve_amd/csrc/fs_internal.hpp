@@ -24,8 +24,10 @@ std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path);
 // drop composite p and fold kept factors into *checksum (returns kept count).
 void fs_sort_factors(fs_factor* f, size_t n);
 int64_t fs_verify_all(const fs_factor* f, size_t n, int threads);
-// CPU or GPU verifier by mode (FS_VERIFY_*) and the amount of work; the
-// contract of fs_verify_factors.
+// CPU or GPU verifier by mode (FS_VERIFY_*) and the amount of work (the GPU
+// above 2^24 multiplications in FS_VERIFY_AUTO); the contract of
+// fs_verify_factors.  fs_verify_uses_gpu tells which one fs_verify_select runs.
+bool fs_verify_uses_gpu(const fs_factor* f, size_t n, int mode);
 int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s);
 size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose);
 uint32_t fs_auto_segment_candidates();
