@@ -98,6 +98,25 @@ def test_batch_paired_steps_match_oracle(gpu, opts):
         assert o.factors == want["factors"]
 
 
+def test_batch_hybrid_high_word_limits_match_oracle(gpu):
+    """The batched kernel's per-wave choice of the hybrid high-word quotient at
+    its limits: p >> 34 crossing n1 just above 2^53, the limit dropping from
+    p >> 34 to p >> 39 at 2^63, and a job far inside each regime; every job
+    against the oracle."""
+    jobs = [((524300 << 34) - 100000, (524300 << 34) + 100000, 523800, 524300),
+            (2 ** 63 - 3000, 2 ** 63 + 3000, 2 ** 24 - 40, 2 ** 24 + 40),
+            (2 ** 55 + 1, 2 ** 55 + 200000, 300000, 301000),
+            (2 ** 63 + 1, 2 ** 63 + 100000, 20000, 21000)]
+    res, _ = gpu.batch(jobs)
+    for job, o in zip(jobs, res):
+        print("job", job, flush=True)
+        want = ol.sieve(*job, threads=8)
+        assert o.status == 0
+        assert (o.factorcount, o.primecount, o.checksum) == (want["factorcount"], want["primecount"],
+                                                             want["checksum"]), job
+        assert o.factors == want["factors"]
+
+
 @pytest.mark.parametrize("seed", [4, 5])
 def test_batch_random_jobs_match_oracle(gpu, seed):
     """Seeded random jobs over every modulus size (every path and loop) in one
