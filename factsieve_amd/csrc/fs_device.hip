@@ -487,9 +487,13 @@ extern "C" int fs_stream_create(fs_stream* s) {
   *s = (fs_stream)h;
   return FS_OK;
 }
+static int ntab_release(hipStream_t st);
 extern "C" int fs_stream_destroy(fs_stream s) {
-  if (s) HIPCHK(hipStreamDestroy(S(s)), "hipStreamDestroy");
-  return FS_OK;
+  if (!s) return FS_OK;
+  // the stream's multiplier table goes with it (its last kernels finish first)
+  int rc = ntab_release(S(s));
+  HIPCHK(hipStreamDestroy(S(s)), "hipStreamDestroy");
+  return rc;
 }
 extern "C" int fs_stream_sync(fs_stream s) {
   HIPCHK(hipStreamSynchronize(S(s)), "hipStreamSynchronize");
@@ -670,6 +674,22 @@ struct NTab {
 };
 static std::mutex g_ntab_mu;
 static std::map<hipStream_t, NTab> g_ntab;
+
+static int ntab_release(hipStream_t st) {
+  double* d = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_ntab_mu);
+    auto it = g_ntab.find(st);
+    if (it == g_ntab.end()) return FS_OK;
+    d = it->second.d;
+    g_ntab.erase(it);
+  }
+  if (d) {
+    HIPCHK(hipStreamSynchronize(st), "hipStreamSynchronize(ntab)");
+    HIPCHK(hipFree(d), "hipFree(ntab)");
+  }
+  return FS_OK;
+}
 
 static int ntab_for(hipStream_t st, uint32_t first, uint32_t count, const double** out,
                     const double** pout = nullptr) {

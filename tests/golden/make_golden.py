@@ -1,6 +1,7 @@
 """Writes the golden fixtures under tests/golden/ from the CPU oracle.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py              # kats, c1, tiny, spsp2 survivors
+    python tests/golden/make_golden.py c2 c3 c4_sub # the full-size ranges (tens of minutes)
 
 kats.json is the reference's own self-test table (cl_sieve.cpp:1282-1375).
 Every other file is produced by the oracle, whose functions the CPU tests pin
@@ -37,9 +38,46 @@ TINY = [
 ]
 
 C1 = [100000000, 102000000, 1000, 10000]
+# Plain single job at production shape and the full-width modulus stress range.
+C2 = [1000000000000, 1001000000000, 100000, 200000]
+C3 = [1 << 62, (1 << 62) + (1 << 30), 500000, 510000]
 
 
-def main():
+def write_range(name, rng, threads=8, jobs_note=None):
+    r = oracle_lib.sieve(*rng, threads=threads)
+    doc = {"range": rng, "factorcount": r["factorcount"], "primecount": r["primecount"],
+           "checksum": f"{r['checksum']:016X}", "device_checksum": f"{r['device_checksum']:016X}",
+           "raw": r["raw"], "factors": r["factors"]}
+    with open(os.path.join(HERE, name + ".json"), "w") as f:
+        json.dump(doc, f, indent=1)
+    print(name, doc["factorcount"], doc["primecount"], doc["checksum"], flush=True)
+
+
+def write_c4_sub(threads=8):
+    """The 64-job mixed batch (seed 20261203, sigma 1.0) with every p-range cut
+    to its first 1/64 (at least one integer) and the full n-ranges."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from factsieve_amd import workloads
+    jobs = []
+    for pmin, pmax, nmin, nmax in workloads.mixed_jobs(64):
+        job = [pmin, pmin + max(1, (pmax - pmin) // 64), nmin, nmax]
+        r = oracle_lib.sieve(*job, threads=threads)
+        jobs.append({"range": job, "factorcount": r["factorcount"], "primecount": r["primecount"],
+                     "checksum": f"{r['checksum']:016X}", "factors": r["factors"]})
+        print("c4_sub", len(jobs), job, r["primecount"], flush=True)
+    with open(os.path.join(HERE, "c4_sub.json"), "w") as f:
+        json.dump({"seed": workloads.DEFAULT_SEED, "sigma": 1.0, "p_fraction": "1/64", "jobs": jobs}, f, indent=1)
+
+
+def main(targets):
+    if "c2" in targets:
+        write_range("c2", C2)
+    if "c3" in targets:
+        write_range("c3", C3)
+    if "c4_sub" in targets:
+        write_c4_sub()
+    if "base" not in targets:
+        return
     with open(os.path.join(HERE, "kats.json"), "w") as f:
         json.dump({"source": "cl_sieve.cpp:1282-1375 (reference self-test)",
                    "fields": ["pmin", "pmax", "nmin", "nmax", "factors", "primecount", "checksum"],
@@ -69,4 +107,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or ["base"])

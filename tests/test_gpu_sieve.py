@@ -175,28 +175,41 @@ def test_validation_flags_iterate_out_of_order(gpu):
         seg.close()
 
 
+def full_size_fixture(name):
+    """(factorcount, primecount, checksum, factors) of a full-size range as the
+    CPU oracle computed it (tests/golden/make_golden.py c2 c3)."""
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name + ".json")) as f:
+        fx = json.load(f)
+    return fx["factorcount"], fx["primecount"], int(fx["checksum"], 16), [tuple(x) for x in fx["factors"]]
+
+
 def test_paths_agree_at_full_size_c2(gpu):
     """Full C2 (3.6e12 p*n steps): the FP64 (paired steps), hybrid and
     Montgomery (the reference's own formulation) paths are independent
-    arithmetic; their checksums and factor lists must agree, and every factor
-    is re-verified by the sieve's own verifier."""
+    arithmetic; their checksums and factor lists must agree with each other and
+    with the CPU oracle's fixture, and every factor is re-verified by the
+    sieve's own verifier."""
     args = (10 ** 12, 10 ** 12 + 10 ** 9, 10 ** 5, 2 * 10 ** 5)
     a = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_FP64)
     b = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_HYBRID)
     c = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_MONT)
     assert a.primecount == b.primecount == c.primecount == 36191004
     assert (a.checksum, a.factors) == (b.checksum, b.factors) == (c.checksum, c.factors)
-    assert a.checksum == 0xF67C312116703E89 and a.factorcount == 7
+    want = full_size_fixture("c2")
+    assert (a.factorcount, a.primecount, a.checksum, a.factors) == want
 
 
 def test_paths_agree_at_full_size_c3(gpu):
     """Full C3 (p in [2^62, 2^62 + 2^30), n in [5e5, 5.1e5)): the hybrid path
-    (high-word quotient, two-stage REDC setup) against the Montgomery path."""
+    (high-word quotient, two-stage REDC setup) against the Montgomery path and
+    the CPU oracle's fixture."""
     args = (2 ** 62, 2 ** 62 + 2 ** 30, 500000, 510000)
     a = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_HYBRID)
     b = gpu.sieve(*args, verify_threads=16, path=gpu.PATH_MONT)
     assert a.primecount == b.primecount == 24983780
     assert (a.checksum, a.factors) == (b.checksum, b.factors)
+    want = full_size_fixture("c3")
+    assert (a.factorcount, a.primecount, a.checksum, a.factors) == want
 
 
 def test_c3_subrange_hybrid_vs_montgomery(gpu):
@@ -417,3 +430,18 @@ def test_missing_factor_lines_detected(gpu, tmp_path):
     with pytest.raises(gpu.FactSieveError) as e:
         gpu.sieve(*g, **opts)
     assert e.value.code == gpu.FS_E_IO and "Missing factors" in str(e.value)
+
+
+def test_repeated_sieves_keep_device_memory_flat(gpu):
+    """Each plain sieve's stream-bound multiplier table (here 2^21 n per
+    launch, about 50 MB) is released with its stream: free device memory
+    does not drift over repeated calls."""
+    job = (10 ** 9, 10 ** 9 + 20000, 10 ** 6, 10 ** 6 + 2 ** 21)
+    first = gpu.sieve(*job)
+    free0 = gpu.device_info()["free_mem"]
+    for _ in range(12):
+        r = gpu.sieve(*job)
+        assert (r.checksum, r.factors) == (first.checksum, first.factors)
+    drift = free0 - gpu.device_info()["free_mem"]
+    print(f"free memory drift over 12 calls: {drift / 2 ** 20:.1f} MiB", flush=True)
+    assert drift < 64 * 2 ** 20
