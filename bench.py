@@ -13,12 +13,12 @@ jobs.  One JSON line is printed.
 roofline: the dominant kernel is the residue iteration.  Its work is counted
 in VALU lane-operations per p*n step of the compiled 8-step loop, timed with
 HIP events on the launch stream.  On the FP64 path (p < 2^53): 4.0 with paired
-steps, used when 128 nmax^2 <= 63 p (one 6-operation product by (n+1)(n+2) per two
-steps -- 2 multiplies, 2 FMAs, 1 add, 1 round -- and two exact compares,
-|v| == (n+1)(n+2) and |v| == n+2, which find +-1 at n and n+1), 3.0 when also
-p nmax^2 <= 2^53 (exact product: 2 multiplies, 1 FMA, 1 round); otherwise 6.5
-with single steps (one compare per two steps: |v[k+1]| <= m[k] covers +-1 at k
-and k+1); 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
+steps, used when nmax <= 2^22 and 64 nmax <= 31 p (one 6-operation product by
+(n+1)(n+2) per two steps -- 2 multiplies, 2 FMAs, 1 add, 1 round -- and two
+exact compares, |v| == 1 on the residue of n and |v'| == n+2 on the product,
+which find +-1 at n and n+1), 3.0 when also p nmax^2 <= 2^53 (exact product:
+2 multiplies, 1 FMA, 1 round); otherwise 6.5 with single steps (one compare per
+two steps: |v[k+1]| <= m[k] covers +-1 at k and k+1); 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
 2 v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, and 6 filter ops per 8 steps), 9.625
 when every modulus is at least nmax * 2^34 (below 2^63; nmax * 2^39 above)
 and the quotient comes from the high word of the residue alone (4 FP64 ops
@@ -58,14 +58,14 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine
 OPS_PER_STEP = {1: 6.5, 2: 11.75, 3: 45}
 OPS_PER_STEP_HYBRID_HI = 9.625                  # hybrid with the high-word quotient: 77 per 8 steps
 OPS_PER_STEP_FP64_EXACT = 4.5                   # FP64 with an exact product (p * nmax <= 2^53): 36 per 8
-OPS_PER_STEP_FP64_PAIR = 4.0                    # FP64 paired steps (128 nmax^2 <= 63 p): 4 x 6 + 8 compares per 8
+OPS_PER_STEP_FP64_PAIR = 4.0                    # FP64 paired steps (64 nmax <= 31 p): 4 x 6 + 8 compares per 8
 OPS_PER_STEP_FP64_PAIR_EXACT = 3.0              # paired steps with an exact product (p nmax^2 <= 2^53): 4 x 4 + 8
 
 
 def fp64_ops_at(pmin, pmax, n1):
     """VALU operations per step of the FP64 iterate loop every wave of a launch
     ending at n1 takes (the per-wave choice of fs_cand_iter_fp64)."""
-    if n1 <= 2 ** 22 and 128 * n1 * n1 <= 63 * pmin:
+    if n1 <= 2 ** 22 and 64 * n1 <= 31 * pmin:
         return OPS_PER_STEP_FP64_PAIR_EXACT if pmax * n1 * n1 <= 2 ** 53 else OPS_PER_STEP_FP64_PAIR
     return OPS_PER_STEP_FP64_EXACT if pmax * n1 <= 2 ** 53 else OPS_PER_STEP[1]
 
@@ -73,10 +73,10 @@ def fp64_ops_at(pmin, pmax, n1):
 def fp64_ops(pmin, pmax, nmin, nmax):
     """Step-weighted VALU operations per step of a job's FP64 iterate launches:
     the driver ends launches where a cheaper loop stops applying (paired steps
-    up to isqrt(63 pmin / 128), exact products up to 2^53 / pmax and
+    up to 31 pmin / 64 and 2^22, exact products up to 2^53 / pmax and
     isqrt(2^53 / pmax)), as for a one-segment job."""
     top = pmax - 1
-    pair = min(math.isqrt(63 * pmin // 128), 2 ** 22)
+    pair = min(31 * pmin // 64, 2 ** 22)
     cuts = sorted({c for c in (pair, min(pair, math.isqrt(2 ** 53 // top)), 2 ** 53 // top) if nmin < c < nmax})
     total, a = 0.0, nmin
     for b in cuts + [nmax]:

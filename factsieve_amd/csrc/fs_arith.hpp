@@ -23,7 +23,7 @@
 //    exact double.  Because |s| < p - 1 (p > 5),
 //    s == +-1 (mod p) iff |s| == 1.  Six full-rate FP64 VALU operations (four
 //    when s*m is exact, fs_fp_mulmod_exact); the iterate loops multiply by
-//    (n+1)(n+2) at once when 128 n1^2 <= 63 p (fs_fp_pair_ok).
+//    (n+1)(n+2) at once when n1 <= 2^22 and 64 n1 <= 31 p (fs_fp_pair_ok).
 //
 //  * Hybrid (p <= 2^64 - 2^46, used above 2^53): signed residue |s| <= 0.51p (and
 //    below 2^63: the step keeps |s| <= (1/2 + 2^-19) p), multiplier
@@ -131,16 +131,18 @@ FS_HD double fs_fp_mulmod_exact(double s, double m, double p, double pinv) {
 
 // Paired FP64 step: one product by M = a*(a+1) advances the residue over two
 // consecutive multipliers a, a+1 <= n1.  fs_fp_pair_ok(p, n1): n1 <= 2^22 and
-// 128 n1^2 <= 63 p.  Then M < n1^2 <= 2^44, so k = rint(fl(h * fl(1/p))) is
+// 64 n1 <= 31 p.  Then M < n1^2 <= 2^44, so k = rint(fl(h * fl(1/p))) is
 // within 1/2 + 3 * 2^-53 * 0.51 * 2^44 < 0.503 of X = s*M/p and every step keeps
 // |s'| <= 0.503 p (fs_fp_mulmod, whose bounds need only M < 2^50 and |s| < 0.8p);
-// and M < n1^2 <= (63/128) p < 0.4922 p, so of the values congruent to +-M only
-// +-M itself has magnitude <= 0.503 p (|M - p| > 0.5078 p): a step's residue
-// congruent to +-M (or to +-(a+1)) is exactly +-M (or +-(a+1)).
+// M itself may exceed p.  And a+1 <= n1 <= (31/64) p < 0.4844 p, so of the
+// values congruent to +-(a+1) only +-(a+1) itself has magnitude <= 0.503 p
+// (|a+1 - p| > 0.5156 p): a paired step's result congruent to +-(a+1), which it
+// is whenever the skipped residue (the one of a, times a+1) is +-1, is exactly
+// +-(a+1).  The residues the loop keeps are tested for +-1 directly.
 // fs_fp_pair_exact_ok adds p * n1^2 <= 2^53: |s*M| < 0.51 * 2^53 and
 // fs_fp_mulmod_exact applies.
 FS_HD bool fs_fp_pair_ok(uint64_t p, uint32_t n1) {
-  return n1 <= (1u << 22) && 128 * (uint64_t)n1 * n1 <= 63 * p;
+  return n1 <= (1u << 22) && 64 * (uint64_t)n1 <= 31 * p;
 }
 FS_HD bool fs_fp_pair_exact_ok(uint64_t p, uint32_t n1) {
   return fs_fp_pair_ok(p, n1) && (unsigned __int128)p * ((uint64_t)n1 * n1) <= ((unsigned __int128)1 << 53);

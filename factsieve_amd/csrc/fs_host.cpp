@@ -944,7 +944,7 @@ uint32_t fs_auto_segment_candidates() {
 // n values where a cheaper iterate loop stops applying to every modulus of
 // [plo, phi) (the kernels choose their loop per wave by the launch's last n):
 // a launch that ends at such a point runs the n below it with that loop.
-// FP64: paired steps while 128 n1^2 <= 63 p and n1 <= 2^22 (fs_fp_pair_ok, smallest
+// FP64: paired steps while 64 n1 <= 31 p and n1 <= 2^22 (fs_fp_pair_ok, smallest
 // p), paired with the exact product while p n1^2 <= 2^53, single steps with the
 // exact product while p n1 <= 2^53 (largest p); hybrid: the high-word quotient
 // while n1 <= fs_sh_hi_mmax(p) (p >> 34 below 2^63, p >> 39 above; the
@@ -962,7 +962,7 @@ static std::vector<uint32_t> loop_boundaries(int path, uint64_t plo, uint64_t ph
   };
   const uint64_t ptop = phi - 1;  // largest modulus of the segment
   if (path == FS_PATH_FP64 && ptop > 0) {
-    const uint64_t pair = std::min<uint64_t>(isqrt(63 * plo / 128), (uint64_t)1 << 22);
+    const uint64_t pair = std::min<uint64_t>(31 * plo / 64, (uint64_t)1 << 22);
     add(pair);
     add(std::min<uint64_t>(pair, isqrt(((uint64_t)1 << 53) / ptop)));
     add(((uint64_t)1 << 53) / ptop);

@@ -209,10 +209,11 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     uint32_t n = n0;
     if (fs_fp_pair_ok(p, n1)) {
       // The paired loop: one product by M = a*(a+1) per two positions, the
-      // compares |v| == M and |v| == a+1 on its result (each pair's compares are
-      // exact on their own, so the device's block length does not matter here).  The odd positions'
-      // residues are formed here by single steps only to check that the
-      // compares fire exactly for the blocks that hold a +-1 residue.
+      // compares |v| == 1 on its input and |v| == a+1 on its result (each
+      // pair's compares are exact on their own, so the device's block length
+      // does not matter here).  The odd positions' residues are formed here by
+      // single steps only to check that the compares fire exactly for the
+      // pairs that hold a +-1 residue.
       const bool pexact = fs_fp_pair_exact_ok(p, n1);
       auto pstep = [&](double x, uint64_t M) {
         return pexact ? fs_fp_mulmod_exact(x, (double)M, pd, pinv) : fs_fp_mulmod(x, (double)M, pd, pinv);
@@ -224,8 +225,7 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
           const uint64_t a = n + 1 + 2 * i;
           v[i + 1] = pstep(v[i], a * (a + 1));
           if (__builtin_fabs(v[i + 1]) > 0.51 * pd) return FS_E_VALIDATE;  // the loop's invariant
-          const bool hit =
-              (__builtin_fabs(v[i + 1]) == (double)(a * (a + 1))) | (__builtin_fabs(v[i + 1]) == (double)(a + 1));
+          const bool hit = (__builtin_fabs(v[i]) == 1.0) | (__builtin_fabs(v[i + 1]) == (double)(a + 1));
           const double u = fs_fp_mulmod(v[i], (double)a, pd, pinv);
           const double odd = pstep(u, a + 1);
           if (fs_fp_to_canon(odd, p) != fs_fp_to_canon(v[i + 1], p)) return FS_E_VALIDATE;
@@ -238,9 +238,9 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
             rec(r[k] > 0.0 ? -(int32_t)nn : (int32_t)nn);
           }
           // A +-1 residue always makes a compare fire.  A compare without one
-          // needs a(a+1) not invertible mod p (a composite probable prime).
+          // needs a+1 not invertible mod p (a composite probable prime).
           if (any && !hit) return FS_E_VALIDATE;
-          if (hit && !any && std::gcd(a * (a + 1), p) == 1) return FS_E_VALIDATE;
+          if (hit && !any && std::gcd(a + 1, p) == 1) return FS_E_VALIDATE;
         }
         s = v[U / 2];
       }

@@ -401,13 +401,14 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t 
 // FP64 path, paired steps (fs_fp_pair_ok(p, n1)): one product by
 // M = a*(a+1) per two multipliers.  pt[2j] = (n0+1+2j)*(n0+2+2j) and
 // pt[2j+1] = n0+2+2j, read with scalar loads, with at least U entries (U/2
-// pairs) past index n1 - n0.  With v[k] the residue of n0 + k and M, a+1 the pair's values,
-// v[k+2] = v[k] * M and v[k+2] = v[k+1] * (a+1), so, both values being below
-// 0.4922 p and every residue of magnitude <= 0.503 p, |v[k+2]| == M exactly when
-// v[k] = +-1 and |v[k+2]| == a+1 exactly when v[k+1] = +-1.  Per two steps:
-// one six-operation product (four when EXACT) and two compares.  The taken
-// branch (a +-1 residue in the block) recomputes the odd positions with single
-// steps and records every +-1 in order.
+// pairs) past index n1 - n0.  With v[i] the residue of nn = nb + 2i and a+1 =
+// nn+2 the pair's second multiplier, v[i+1] = v[i] * M = (residue of nn+1) *
+// (a+1); a+1 being below 0.4844 p and every residue of magnitude <= 0.503 p,
+// |v[i+1]| == a+1 exactly when the residue of nn+1 is +-1, and |v[i]| == 1
+// exactly when the residue of nn is.  Per two steps: one six-operation
+// product (four when EXACT) and two compares.  The taken branch (a +-1 residue
+// in the block) recomputes the odd positions with single steps and records
+// every +-1 in order.
 template <int U, bool EXACT, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                                 const double* __restrict__ nt,
@@ -433,7 +434,7 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint
     uint64_t hit = 0;
 #pragma unroll
     for (int i = 0; i < H; ++i)
-      hit |= __ballot(__builtin_fabs(v[i + 1]) == w[2 * i]) | __ballot(__builtin_fabs(v[i + 1]) == w[2 * i + 1]);
+      hit |= __ballot(__builtin_fabs(v[i]) == 1.0) | __ballot(__builtin_fabs(v[i + 1]) == w[2 * i + 1]);
     if (hit) {
 #pragma unroll
       for (int i = 0; i < H; ++i) {

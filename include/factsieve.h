@@ -219,9 +219,9 @@ int fs_hostemu_fp_exact(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_
  * otherwise); |raw| <= 0.7500005p, and <= 0.51p in the second case. */
 int fs_hostemu_mulmod_hi(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64_t* res);
 /* The FP64 paired step, one product by a*(a+1) for the multipliers a and a+1,
- * used when 128 * n1^2 <= 63 * p and n1 <= 2^22 (for every pair a+1 <= n1 of a launch;
+ * used when 64 * n1 <= 31 * p and n1 <= 2^22 (for every pair a+1 <= n1 of a launch;
  * without the error term when also p * n1^2 <= 2^53): |s| <= 0.51p, 1 <= a,
- * 128 (a+1)^2 <= 63 p, a + 1 <= 2^22, odd p < 2^53 (FS_E_ARG otherwise); *exact
+ * 64 (a+1) <= 31 p, a + 1 <= 2^22, odd p < 2^53 (FS_E_ARG otherwise); *exact
  * tells which form ran; |raw| <= 0.503p. */
 int fs_hostemu_fp_pair(uint64_t p, int64_t s, uint32_t a, int64_t* raw, uint64_t* res, int* exact);
 /* One multiply (square = 0) or square (square = 1, m ignored) of the HYBRID

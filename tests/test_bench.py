@@ -32,7 +32,7 @@ def test_fp64_ops_model_matches_kernel_loops(fs):
     want = {(True, True): bench.OPS_PER_STEP_FP64_PAIR_EXACT, (True, False): bench.OPS_PER_STEP_FP64_PAIR,
             (False, True): bench.OPS_PER_STEP_FP64_EXACT, (False, False): bench.OPS_PER_STEP[1]}
     for p in ps:
-        edges = {math.isqrt(63 * p // 128), math.isqrt(2 ** 53 // p), 2 ** 53 // p, 2 ** 22}
+        edges = {31 * p // 64, math.isqrt(2 ** 53 // p), 2 ** 53 // p, 2 ** 22}
         n1s = {n + d for n in edges for d in (-1, 0, 1)} | {rng.randrange(2, 2 ** 23)}
         for n1 in sorted(n1s):
             if not 2 <= n1 < 2 ** 31 or n1 >= p:

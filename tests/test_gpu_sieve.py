@@ -445,3 +445,120 @@ def test_repeated_sieves_keep_device_memory_flat(gpu):
     drift = free0 - gpu.device_info()["free_mem"]
     print(f"free memory drift over 12 calls: {drift / 2 ** 20:.1f} MiB", flush=True)
     assert drift < 64 * 2 ** 20
+
+
+def reference_order(factors):
+    """True when a factor list is in the reference's order: p ascending, then
+    |n| ascending (factorcompare, cl_sieve.cpp:421-436), with no duplicates."""
+    keys = [(p, abs(nc)) for p, nc in factors]
+    return all(a < b for a, b in zip(keys, keys[1:]))
+
+
+@pytest.mark.parametrize("lo,n0,note", [
+    (10 ** 12, 100000, "inexact pairs, a(a+1) < p"),
+    (10 ** 8, 5000, "exact pairs (p n1^2 <= 2^53)"),
+    (2 ** 32 + 1, 300000, "inexact pairs, a(a+1) far above p"),
+])
+def test_paired_loop_planted_every_position(gpu, lo, n0, note):
+    """The paired FP64 iterate kernel (16 steps per block, one product by
+    (n+1)(n+2) per two steps, ballots of |v| == 1 and |v'| == n+2) from start
+    residues that put +-1 at every position of a block, both signs, in three
+    consecutive blocks and the tail; end residues and factor lists against
+    the oracle."""
+    n1 = n0 + 3 * 16 + 7
+    assert 64 * n1 <= 31 * lo
+    seg = gpu.Segment(cap=8192)
+    try:
+        print(f"planted pairs: p from {lo}, n=[{n0},{n1}) ({note})", flush=True)
+        seg.clearresult()
+        seg.clearn()
+        seg.getsegprimes(lo, lo + 60000)
+        n = seg.counters()[gpu.CTR_COUNT]
+        assert 200 <= n <= 8192
+        ps = seg.read_u64("p", n)
+        starts, planted = [], set()
+        for i, p in enumerate(ps):
+            k, sign = i % (n1 - n0), (1 if (i // (n1 - n0)) % 2 == 0 else -1)
+            prod = 1
+            for j in range(1, k + 1):
+                prod = prod * (n0 + j) % p
+            if i < 4 * (n1 - n0) and math.gcd(prod, p) == 1:
+                starts.append(sign * pow(prod, -1, p) % p)
+                planted.add((p, -(n0 + k) if sign == 1 else n0 + k))
+            else:
+                starts.append((i * 7919 + 12345) % p)
+        covered = {(abs(nc) - n0, nc < 0) for _, nc in planted}
+        assert len(covered) == 2 * (n1 - n0)      # every position, both signs
+        seg.write_u64("res", starts)
+        seg.write_u32("nnext", [n0] * n)
+        seg.iterate(n0, n1, gpu.PATH_FP64)
+        res = seg.read_u64("res", n)
+        facs = seg.factors()
+        assert seg.counters()[gpu.CTR_VALIDATE] == 0
+        want_f = []
+        for p, s0, r in zip(ps, starts, res):
+            want_r, f = oracle_iterate(p, n0, n1, s0)
+            assert r == want_r, p
+            want_f += f
+        assert sorted(facs) == sorted(want_f)
+        assert planted <= set(facs)
+    finally:
+        seg.close()
+
+
+@pytest.mark.parametrize("fault", ["residue", "power_table"])
+def test_verify_kernels_detect_wrong_setup(gpu, fault):
+    """The verify kernels (verifyslow/pow/reduce/result) compute nmin! mod p of
+    the segment's first candidate by the direct product and by the reference
+    power table and compare both with the setup kernel's residue
+    (verifyresult.cl -> cl_sieve.cpp:903-907).  A corrupted residue, or a
+    power table with one exponent off, raises the power-table flag; the
+    untouched segment does not."""
+    import ctypes
+    L = gpu.lib()
+    nmin = 5000
+    seg = gpu.Segment(cap=8192)
+    pb, pe = ctypes.c_void_p(), ctypes.c_void_p()
+    try:
+        seg.clearresult()
+        seg.clearn()
+        seg.getsegprimes(10 ** 9, 10 ** 9 + 20000)
+        seg.load_program(nmin, gpu.PATH_FP64)
+        seg.setup(nmin, gpu.PATH_FP64)
+        seg.sync()
+        seg.verify(nmin)
+        assert seg.counters()[gpu.CTR_POWERTABLE] == 0
+        b, e, _ = gpu.power_table(nmin)
+        if fault == "residue":
+            r0 = seg.read_u64("res", 1)[0]
+            p0 = seg.read_u64("p", 1)[0]
+            seg.write_u64("res", [(r0 + 1) % p0])
+        else:
+            e = list(e)
+            e[len(e) // 2] += 1
+        gpu.check(L.fs_malloc(ctypes.byref(pb), len(b) * 8))
+        gpu.check(L.fs_malloc(ctypes.byref(pe), len(e) * 4))
+        ab, ae = (gpu.u64 * len(b))(*b), (gpu.u32 * len(e))(*e)
+        gpu.check(L.fs_memcpy_htod(pb, ab, len(b) * 8, seg.stream, 1))
+        gpu.check(L.fs_memcpy_htod(pe, ae, len(e) * 4, seg.stream, 1))
+        gpu.check(L.fs_verifyslow(ctypes.byref(seg.c), seg.ctr, seg.ver, nmin, seg.stream))
+        gpu.check(L.fs_verifypow(ctypes.byref(seg.c), seg.ctr, pb, pe, len(b), seg.ver, seg.stream))
+        gpu.check(L.fs_verifyreduce(ctypes.byref(seg.c), seg.ctr, seg.ver, seg.stream))
+        gpu.check(L.fs_verifyresult(ctypes.byref(seg.c), seg.ctr, seg.ver, seg.stream))
+        seg.sync()
+        assert seg.counters()[gpu.CTR_POWERTABLE] == 1
+    finally:
+        L.fs_free(pb)
+        L.fs_free(pe)
+        seg.close()
+
+
+def test_factor_lists_in_reference_order(gpu):
+    """Kept factors come back sorted as the reference sorts them: p ascending,
+    then |n| (cl_sieve.cpp:421-436); the golden lists are in that order too."""
+    for case in TINY + [C1]:
+        assert reference_order([tuple(f) for f in case["factors"]])
+    r = gpu.sieve(127, 20000, 127, 30000)   # p <= nmax: many factors per p
+    assert r.factorcount > 1000 and reference_order(r.factors)
+    res, _ = gpu.batch([(127, 20000, 127, 30000), (10 ** 8, 10 ** 8 + 300000, 1000, 3000)])
+    assert all(reference_order(o.factors) for o in res)

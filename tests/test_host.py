@@ -314,17 +314,17 @@ def test_fp64_exact_step_at_error_bound_edges(fs):
 
 def test_fp64_pair_step_at_error_bound_edges(fs):
     """The FP64 paired step (one product by a*(a+1), used when
-    128 n1^2 <= 63 p and n1 <= 2^22): exact, and |s'| <= 0.503p, at the
-    largest pair the bounds allow (a + 1 = min(2^22, sqrt(63 p / 128))), |s|
-    up to 0.51p, moduli up to 2^53; the exact-product form exactly when
-    p (a+1)^2 <= 2^53."""
+    64 n1 <= 31 p and n1 <= 2^22): exact, and |s'| <= 0.503p, at the
+    largest pair the bounds allow (a + 1 = min(2^22, 31 p / 64), where
+    a*(a+1) may exceed p), |s| up to 0.51p, moduli up to 2^53; the
+    exact-product form exactly when p (a+1)^2 <= 2^53."""
     rng = random.Random(41)
     ps = [2 ** 53 - 111, 2 ** 53 - 1, 2 ** 46 + 15, 2 ** 46 - 1, 2 ** 40 + 15, 2 ** 33 + 17, 2 ** 26 + 15, 4 * 9 + 1,
           4 * 2 ** 44 + 1, 1000003, 10 ** 12 + 39]
     ps += [rng.randrange(2 ** 20, 2 ** 53) | 1 for _ in range(30)]
     for p in ps:
-        atop = min(2 ** 22, math.isqrt(63 * p // 128)) - 1   # largest a with 128 (a+1)^2 <= 63 p, a+1 <= 2^22
-        assert 128 * (atop + 1) ** 2 <= 63 * p and atop * (atop + 1) < 0.4922 * p
+        atop = min(2 ** 22, 31 * p // 64) - 1   # largest a with 64 (a+1) <= 31 p, a+1 <= 2^22
+        assert 64 * (atop + 1) <= 31 * p
         smax = int(0.51 * p)
         while smax > 0.51 * p:
             smax -= 1
@@ -337,10 +337,10 @@ def test_fp64_pair_step_at_error_bound_edges(fs):
                 assert res == (s * M) % p, (p, s, a)
                 assert raw % p == res and abs(raw) <= 0.503 * p, (p, s, a, raw)
                 assert exact == (p * (a + 1) ** 2 <= 2 ** 53), (p, a)
-    p = 2 ** 40 + 15
-    a = math.isqrt(63 * p // 128)                            # 128 (a+1)^2 > 63 p: no paired step
+    p = 1000003
+    fs.hostemu_fp_pair(p, 5, 31 * p // 64 - 1)               # 64 (a+1) <= 31 p: the largest pair
     with pytest.raises(fs.FactSieveError):
-        fs.hostemu_fp_pair(p, 5, a)
+        fs.hostemu_fp_pair(p, 5, 31 * p // 64)               # 64 (a+1) > 31 p: no paired step
     with pytest.raises(fs.FactSieveError):
         fs.hostemu_fp_pair(2 ** 52 + 1, 5, 2 ** 22)         # a + 1 above 2^22
 
@@ -358,18 +358,20 @@ def oracle_iterate(p, n0, n1, start, cap=64):
 
 def test_fp64_pair_loop_matches_oracle(fs):
     """The FP64 iterate with paired steps against the oracle: moduli on both
-    sides of 128 n1^2 = 63 p (paired loop just inside, single-step loop just
-    outside), with and without the exact product, from random start residues
-    and from residues that put +-1 at every position of an 8-step block.  The
-    host execution also returns an error if one paired step ever differs from
-    two single steps, leaves |s| > 0.51p, or if the block's compares
-    (|v| == a*(a+1), |v| == a+1) disagree with an exact test of its residues."""
+    sides of 64 n1 = 31 p (paired loop just inside, single-step loop just
+    outside), with and without the exact product, with a*(a+1) below and far
+    above p, from random start residues and from residues that put +-1 at
+    every position of an 8-step block.  The host execution also returns an
+    error if one paired step ever differs from two single steps, leaves
+    |s| > 0.51p, or if a pair's compares (|v| == 1 on its input, |v'| == a+1 on
+    its product) disagree with an exact test of its residues."""
     rng = random.Random(43)
     cases = []
     for n1 in (1040, 5003, 20000, 300000, 2 ** 22):
-        edge = -(-128 * n1 * n1 // 63)                          # smallest p with 128 n1^2 <= 63 p
+        edge = -(-64 * n1 // 31)                                # smallest p with 64 n1 <= 31 p
         cases += [(edge | 1, n1), ((edge | 1) + 2, n1), (edge - 1 - edge % 2, n1)]   # pair, pair, single
-    cases += [(2 ** 26 + 15, 1030), (2 ** 33 + 17, 20001), (10 ** 12 + 39, 200000), (2 ** 53 - 111, 2 ** 22)]
+    cases += [(2 ** 26 + 15, 1030), (2 ** 33 + 17, 20001), (10 ** 12 + 39, 200000), (2 ** 53 - 111, 2 ** 22),
+              (10 ** 8 + 7, 9000), (2 ** 32 + 15, 500000), (2 ** 36 + 31, 2 ** 22)]   # a*(a+1) > p
     for p, n1 in cases:
         for n0 in (max(1, n1 - 8 * 4 - 3), max(1, n1 - 17)):
             starts = [rng.randrange(p) for _ in range(2)]
