@@ -123,6 +123,7 @@ _SIGS = [
     ("fs_getsegprimes", ctypes.c_int, [u64, u64, ctypes.POINTER(Cands), vp, vp]),
     ("fs_setup", ctypes.c_int, [ctypes.POINTER(Cands), vp, vp, u32, u32, u32, u32, ctypes.c_int, vp]),
     ("fs_iterate", ctypes.c_int, [ctypes.POINTER(Cands), vp, vp, u32, u32, u32, ctypes.c_int, vp]),
+    ("fs_iterate_noalloc", ctypes.c_int, [ctypes.POINTER(Cands), vp, vp, u32, u32, u32, ctypes.c_int, vp]),
     ("fs_check", ctypes.c_int, [ctypes.POINTER(Cands), vp, vp, u32, vp]),
     ("fs_verify_bytes", ctypes.c_size_t, []),
     ("fs_verifyslow", ctypes.c_int, [ctypes.POINTER(Cands), vp, vp, u32, vp]),
@@ -477,6 +478,11 @@ class Segment:
     def iterate(self, n0, n1, path):
         check(lib().fs_iterate(ctypes.byref(self.c), self.ctr, self.fac, self.factor_cap, n0, n1, path,
                                self.stream))
+
+    def iterate_noalloc(self, n0, n1, path):
+        """fs_iterate with the library's static tables (no device allocation)."""
+        check(lib().fs_iterate_noalloc(ctypes.byref(self.c), self.ctr, self.fac, self.factor_cap, n0, n1, path,
+                                       self.stream))
 
     def check(self, nmax):
         check(lib().fs_check(ctypes.byref(self.c), self.ctr, self.sum, nmax, self.stream))

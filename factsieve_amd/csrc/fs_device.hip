@@ -356,6 +356,19 @@ static int g_device = -1;
 static std::mutex g_init_mu;
 static int g_inject = 0;
 
+// Tables of fs_iterate_noalloc: one file-scope device array, a copy on every
+// GPU the library runs on (held until the process ends).  Layout as a
+// multiplier table of kScrCap entries (NTab below): kScrCap multipliers, then
+// the pair table's two halves of kScrCap / 2 + 1 pairs of two doubles.
+static constexpr uint32_t kScrCap = FS_ITERATE_NOALLOC_MAX_N + 64;
+static constexpr size_t kScrDoubles = (size_t)kScrCap + 4 * ((size_t)kScrCap / 2 + 1);
+static __device__ __align__(256) double scratch[kScrDoubles];
+// Device address of `scratch` per GPU, from hipGetSymbolAddress with that GPU
+// current (the array's name in host code is not a device address).
+static constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static hipError_t g_scratch_err[kMaxDevices] = {};
+
 extern "C" int fs_init(int device_index) {
   std::lock_guard<std::mutex> lk(g_init_mu);
   int n = 0;
@@ -370,6 +383,13 @@ extern "C" int fs_init(int device_index) {
   }
   HIPCHK(hipSetDevice(device_index), "hipSetDevice");
   g_device = device_index;
+  if (device_index < kMaxDevices && !g_scratch_ptr[device_index] && g_scratch_err[device_index] == hipSuccess) {
+    void* ptr = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (se == hipSuccess && ptr) g_scratch_ptr[device_index] = (double*)ptr;
+    else g_scratch_err[device_index] = se != hipSuccess ? se : hipErrorInvalidSymbol;  // reported by the
+                                                                                    // entry point that needs it
+  }
   return FS_OK;
 }
 
@@ -758,6 +778,54 @@ extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_facto
     k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_FACTOROVF);
     g_inject = 0;
   }
+  return FS_OK;
+}
+
+extern "C" int fs_iterate_noalloc(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t fcap,
+                                  uint32_t n0, uint32_t n1, int path, fs_stream s) {
+  int rc = check_cands(c, "iterate_noalloc");
+  if (rc) return rc;
+  if (n0 > n1 || n1 > 0x7fffffffu) {
+    g_last_error = "iterate_noalloc: need startN <= endN < 2^31";
+    return FS_E_ARG;
+  }
+  if (n1 - n0 > FS_ITERATE_NOALLOC_MAX_N) {
+    g_last_error = "iterate_noalloc: endN - startN exceeds FS_ITERATE_NOALLOC_MAX_N (2^20)";
+    return FS_E_ARG;
+  }
+  if (path != FS_PATH_FP64 && path != FS_PATH_HYBRID && path != FS_PATH_MONT) {
+    g_last_error = "iterate_noalloc: unknown path";
+    return FS_E_ARG;
+  }
+  if ((rc = ensure_init())) return rc;
+  double* base = g_device < kMaxDevices ? g_scratch_ptr[g_device] : nullptr;
+  if (!base) {
+    set_error("hipGetSymbolAddress(scratch)",
+              g_device < kMaxDevices && g_scratch_err[g_device] != hipSuccess ? g_scratch_err[g_device]
+                                                                              : hipErrorInvalidSymbol);
+    return FS_E_HIP;
+  }
+  const uint32_t g = cand_grid(c);
+  if (path != FS_PATH_MONT) {
+    // multipliers n0+1 .. and their pairs, 32 entries past the range: the
+    // loops read one unrolled block ahead (at most 16 multipliers or 16 pair
+    // words); count + 32 <= kScrCap keeps every entry inside the array
+    const uint32_t len = (n1 - n0) + 32;
+    k_fill_ntab<<<(len + 255) / 256, 256, 0, S(s)>>>(base, base + kScrCap, kScrCap / 2 + 1, n0 + 1, len);
+    LAUNCHCHK("iterate_noalloc tables");
+  }
+  switch (path) {
+    case FS_PATH_FP64:
+      k_iterate_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, base, base + kScrCap);
+      break;
+    case FS_PATH_HYBRID:
+      k_iterate_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, base);
+      break;
+    default:
+      k_iterate_mont<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1);
+      break;
+  }
+  LAUNCHCHK("iterate_noalloc");
   return FS_OK;
 }
 

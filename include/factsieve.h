@@ -149,6 +149,20 @@ int fs_setup(const fs_cands* c, const uint32_t* d_ctr, const void* d_prog, uint3
  * (p, +n) if n! = -1 (mod p), then res <- res * (n+1).  Bounded factor append. */
 int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t factor_cap,
                uint32_t startN, uint32_t endN, int path, fs_stream s);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fs_iterate with its multiplier and pair tables in a
+ * static device array of the library, refilled by a small kernel on stream s
+ * at every call, instead of a table that fs_iterate allocates per stream.  The
+ * array holds tables for at most FS_ITERATE_NOALLOC_MAX_N n-values per call
+ * (about 24 MiB, one copy on every GPU the library runs on, held until the
+ * process ends); endN - startN above that returns FS_E_ARG and launches
+ * nothing.  Same arguments, results and errors as fs_iterate otherwise.  Every
+ * call on one GPU rewrites the same array and reads it while its kernel runs:
+ * calls that overlap on one GPU (other streams or threads) race. */
+#define FS_ITERATE_NOALLOC_MAX_N (1u << 20)
+int fs_iterate_noalloc(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t factor_cap,
+                       uint32_t startN, uint32_t endN, int path, fs_stream s);
 /* check.cl:50-109: fold mont(nmax!) + mont(nmax) into d_sum[1], count into
  * d_sum[0], track the max count, validate every residue reached nmax. */
 int fs_check(const fs_cands* c, uint32_t* d_ctr, uint64_t* d_sum, uint32_t nmax, fs_stream s);
