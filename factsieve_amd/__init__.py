@@ -149,6 +149,7 @@ _SIGS = [
     ("fs_hostemu_mulmod_hi", ctypes.c_int, [u64, i64, u32, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
     ("fs_hostemu_fp_pair", ctypes.c_int, [u64, i64, u32, ctypes.POINTER(i64), ctypes.POINTER(u64),
                                           ctypes.POINTER(ctypes.c_int)]),
+    ("fs_hostemu_sh_pair", ctypes.c_int, [u64, i64, u32, ctypes.POINTER(i64), ctypes.POINTER(u64)]),
     ("fs_hostemu_setup_mul", ctypes.c_int, [u64, u64, u64, ctypes.c_int, ctypes.POINTER(u64), ctypes.POINTER(u64),
                                             ctypes.POINTER(u32)]),
     ("fs_default_options", None, [ctypes.POINTER(Options)]),
@@ -320,11 +321,19 @@ def hostemu_mulmod_hi(p, s, m):
 
 
 def hostemu_fp_pair(p, s, a):
-    """One FP64 paired step s*a*(a+1) mod p (128 (a+1)^2 <= 63 p) on the host:
+    """One FP64 paired step s*a*(a+1) mod p (64 (a+1) <= 31 p) on the host:
     (raw signed value, canonical, whether the exact-product form ran)."""
     raw, res, ex = i64(), u64(), ctypes.c_int()
     check(lib().fs_hostemu_fp_pair(p, s, a, ctypes.byref(raw), ctypes.byref(res), ctypes.byref(ex)))
     return raw.value, res.value, bool(ex.value)
+
+
+def hostemu_sh_pair(p, s, a):
+    """One hybrid paired step s*a*(a+1) mod p (a + 1 <= 2^22, 2^45 <= p <
+    2^64 - 2^58) on the host: (raw signed value, canonical)."""
+    raw, res = i64(), u64()
+    check(lib().fs_hostemu_sh_pair(p, s, a, ctypes.byref(raw), ctypes.byref(res)))
+    return raw.value, res.value
 
 
 def hostemu_setup_mul(p, s, m, square=False):

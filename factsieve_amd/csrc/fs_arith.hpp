@@ -238,6 +238,49 @@ FS_HD int64_t fs_sh_mulmod_hi(int64_t s, uint32_t m, double md, uint64_t c31, ui
   return (int64_t)(((uint64_t)hi << 32) | (uint32_t)b);
 }
 
+// Paired signed step: s*M mod p for M = a*(a+1), the product of two
+// consecutive multipliers a, a+1 <= n1, for |s| <= 0.51 p (fs_sh_pair_ok:
+// n1 <= 2^22, so M < 2^44, and 2^45 <= p < 2^64 - 2^58).  y is the
+// estimate of s/p that the previous step left (fs_sh_estimate): within
+// 2^-52 |s/p| + 2^-21/p of it (two roundings of the sum and one of
+// fl(1/p), plus the rounding of s0 * fl(1/p) with s0 < 2^32), so y*M misses
+// X = s*M/p by less than 2^44 * 2^-52 * 0.51 + 2^44 * 2^-21 / 2^45 < 0.0022.
+// kd = fma(y, M, 1.5 * 2^52) is
+// 1.5 * 2^52 + k with k = round(y*M), |k| < 2^51, exactly (spacing 1 there),
+// so the bits of kd are 0x4338000000000000 + k: k mod 2^32 is their low word
+// k0 and floor(k / 2^32) their high word kb1 minus 0x43380000.  r = s*M - k*p
+// then has |r| <= (1/2 + 0.0022) p < 0.989 * 2^63 and is formed exactly mod 2^64 as
+// s*M + k*pn (pn = 2^64 - p) from two 32x32->64 multiply-adds and four 32-bit
+// products: k*pn = k0*pn + kb1*pn0*2^32 - 0x43380000*pn0*2^32, the last term
+// a per-modulus constant folded into the first multiply-add's addend
+// (fs_sh_pair_c).  MS0, MS1 are the words of M (scalar).  *yr = the estimate
+// of r/p for the next step: nine operations for the step and four for the
+// estimate.  Because a+1 <= 2^22 < 0.49 p, a result congruent to +-(a+1), as
+// it is when the skipped residue (that of s times a) is +-1, is exactly
+// +-(a+1).
+#define FS_SH_PAIR_PLIM (0xFFFFFFFFFFFFFFFFull - ((uint64_t)1 << 58) + 1)  // 2^64 - 2^58, exclusive
+FS_HD bool fs_sh_pair_ok(uint64_t p, uint32_t n1) {
+  return n1 <= (1u << 22) && p >= ((uint64_t)1 << 45) && p < FS_SH_PAIR_PLIM;
+}
+FS_HD uint64_t fs_sh_pair_c(uint64_t pn) { return (uint64_t)(0u - 0x43380000u * (uint32_t)pn) << 32; }
+FS_HD double fs_sh_estimate(int64_t s, double pinv, double pinv32) {
+#pragma clang fp contract(off)
+  return __builtin_fma((double)(int32_t)((uint64_t)s >> 32), pinv32, (double)(uint32_t)s * pinv);
+}
+FS_HD int64_t fs_sh_mulmod_pair(int64_t s, double y, uint32_t MS0, uint32_t MS1, double Md, uint64_t pn,
+                                uint64_t cpair, double pinv, double pinv32, double* yr) {
+#pragma clang fp contract(off)
+  const double kd = __builtin_fma(y, Md, 6755399441055744.0);  // 1.5 * 2^52
+  const uint64_t kb = __builtin_bit_cast(uint64_t, kd);
+  const uint32_t k0 = (uint32_t)kb, kb1 = (uint32_t)(kb >> 32);
+  const uint32_t s0 = (uint32_t)s, s1 = (uint32_t)((uint64_t)s >> 32);
+  const uint64_t u = (uint64_t)k0 * (uint32_t)pn + ((uint64_t)s0 * MS0 + cpair);
+  const uint32_t hi = (uint32_t)(u >> 32) + s1 * MS0 + s0 * MS1 + k0 * (uint32_t)(pn >> 32) + kb1 * (uint32_t)pn;
+  const int64_t r = (int64_t)(((uint64_t)hi << 32) | (uint32_t)u);
+  *yr = fs_sh_estimate(r, pinv, pinv32);
+  return r;
+}
+
 // Symmetric signed residue from a canonical one, and back.
 FS_HD int64_t fs_canon_to_signed(uint64_t r, uint64_t p) { return r > (p >> 1) ? (int64_t)(r - p) : (int64_t)r; }
 FS_HD uint64_t fs_signed_to_canon(int64_t s, uint64_t p) { return s < 0 ? (uint64_t)s + p : (uint64_t)s; }

@@ -173,7 +173,9 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bsieve(const fs_bjob* __restrict__
           s = fs_cand_iter_fp64<U, (U == 8 ? 16 : U)>(p, s, n0, n1, nt, pt, pairs, rec);
         } else if (J.path == FS_PATH_HYBRID) {
           s = first ? fs_cand_setup_hybrid(p, q, r2, 0, jp, J.prog_terms, 0, J.prog_terms) : c.res[g];
-          s = fs_cand_iter_hybrid<U>(p, s, n0, n1, nt, rec);
+          // paired hybrid blocks of U steps (16 would hold 64 scalar registers of
+          // pair words and push the kernel to 5 waves per SIMD)
+          s = fs_cand_iter_hybrid<U, U>(p, s, n0, n1, nt, pt, pairs, rec);
         } else {
           s = first ? fs_cand_setup_mont(p, q, r2, 0, jp, J.prog_terms, 0, J.prog_terms) : c.res[g];
           s = fs_cand_iter_mont(p, q, r2, s, n0, n1, rec);

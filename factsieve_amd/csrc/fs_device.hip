@@ -147,12 +147,13 @@ __global__ __launch_bounds__(FS_BLOCK) void k_iterate_fp64(fs_cands c, uint32_t*
 
 __global__ __launch_bounds__(FS_BLOCK) void k_iterate_hybrid(fs_cands c, uint32_t* ctr, fs_factor* fac,
                                                              uint32_t fcap, uint32_t n0, uint32_t n1,
-                                                             const double* __restrict__ nt) {
+                                                             const double* __restrict__ nt,
+                                                             const double* __restrict__ pt) {
   const uint32_t i = blockIdx.x * FS_BLOCK + threadIdx.x;
   if (i >= fs_live(ctr, c.cap)) return;
   const uint64_t p = c.p[i];
   if (c.nnext[i] != n0) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
-  c.res[i] = fs_cand_iter_hybrid(p, c.res[i], n0, n1, nt, FS_REC(p));
+  c.res[i] = fs_cand_iter_hybrid<FS_UNROLL_DEFAULT, 16>(p, c.res[i], n0, n1, nt, pt, true, FS_REC(p));
   c.nnext[i] = n1;
 }
 
@@ -761,9 +762,9 @@ extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_facto
       break;
     }
     case FS_PATH_HYBRID: {
-      const double* nt = nullptr;
-      if ((rc = ntab_for(S(s), n0 + 1, n1 - n0, &nt))) return rc;
-      k_iterate_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, nt);
+      const double *nt = nullptr, *pt = nullptr;
+      if ((rc = ntab_for(S(s), n0 + 1, n1 - n0, &nt, &pt))) return rc;
+      k_iterate_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, nt, pt);
       break;
     }
     case FS_PATH_MONT:
@@ -819,7 +820,7 @@ extern "C" int fs_iterate_noalloc(const fs_cands* c, uint32_t* d_ctr, fs_factor*
       k_iterate_fp64<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, base, base + kScrCap);
       break;
     case FS_PATH_HYBRID:
-      k_iterate_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, base);
+      k_iterate_hybrid<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1, base, base + kScrCap);
       break;
     default:
       k_iterate_mont<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, n0, n1);

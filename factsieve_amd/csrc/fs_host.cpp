@@ -946,9 +946,11 @@ uint32_t fs_auto_segment_candidates() {
 // a launch that ends at such a point runs the n below it with that loop.
 // FP64: paired steps while 64 n1 <= 31 p and n1 <= 2^22 (fs_fp_pair_ok, smallest
 // p), paired with the exact product while p n1^2 <= 2^53, single steps with the
-// exact product while p n1 <= 2^53 (largest p); hybrid: the high-word quotient
-// while n1 <= fs_sh_hi_mmax(p) (p >> 34 below 2^63, p >> 39 above; the
-// smallest over the segment is at plo or, past 2^63, at max(plo, 2^63)).
+// exact product while p n1 <= 2^53 (largest p); hybrid: paired steps while
+// n1 <= 2^22 (fs_sh_pair_ok, every p of the segment in [2^45, 2^64 - 2^58)),
+// then the high-word quotient while n1 <= fs_sh_hi_mmax(p) (p >> 34 below 2^63,
+// p >> 39 above; the smallest over the segment is at plo or, past 2^63, at
+// max(plo, 2^63)).
 static std::vector<uint32_t> loop_boundaries(int path, uint64_t plo, uint64_t phi, uint32_t nmin, uint32_t nmax) {
   std::vector<uint32_t> b;
   auto add = [&](uint64_t n) {
@@ -966,11 +968,15 @@ static std::vector<uint32_t> loop_boundaries(int path, uint64_t plo, uint64_t ph
     add(pair);
     add(std::min<uint64_t>(pair, isqrt(((uint64_t)1 << 53) / ptop)));
     add(((uint64_t)1 << 53) / ptop);
-  } else if (path == FS_PATH_HYBRID && ptop < FS_SH_HI_PMAX) {
-    const uint64_t p63 = (uint64_t)1 << 63;
-    uint64_t hi = fs_sh_hi_mmax(plo);
-    if (ptop >= p63) hi = std::min(hi, fs_sh_hi_mmax(std::max(plo, p63)));
-    add(hi);
+  } else if (path == FS_PATH_HYBRID) {
+    const bool pairs = fs_sh_pair_ok(plo, 1) && fs_sh_pair_ok(ptop, 1);
+    if (pairs) add((uint64_t)1 << 22);
+    if (ptop < FS_SH_HI_PMAX) {
+      const uint64_t p63 = (uint64_t)1 << 63;
+      uint64_t hi = fs_sh_hi_mmax(plo);
+      if (ptop >= p63) hi = std::min(hi, fs_sh_hi_mmax(std::max(plo, p63)));
+      if (!pairs || hi > ((uint64_t)1 << 22)) add(hi);
+    }
   }
   std::sort(b.begin(), b.end());
   b.erase(std::unique(b.begin(), b.end()), b.end());

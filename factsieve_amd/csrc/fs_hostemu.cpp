@@ -76,6 +76,20 @@ extern "C" int fs_hostemu_fp_pair(uint64_t p, int64_t s, uint32_t a, int64_t* ra
   return FS_OK;
 }
 
+extern "C" int fs_hostemu_sh_pair(uint64_t p, int64_t s, uint32_t a, int64_t* raw, uint64_t* res) {
+  if ((p & 1) == 0 || a == 0 || a >= (1u << 22) || !fs_sh_pair_ok(p, a + 1) || !raw || !res) return FS_E_ARG;
+  const uint64_t as = s < 0 ? (uint64_t)0 - (uint64_t)s : (uint64_t)s;
+  if ((double)as > 0.51 * (double)p) return FS_E_ARG;
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+  const uint64_t pn = 0 - p, M = (uint64_t)a * (a + 1);
+  double y;
+  const int64_t r = fs_sh_mulmod_pair(s, fs_sh_estimate(s, pinv, pinv32), (uint32_t)M, (uint32_t)(M >> 32),
+                                      (double)M, pn, fs_sh_pair_c(pn), pinv, pinv32, &y);
+  *raw = r;
+  *res = fs_signed_to_canon(r, p);
+  return FS_OK;
+}
+
 extern "C" int fs_hostemu_setup_mul(uint64_t p, uint64_t s, uint64_t m, int square, uint64_t* raw, uint64_t* res,
                                     uint32_t* shift) {
   if ((p & 1) == 0 || p < 3 || p > FS_HYBRID_PMAX || m >= ((uint64_t)1 << 60) || !raw || !res || !shift)
@@ -270,6 +284,56 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
       s = step(s, n + 1);
     }
     *res = fs_fp_to_canon(s, p);
+  } else if (path == FS_PATH_HYBRID && fs_sh_pair_ok(p, n1)) {
+    // The paired loop: one product by M = a*(a+1) per two positions, each
+    // leaving the estimate y of its result over p; a block's filter is
+    // |y| <= 1.5 (n1 + 2)/p over its input and its U/2 results.  Here each
+    // pair is also checked against two single steps and the loop's residue
+    // bound, and a +-1 residue in a block whose filter does not fire is an
+    // error.
+    const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+    const double thr = 1.5 * (double)(n1 + 2) * pinv;
+    const uint64_t pn = 0 - p, cpair = fs_sh_pair_c(pn), c31 = p << 31;
+    int64_t s = fs_canon_to_signed(*res, p);
+    double ys = fs_sh_estimate(s, pinv, pinv32);
+    uint32_t n = n0;
+    for (; n < nu; n += U) {
+      int64_t v[U / 2 + 1];
+      double y[U / 2 + 1];
+      v[0] = s;
+      y[0] = ys;
+      bool fire = __builtin_fabs(y[0]) <= thr;
+      for (uint32_t i = 0; i < U / 2; ++i) {
+        const uint64_t a = n + 1 + 2 * i, M = a * (a + 1);
+        v[i + 1] = fs_sh_mulmod_pair(v[i], y[i], (uint32_t)M, (uint32_t)(M >> 32), (double)M, pn, cpair, pinv,
+                                     pinv32, &y[i + 1]);
+        if (__builtin_fabs((double)v[i + 1]) > 0.51 * (double)p) return FS_E_VALIDATE;  // the loop's bound
+        fire |= __builtin_fabs(y[i + 1]) <= thr;
+        double yu;
+        const int64_t u = fs_sh_mulmod(v[i], (uint32_t)a, (double)a, c31, pn, pinv, pinv32, &yu);
+        const int64_t two = fs_sh_mulmod(u, (uint32_t)(a + 1), (double)(a + 1), c31, pn, pinv, pinv32, &yu);
+        if (fs_signed_to_canon(two, p) != fs_signed_to_canon(v[i + 1], p)) return FS_E_VALIDATE;
+      }
+      for (uint32_t i = 0; i < U / 2; ++i) {
+        const uint32_t nn = n + 2 * i;
+        double yu;
+        const int64_t u = fs_sh_mulmod(v[i], nn + 1, (double)(nn + 1), c31, pn, pinv, pinv32, &yu);
+        const int64_t r[2] = {v[i], u};  // residues of nn and nn + 1
+        for (uint32_t k = 0; k < 2; ++k) {
+          if ((r[k] != 1) & (r[k] != -1)) continue;
+          if (!fire) return FS_E_VALIDATE;  // the filter would have missed it
+          rec(r[k] == 1 ? -(int32_t)(nn + k) : (int32_t)(nn + k));
+        }
+      }
+      s = v[U / 2];
+      ys = y[U / 2];
+    }
+    for (; n < n1; ++n) {
+      if ((s == 1) | (s == -1)) rec(s == 1 ? -(int32_t)n : (int32_t)n);
+      double y;
+      s = fs_sh_mulmod(s, n + 1, (double)(n + 1), c31, pn, pinv, pinv32, &y);
+    }
+    *res = fs_signed_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {
     if (p > FS_HYBRID_PMAX) return FS_E_ARG;
     const bool hi = fs_sh_hi_ok(p, n1);  // the device loops' choice (per wave there)

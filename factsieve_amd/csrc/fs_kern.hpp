@@ -559,10 +559,96 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_loop(uint64_t p, uint64_
   return fs_signed_to_canon(s, p);
 }
 
-// Per wave: the high-word quotient when every active lane's modulus allows it.
-template <int U = FS_UNROLL_DEFAULT, class Rec>
+// Hybrid path, paired steps (fs_sh_pair_ok(p, n1)): one product by
+// M = a*(a+1) per two multipliers (fs_sh_mulmod_pair), the words of M formed
+// on the scalar unit from the uniform a, its double read from the FP64 pair
+// table pt (as for the FP64 path, pt[2j] = (n0+1+2j)*(n0+2+2j)), with the
+// same double-buffered scalar loads.  Each step leaves the estimate y of its
+// result over p, which the next step needs for its quotient; the block's
+// filter is |y| <= 1.5 (n1 + 2)/p over its input and all H results, which
+// holds whenever one of them is +-1 or +-(a+1), i.e. whenever a residue of
+// positions nb - 1 .. nb + U is +-1 (by the estimate's error bound).  The
+// taken branch tests positions nb .. nb + U - 1 exactly, the odd ones by one
+// single step each; the positions just outside are the neighbouring blocks'.
+// Per two steps: thirteen operations and one of the filter, plus two per block.
+template <int U, class Rec>
+__device__ __forceinline__ uint64_t fs_cand_iter_hybrid_pair_loop(uint64_t p, uint64_t s_in, uint32_t n0,
+                                                                  uint32_t n1, const double* __restrict__ nt,
+                                                                  const double* __restrict__ pt, Rec rec) {
+  static_assert(U == 2 || U == 4 || U == 8 || U == 16, "unroll depth");
+  constexpr int H = U / 2;
+  const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
+  const double thr = 1.5 * (double)(n1 + 2) * pinv;
+  const uint64_t pn = 0 - p, c31 = p << 31, cpair = fs_sh_pair_c(pn);
+  int64_t s = fs_canon_to_signed(s_in, p);
+  double ys = fs_sh_estimate(s, pinv, pinv32);
+  uint32_t n = n0;
+  const uint32_t nu = n0 + ((n1 - n0) & ~(uint32_t)(U - 1));
+  // The block keeps only its input and the running minimum of |y| while it
+  // steps; the rare taken branch recomputes the block's residues from its
+  // input with single steps (fs_sh_mulmod: other representatives of the same
+  // residues, within 0.51 p, so the same +-1 tests), which keeps the loop's
+  // registers low.
+  auto block = [&](const double (&w)[U], uint32_t nb) {
+    const int64_t sb = s;
+    uint32_t a = nb + 1;
+    asm volatile("" : "+s"(a));  // a uniform 32-bit scalar: the words of M on the scalar unit
+    double ymin = __builtin_fabs(ys);
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      const uint32_t ai = a + 2 * i;
+      const uint64_t M = (uint64_t)ai * (ai + 1);
+      s = fs_sh_mulmod_pair(s, ys, (uint32_t)M, (uint32_t)(M >> 32), w[2 * i], pn, cpair, pinv, pinv32, &ys);
+      ymin = __builtin_fmin(ymin, __builtin_fabs(ys));
+    }
+    if (__ballot(ymin <= thr)) {  // whole wave: exact tests where any lane's filter passed
+      int64_t v = sb;
+#pragma unroll 1
+      for (uint32_t nn = nb; nn < nb + U; ++nn) {
+        if ((v == 1) | (v == -1)) rec(v == 1 ? -(int32_t)nn : (int32_t)nn);
+        double yu;
+        v = fs_sh_mulmod(v, nn + 1, nt[nn - n0], c31, pn, pinv, pinv32, &yu);
+      }
+    }
+  };
+  double wa[U], wb[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) wa[k] = pt[k];
+  while (n < nu) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" ::"s"(wa[k]));
+#pragma unroll
+    for (int k = 0; k < U; ++k) wb[k] = pt[(n - n0) + U + k];
+    asm volatile("" : "+v"(s)::"memory");
+    block(wa, n);
+    n += U;
+    if (n >= nu) break;
+#pragma unroll
+    for (int k = 0; k < U; ++k) asm volatile("" ::"s"(wb[k]));
+#pragma unroll
+    for (int k = 0; k < U; ++k) wa[k] = pt[(n - n0) + U + k];
+    asm volatile("" : "+v"(s)::"memory");
+    block(wb, n);
+    n += U;
+  }
+  for (; n < n1; ++n) {
+    if ((s == 1) | (s == -1)) rec(s == 1 ? -(int32_t)n : (int32_t)n);
+    double y;
+    s = fs_sh_mulmod(s, n + 1, nt[n - n0], c31, pn, pinv, pinv32, &y);
+  }
+  return fs_signed_to_canon(s, p);
+}
+
+// Per wave: paired steps when the pair table covers the launch (pairs) and
+// every active lane's modulus allows them, else the high-word quotient when
+// every active lane's modulus allows it, else the full quotient.  pt is always a valid table
+// pointer (never selected against a null pointer), which keeps its loads
+// scalar.
+template <int U = FS_UNROLL_DEFAULT, int UP = U, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_hybrid(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
-                                                        const double* __restrict__ nt, Rec rec) {
+                                                        const double* __restrict__ nt, const double* __restrict__ pt,
+                                                        bool pairs, Rec rec) {
+  if (pairs && __all(fs_sh_pair_ok(p, n1))) return fs_cand_iter_hybrid_pair_loop<UP>(p, s_in, n0, n1, nt, pt, rec);
   if (__all(fs_sh_hi_ok(p, n1))) return fs_cand_iter_hybrid_loop<U, true>(p, s_in, n0, n1, nt, rec);
   return fs_cand_iter_hybrid_loop<U, false>(p, s_in, n0, n1, nt, rec);
 }

@@ -238,6 +238,11 @@ int fs_hostemu_mulmod_hi(uint64_t p, int64_t s, uint32_t m, int64_t* raw, uint64
  * 64 (a+1) <= 31 p, a + 1 <= 2^22, odd p < 2^53 (FS_E_ARG otherwise); *exact
  * tells which form ran; |raw| <= 0.503p. */
 int fs_hostemu_fp_pair(uint64_t p, int64_t s, uint32_t a, int64_t* raw, uint64_t* res, int* exact);
+/* The HYBRID paired step, one product by a*(a+1) with the quotient from the
+ * estimate of s/p, used when every pair a+1 <= n1 of a launch has n1 <= 2^22
+ * and 2^45 <= p < 2^64 - 2^58: |s| <= 0.51p, 1 <= a, a + 1 <= 2^22, odd p in
+ * that range (FS_E_ARG otherwise); |raw| <= 0.503p. */
+int fs_hostemu_sh_pair(uint64_t p, int64_t s, uint32_t a, int64_t* raw, uint64_t* res);
 /* One multiply (square = 0) or square (square = 1, m ignored) of the HYBRID
  * setup program, s*m*2^-k mod p: below 2^64/3 a multiply reduces by k = 62 and
  * a square by k = 64, both lazily (s < 2p, result < 2p); below 2^63 both by

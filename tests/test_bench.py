@@ -64,3 +64,20 @@ def test_hybrid_hi_model_matches_kernel_loop(fs):
 def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, bench.__file__, "--gpus", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_hybrid_pair_model_matches_kernel_loop(fs):
+    """bench.hybrid_pair against the predicate of the kernels' paired hybrid
+    loop, reached through the host execution of one paired step (a + 1 = n1)."""
+    rng = random.Random(71)
+    ps = [2 ** 45 - 1, 2 ** 45 + 1, 2 ** 53 + 5, 2 ** 63 + 29, 2 ** 64 - 2 ** 58 - 1, 2 ** 64 - 2 ** 58 + 1,
+          2 ** 64 - 2 ** 46 - 3]
+    ps += [rng.randrange(2 ** 40, 2 ** 64 - 2 ** 46) | 1 for _ in range(40)]
+    for p in ps:
+        for n1 in (2, 1000, 2 ** 22 - 1, 2 ** 22, 2 ** 22 + 1, rng.randrange(2, 2 ** 23)):
+            try:
+                fs.hostemu_sh_pair(p, 5, n1 - 1)
+                c_pair = True
+            except fs.FactSieveError:
+                c_pair = False
+            assert bench.hybrid_pair(p, p + 1, n1) == c_pair, (p, n1)

@@ -455,18 +455,22 @@ def reference_order(factors):
 
 
 @pytest.mark.parametrize("lo,n0,note", [
-    (10 ** 12, 100000, "inexact pairs, a(a+1) < p"),
-    (10 ** 8, 5000, "exact pairs (p n1^2 <= 2^53)"),
-    (2 ** 32 + 1, 300000, "inexact pairs, a(a+1) far above p"),
+    (10 ** 12, 100000, "FP64 inexact pairs, a(a+1) < p"),
+    (10 ** 8, 5000, "FP64 exact pairs (p n1^2 <= 2^53)"),
+    (2 ** 32 + 1, 300000, "FP64 inexact pairs, a(a+1) far above p"),
+    (2 ** 62, 500000, "hybrid pairs"),
+    (2 ** 53 + 2 ** 20, 3000, "hybrid pairs just above 2^53"),
+    (2 ** 64 - 2 ** 58 - 10 ** 6, 2 ** 22 - 64, "hybrid pairs at the top of their range"),
 ])
 def test_paired_loop_planted_every_position(gpu, lo, n0, note):
-    """The paired FP64 iterate kernel (16 steps per block, one product by
-    (n+1)(n+2) per two steps, ballots of |v| == 1 and |v'| == n+2) from start
-    residues that put +-1 at every position of a block, both signs, in three
-    consecutive blocks and the tail; end residues and factor lists against
-    the oracle."""
+    """The paired iterate kernels (16 steps per block, one product by
+    (n+1)(n+2) per two steps; FP64: ballots of |v| == 1 and |v'| == n+2;
+    hybrid: a filter on the quotient estimates) from start residues that put
+    +-1 at every position of a block, both signs, in three consecutive blocks
+    and the tail; end residues and factor lists against the oracle."""
     n1 = n0 + 3 * 16 + 7
-    assert 64 * n1 <= 31 * lo
+    path = gpu.PATH_FP64 if lo + 60000 <= 2 ** 53 else gpu.PATH_HYBRID
+    assert 64 * n1 <= 31 * lo and (path == gpu.PATH_FP64 or n1 <= 2 ** 22)
     seg = gpu.Segment(cap=8192)
     try:
         print(f"planted pairs: p from {lo}, n=[{n0},{n1}) ({note})", flush=True)
@@ -491,7 +495,7 @@ def test_paired_loop_planted_every_position(gpu, lo, n0, note):
         assert len(covered) == 2 * (n1 - n0)      # every position, both signs
         seg.write_u64("res", starts)
         seg.write_u32("nnext", [n0] * n)
-        seg.iterate(n0, n1, gpu.PATH_FP64)
+        seg.iterate(n0, n1, path)
         res = seg.read_u64("res", n)
         facs = seg.factors()
         assert seg.counters()[gpu.CTR_VALIDATE] == 0

@@ -387,6 +387,76 @@ def test_fp64_pair_loop_matches_oracle(fs):
                 assert (r, facs[:nf]) == (want_r, want_f), (p, n0, n1, start)
 
 
+HYBRID_PMAX = 2 ** 64 - 2 ** 46
+PAIR_PLIM = 2 ** 64 - 2 ** 58   # the hybrid paired step takes p below this
+
+
+def test_hybrid_pair_step_at_error_bound_edges(fs):
+    """The hybrid paired step (one product by a*(a+1), quotient from the
+    estimate of s/p, used when n1 <= 2^22 and 2^45 <= p < 2^64 - 2^58): exact
+    and |s'| <= 0.503p at the largest pair (a + 1 = 2^22), the smallest and
+    largest moduli, |s| up to 0.51p and low words up to 2^32 - 1; a residue
+    whose product with a is +-1 gives exactly +-(a+1), which the loop's
+    compare relies on."""
+    rng = random.Random(61)
+    ps = [2 ** 45 + 59, 2 ** 45 + 1, 2 ** 53 + 5, 2 ** 62 + 135, 2 ** 63 - 25, 2 ** 63 + 29, PAIR_PLIM - 1,
+          PAIR_PLIM - 3]
+    ps += [rng.randrange(2 ** 45, PAIR_PLIM) | 1 for _ in range(40)]
+    for p in ps:
+        smax = min(int(0.51 * p), 2 ** 63 - 1) - 2 ** 12
+        As = [2 ** 22 - 1, 2 ** 22 - 2, 1, 2] + [rng.randrange(1, 2 ** 22) for _ in range(30)]
+        s0max = (((smax >> 32) - 1) << 32) | 0xFFFFFFFF
+        ss = [smax, -smax, 1, -1, 0, s0max, -s0max] + [rng.randrange(-smax, smax + 1) for _ in range(30)]
+        for s in ss:
+            for a in As[:4] + [As[4 + (s % 30)]]:
+                raw, res = fs.hostemu_sh_pair(p, s, a)
+                assert res == (s * a * (a + 1)) % p, (p, s, a)
+                assert raw % p == res and abs(raw) <= 0.503 * p, (p, s, a, raw)
+        for a in As[:8]:
+            if math.gcd(a, p) != 1:
+                continue
+            inv = pow(a, -1, p)
+            for sign in (1, -1):
+                s = sign * inv % p
+                s = s - p if s > p // 2 else s
+                raw, _ = fs.hostemu_sh_pair(p, s, a)
+                assert raw == sign * (a + 1), (p, a, sign, raw)
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_sh_pair(2 ** 45 - 1, 5, 1000)          # modulus below 2^45
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_sh_pair(2 ** 60 + 33, 5, 2 ** 22)      # a + 1 above 2^22
+    with pytest.raises(fs.FactSieveError):
+        fs.hostemu_sh_pair(PAIR_PLIM + 1, 5, 1000)        # |s'| could reach 2^63
+
+
+def test_hybrid_pair_loop_matches_oracle(fs):
+    """The hybrid iterate with paired steps against the oracle, from random
+    residues and from residues that put +-1 at every position of an 8-step
+    block, at moduli from 2^45 to the top of the hybrid range (single steps
+    from 2^64 - 2^58) and n1 up to 2^22 (where the paired loop stops).  The host execution also returns an
+    error if a pair differs from two single steps, a residue leaves 0.51p, or
+    a block's filter misses a +-1."""
+    rng = random.Random(67)
+    cases = [(2 ** 45 + 59, 1040), (2 ** 53 + 5, 20001), (2 ** 58 + 27, 500000), (2 ** 62 + 135, 300000),
+             (2 ** 63 + 29, 2 ** 22), (PAIR_PLIM - 1, 2 ** 22), (PAIR_PLIM + 1, 77777), (HYBRID_PMAX - 3, 7777),
+             (2 ** 50 + 55, 2 ** 22 - 5)]
+    cases += [(rng.randrange(2 ** 45, HYBRID_PMAX) | 1, rng.randrange(40, 2 ** 22)) for _ in range(10)]
+    for p, n1 in cases:
+        for n0 in (max(1, n1 - 8 * 4 - 3), max(1, n1 - 17)):
+            starts = [rng.randrange(p) for _ in range(2)]
+            for pos in range(8):
+                prod = 1
+                for j in range(1, pos + 1):
+                    prod = prod * (n0 + j) % p
+                if math.gcd(prod, p) == 1:
+                    for sign in (1, -1):
+                        starts.append(sign * pow(prod, -1, p) % p)
+            for start in starts:
+                r, facs, nf = fs.hostemu_iterate(fs.PATH_HYBRID, p, n0, n1, start)
+                want_r, want_f = oracle_iterate(p, n0, n1, start)
+                assert (r, facs[:nf]) == (want_r, want_f), (p, n0, n1, start)
+
+
 def hi_mmax(p):
     """Largest multiplier the hybrid high-word step takes for p: p >> 34 below
     2^63 (|s'| <= 3/4 p + 2^-21 p), p >> 39 up to 2^64 - 2^58 (|s'| <= 0.51p)."""
