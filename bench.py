@@ -168,7 +168,25 @@ def bench_batched(fs, njobs=64, seed=None):
         {tuple(j): r for j, r in zip(jobs, res)}
 
 
-def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192):
+def bench_graph_replay(fs, reps=300):
+    """Per-call wall time of a tiny batch (one job of ~260 candidates and 100
+    n) with its launches replayed from the cached graph against enqueued one
+    by one: the launch cost a quantum saves with the graph."""
+    job = [(10 ** 8, 10 ** 8 + 5000, 1000, 1100)]
+    out = {}
+    for graph in (1, 0):
+        for _ in range(20):
+            fs.batch(job, use_graph=graph)
+        t = time.perf_counter()
+        for _ in range(reps):
+            fs.batch(job, use_graph=graph)
+        out["graph_ms" if graph else "direct_ms"] = (time.perf_counter() - t) / reps * 1e3
+    out["saving_us_per_launch"] = (out["direct_ms"] - out["graph_ms"]) * 1e3
+    out["workload"] = "one job p in [1e8, 1e8+5000), n in [1000, 1100), %d calls each" % reps
+    return out
+
+
+def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192, use_graph=1):
     """Poisson arrivals of mixed workunits into the continuous-batching server;
     p50/p99 arrival-to-result latency and aggregate steps/s."""
     from factsieve_amd import workloads
@@ -177,8 +195,9 @@ def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192):
     cap = bench_serving.capacity_steps_per_s
     rate = rate_frac * cap / mean_steps
     arrivals = workloads.poisson_arrivals(rate, nwu)
-    srv = fs.Server(n_chunk=n_chunk)
+    srv = fs.Server(n_chunk=n_chunk, use_graph=use_graph)
     ids, lat, results = {}, {}, {}
+    steps_run, step_s = 0, 0.0
     t0 = time.perf_counter()
     nxt = 0
     try:
@@ -191,7 +210,10 @@ def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192):
             if st["active"] == 0 and st["queued"] == 0 and nxt < nwu:
                 time.sleep(max(0.0, arrivals[nxt] - (time.perf_counter() - t0)))
                 continue
+            ts = time.perf_counter()
             srv.step()
+            step_s += time.perf_counter() - ts
+            steps_run += 1
             now = time.perf_counter() - t0
             for wid, k in ids.items():
                 if k in results:
@@ -209,6 +231,7 @@ def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192):
                 reference[tuple(jobs[k])].checksum, reference[tuple(jobs[k])].factors) for k in results)
     ls = sorted(lat.values())
     return {"workunits": nwu, "arrival_rate_per_s": rate, "target_utilisation": rate_frac, "n_chunk": n_chunk,
+            "graph": bool(use_graph), "steps": steps_run, "mean_step_ms": step_s / max(steps_run, 1) * 1e3,
             "p50_latency_s": ls[len(ls) // 2], "p99_latency_s": ls[min(len(ls) - 1, int(0.99 * len(ls)))],
             "makespan_s": makespan, "steps_per_s": steps / makespan, "bit_exact_vs_batched_alone": same}
 
@@ -305,6 +328,8 @@ def main():
         line["batched"] = bat
         bench_serving.capacity_steps_per_s = bat["batched_steps_per_s"]
         line["serving"] = bench_serving(fs, ref)
+        line["serving_without_graph"] = bench_serving(fs, ref, use_graph=0)
+        line["graph_replay"] = bench_graph_replay(fs)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(pmin, nmin, nmax)
         line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]

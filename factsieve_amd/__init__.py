@@ -81,7 +81,8 @@ class BatchOptions(ctypes.Structure):
 
 
 class ServerOptions(ctypes.Structure):
-    _fields_ = [("mem_budget", u64), ("max_slots", u32), ("n_chunk", u32), ("verify_threads", ctypes.c_int)]
+    _fields_ = [("mem_budget", u64), ("max_slots", u32), ("n_chunk", u32), ("verify_threads", ctypes.c_int),
+                ("use_graph", u32)]
 
 
 class FactSieveError(RuntimeError):
@@ -597,10 +598,11 @@ def batch_autotune(jobs, **opts):
 class Server:
     """Continuous-batching serving loop over the C ABI (fs_server_*)."""
 
-    def __init__(self, mem_budget=0, max_slots=256, n_chunk=8192, verify_threads=16):
+    def __init__(self, mem_budget=0, max_slots=256, n_chunk=8192, verify_threads=16, use_graph=1):
         o = ServerOptions()
         lib().fs_default_server_options(ctypes.byref(o))
         o.mem_budget, o.max_slots, o.n_chunk, o.verify_threads = mem_budget, max_slots, n_chunk, verify_threads
+        o.use_graph = use_graph
         self.h = vp()
         check(lib().fs_server_create(ctypes.byref(o), ctypes.byref(self.h)))
 

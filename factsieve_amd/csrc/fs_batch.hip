@@ -45,17 +45,19 @@
 
 // Clears the counters of the jobs that start in this launch and the global words.
 __global__ void k_bclear(const fs_bjob* __restrict__ jobs, uint32_t* jcnt, uint32_t* jflag, uint64_t* jsum,
-                         uint32_t nj, uint32_t* glob) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < nj && (jobs[t].flags & FS_BJ_NEW)) {
-    jcnt[t] = 0;
-    jflag[t] = 0;
-    jsum[t] = 0;
+                         const uint32_t* __restrict__ params, uint32_t* glob) {
+  const uint32_t nj = params[FS_BP_NJ];
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nj; t += gridDim.x * blockDim.x) {
+    if (jobs[t].flags & FS_BJ_NEW) {
+      jcnt[t] = 0;
+      jflag[t] = 0;
+      jsum[t] = 0;
+    }
   }
-  if (t < FS_BG_WORDS) glob[t] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < FS_BG_WORDS) glob[threadIdx.x] = 0;
 }
 
-// Job of primegen workgroup b: last job with pg_first <= b.
+// Job of primegen virtual block b: last job with pg_first <= b.
 __device__ __forceinline__ uint32_t job_of_block(const fs_bjob* jobs, uint32_t nj, uint32_t b) {
   uint32_t lo = 0, hi = nj - 1;
   while (lo < hi) {
@@ -66,41 +68,55 @@ __device__ __forceinline__ uint32_t job_of_block(const fs_bjob* jobs, uint32_t n
   return lo;
 }
 
-__global__ __launch_bounds__(FS_BLOCK) void k_bprimegen(const fs_bjob* __restrict__ jobs, uint32_t nj, fs_cands c,
+// Candidate generation of every new job: the workgroups stride over the
+// FS_BP_PGBLOCKS virtual blocks; virtual block b belongs to the job whose
+// block range holds it and appends into that job's candidate region (bounded).
+__global__ __launch_bounds__(FS_BLOCK) void k_bprimegen(const fs_bjob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ params, fs_cands c,
                                                         uint32_t* jcnt, uint32_t* jflag) {
   __shared__ uint64_t s_tab[FS_NMASK];
   __shared__ uint64_t s_mask[FS_BLOCK];
   __shared__ uint32_t s_excl[FS_BLOCK];
   __shared__ uint32_t s_wsum[FS_BLOCK / 64 + 1];
+  const uint32_t nj = params[FS_BP_NJ], total = params[FS_BP_PGBLOCKS];
+  if (blockIdx.x >= total || nj == 0) return;  // uniform per workgroup
   fs_sieve_table_init(s_tab);
-  const uint32_t j = job_of_block(jobs, nj, blockIdx.x);
-  const fs_bjob J = jobs[j];
-  if (!(J.flags & FS_BJ_NEW)) return;  // uniform per workgroup
-  fs_primegen_block(
-      J.pmin | 1, J.pmax, blockIdx.x - J.pg_first, s_tab, s_mask, s_excl, s_wsum,
-      [&](uint32_t n) { return atomicAdd(&jcnt[j], n); },
-      [&](uint32_t idx, uint64_t N, uint64_t q, uint64_t r2) {
-        if (idx < J.cand_cap) {
-          const uint32_t g = J.cand_off + idx;
-          c.p[g] = N;
-          c.q[g] = q;
-          c.r2[g] = r2;
-          c.res[g] = 0;
-          c.nnext[g] = J.nmin;
-        } else {
-          atomicOr(&jflag[j], FS_BJF_CANDOVF);
-        }
-      });
+  for (uint32_t b = blockIdx.x; b < total; b += gridDim.x) {
+    const uint32_t j = job_of_block(jobs, nj, b);
+    const fs_bjob J = jobs[j];
+    if (J.flags & FS_BJ_NEW) {  // uniform per workgroup
+      fs_primegen_block(
+          J.pmin | 1, J.pmax, b - J.pg_first, s_tab, s_mask, s_excl, s_wsum,
+          [&](uint32_t n) { return atomicAdd(&jcnt[j], n); },
+          [&](uint32_t idx, uint64_t N, uint64_t q, uint64_t r2) {
+            if (idx < J.cand_cap) {
+              const uint32_t g = J.cand_off + idx;
+              c.p[g] = N;
+              c.q[g] = q;
+              c.r2[g] = r2;
+              c.res[g] = 0;
+              c.nnext[g] = J.nmin;
+            } else {
+              atomicOr(&jflag[j], FS_BJF_CANDOVF);
+            }
+          });
+    }
+    __syncthreads();  // the shared masks are rewritten by the next virtual block
+  }
 }
 
-// Tile list for the jobs of `order` (cost order), covering n in [n0, n1) of
-// each job as given by chunk[] (n0 = chunk[2k], n1 = chunk[2k+1] for job
-// order[k]); pref: nj+1 words of scratch.
+// Tile list of round r: the params-given entries of `order` (cost order),
+// covering n in [n0, n1) of each job as given by chunk[] (n0 = chunk[2k],
+// n1 = chunk[2k+1] for job order[k]); pref: entries+1 words of scratch.
 __global__ __launch_bounds__(FS_BLOCK) void k_bmake_tiles(const fs_bjob* __restrict__ jobs,
                                                           const uint32_t* __restrict__ order,
-                                                          const uint32_t* __restrict__ chunk, uint32_t nj,
+                                                          const uint32_t* __restrict__ chunk,
+                                                          const uint32_t* __restrict__ params, uint32_t r,
                                                           const uint32_t* jcnt, uint32_t tile, uint4* tiles,
                                                           uint32_t tiles_cap, uint32_t* pref, uint32_t* glob) {
+  const uint32_t first = params[FS_BP_ROUND0 + 2 * r], nj = params[FS_BP_ROUND0 + 2 * r + 1];
+  order += first;
+  chunk += 2 * first;
   if (threadIdx.x == 0) {
     glob[FS_BG_HEAD] = 0;  // the previous round's persistent kernel has finished (stream order)
     uint32_t acc = 0;
@@ -212,18 +228,18 @@ __global__ void k_bfill_ntab(double* t, uint32_t count, double* pt, uint32_t pai
 
 // ============================================================ launch helpers
 
-int fs_batch_launch(const fs_batch_dev& d, uint32_t nj, const std::vector<uint32_t>& norder,
-                    const std::vector<uint32_t>& off, uint32_t pg_blocks, uint32_t tile, uint32_t blocks,
-                    uint32_t unroll, hipStream_t st) {
-  k_bclear<<<(nj + 255) / 256 + 1, 256, 0, st>>>(d.jobs, d.jcnt, d.jflag, d.jsum, nj, d.glob);
+int fs_batch_launch(const fs_batch_dev& d, uint32_t jobs_cap, uint32_t rounds, uint32_t pg_grid, uint32_t tile,
+                    uint32_t blocks, uint32_t unroll, hipStream_t st) {
+  k_bclear<<<std::min<uint32_t>((jobs_cap + 255) / 256, 64) + 1, 256, 0, st>>>(d.jobs, d.jcnt, d.jflag, d.jsum,
+                                                                               d.params, d.glob);
   BCHK(hipGetLastError(), "batch clear");
-  if (pg_blocks) {
-    k_bprimegen<<<pg_blocks, FS_BLOCK, 0, st>>>(d.jobs, nj, d.c, d.jcnt, d.jflag);
+  if (pg_grid) {
+    k_bprimegen<<<pg_grid, FS_BLOCK, 0, st>>>(d.jobs, d.params, d.c, d.jcnt, d.jflag);
     BCHK(hipGetLastError(), "batch primegen");
   }
-  for (size_t r = 0; r < norder.size(); ++r) {
-    k_bmake_tiles<<<1, FS_BLOCK, 0, st>>>(d.jobs, d.order + off[r], d.chunk + 2 * off[r], norder[r], d.jcnt, tile,
-                                          d.tiles, d.tiles_cap, d.pref, d.glob);
+  for (uint32_t r = 0; r < rounds; ++r) {
+    k_bmake_tiles<<<1, FS_BLOCK, 0, st>>>(d.jobs, d.order, d.chunk, d.params, r, d.jcnt, tile, d.tiles, d.tiles_cap,
+                                          d.pref, d.glob);
     BCHK(hipGetLastError(), "batch tiles");
     if (unroll == 8)
       k_bsieve<8><<<blocks, FS_BLOCK, 0, st>>>(d.jobs, d.tiles, d.c, d.prog, d.ntab, d.pair_len, d.ntab + d.ntab_len,

@@ -51,7 +51,7 @@ int grow(T** p, size_t want, const char* what) {
 }  // namespace
 
 int fs_batch_bufs::ensure(uint32_t nj, uint32_t ncands, uint32_t nprog, uint32_t ntiles, uint32_t nmax_max,
-                          uint32_t fcap) {
+                          uint32_t fcap, uint32_t nparams) {
   int rc;
   if (!st) {
     HCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "batch stream");
@@ -94,12 +94,79 @@ int fs_batch_bufs::ensure(uint32_t nj, uint32_t ncands, uint32_t nprog, uint32_t
     if ((rc = grow(&d.fac, fcap, "batch factors"))) return rc;
     d.fcap = fcap;
   }
+  if (nparams > d.params_cap || !d.params) {
+    const uint32_t n = std::max<uint32_t>(nparams, 64);
+    if ((rc = grow(&d.params, n, "batch params"))) return rc;
+    d.params_cap = n;
+  }
   return FS_OK;
 }
 
+void fs_batch_bufs::drop_graphs() {
+  for (auto& g : graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  graphs.clear();
+}
+
+int fs_batch_bufs::enqueue(const fs_batch_dev& dv, uint32_t rounds, uint32_t pg_grid, uint32_t tile,
+                           uint32_t blocks, uint32_t unroll, bool graph) {
+  if (!graph) return fs_batch_launch(dv, jobs_cap, rounds, pg_grid, tile, blocks, unroll, st);
+  // everything a launch bakes into its kernel arguments and grids
+  std::vector<uint64_t> key = {
+      (uint64_t)(uintptr_t)dv.jobs, (uint64_t)(uintptr_t)dv.order, (uint64_t)(uintptr_t)dv.chunk,
+      (uint64_t)(uintptr_t)dv.jcnt, (uint64_t)(uintptr_t)dv.jflag, (uint64_t)(uintptr_t)dv.jsum,
+      (uint64_t)(uintptr_t)dv.glob, (uint64_t)(uintptr_t)dv.pref, (uint64_t)(uintptr_t)dv.tiles, dv.tiles_cap,
+      (uint64_t)(uintptr_t)dv.c.p, (uint64_t)(uintptr_t)dv.c.q, (uint64_t)(uintptr_t)dv.c.r2,
+      (uint64_t)(uintptr_t)dv.c.res, (uint64_t)(uintptr_t)dv.c.nnext, dv.c.cap, dv.c.grid_hint,
+      (uint64_t)(uintptr_t)dv.prog, (uint64_t)(uintptr_t)dv.ntab, dv.ntab_len, dv.pair_len,
+      (uint64_t)(uintptr_t)dv.fac, dv.fcap, (uint64_t)(uintptr_t)dv.params, jobs_cap, rounds, pg_grid, tile,
+      blocks, unroll};
+  for (size_t k = 0; k < graphs.size(); ++k) {
+    if (graphs[k].key == key) {
+      HCHK(hipGraphLaunch(graphs[k].exec, st), "batch graph launch");
+      ++graph_replays;
+      return FS_OK;
+    }
+  }
+  hipGraph_t g;
+  HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "batch capture");
+  int rc = fs_batch_launch(dv, jobs_cap, rounds, pg_grid, tile, blocks, unroll, st);
+  hipError_t ce = hipStreamEndCapture(st, &g);
+  if (rc) return rc;
+  HCHK(ce, "batch end capture");
+  hipGraphExec_t ge;
+  hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HCHK(ie, "batch graph instantiate");
+  if (graphs.size() >= 4) {  // keep the most recent few shapes
+    (void)hipGraphExecDestroy(graphs.front().exec);
+    graphs.erase(graphs.begin());
+  }
+  graphs.push_back({key, ge});
+  ++graph_captures;
+  HCHK(hipGraphLaunch(ge, st), "batch graph launch");
+  return FS_OK;
+}
+
+std::vector<uint32_t> fs_batch_params(uint32_t nj, uint32_t pg_blocks, const std::vector<uint32_t>& rounds) {
+  std::vector<uint32_t> p(FS_BP_WORDS(rounds.size()));
+  p[FS_BP_NJ] = nj;
+  p[FS_BP_PGBLOCKS] = pg_blocks;
+  uint32_t first = 0;
+  for (size_t r = 0; r < rounds.size(); ++r) {
+    p[FS_BP_ROUND0 + 2 * r] = first;
+    p[FS_BP_ROUND0 + 2 * r + 1] = rounds[r];
+    first += rounds[r];
+  }
+  return p;
+}
+
 void fs_batch_bufs::release() {
+  if (st) (void)hipStreamSynchronize(st);
+  drop_graphs();
   if (d.c.p) fs_cands_free(&d.c);
-  void* ps[] = {d.jobs, d.order, d.chunk, d.jcnt, d.jflag, d.jsum, d.glob, d.pref, d.tiles, d.prog, d.ntab, d.fac};
+  void* ps[] = {d.jobs, d.order, d.chunk, d.jcnt, d.jflag, d.jsum, d.glob, d.pref, d.tiles, d.prog, d.ntab, d.fac,
+                d.params};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   d = fs_batch_dev();
@@ -116,11 +183,10 @@ int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const 
                         uint32_t ncands, uint32_t tile, uint32_t blocks, uint32_t fcap, bool graph, float* dev_ms,
                         const std::vector<uint32_t>& rounds, uint32_t unroll) {
   const uint32_t nj = (uint32_t)bj.size();
-  std::vector<uint32_t> norder = rounds, off;
+  std::vector<uint32_t> norder = rounds;
   if (norder.empty()) norder.push_back((uint32_t)order.size());
   uint32_t nmax_max = 0, ntiles = 0, k = 0;
   for (uint32_t r : norder) {  // the tile list holds one round at a time
-    off.push_back(k);
     uint32_t t = 0;
     for (uint32_t e = 0; e < r; ++e, ++k) {
       const fs_bjob& b = bj[order[k]];
@@ -133,8 +199,9 @@ int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const 
     fs_set_error_msg("batch: rounds do not cover the tile order");
     return FS_E_ARG;
   }
+  const std::vector<uint32_t> params = fs_batch_params(nj, pg_blocks, norder);
   int rc = E.ensure(std::max<uint32_t>(nj, (uint32_t)order.size()), ncands, (uint32_t)words.size(), ntiles,
-                    nmax_max, fcap);
+                    nmax_max, fcap, (uint32_t)params.size());
   if (rc) return rc;
   fs_batch_dev& d = E.d;
   HCHK(hipMemcpyAsync(d.jobs, bj.data(), nj * sizeof(fs_bjob), hipMemcpyHostToDevice, E.st), "batch upload");
@@ -144,27 +211,11 @@ int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const 
   }
   if (!words.empty())
     HCHK(hipMemcpyAsync(d.prog, words.data(), words.size() * 8, hipMemcpyHostToDevice, E.st), "batch upload");
+  HCHK(hipMemcpyAsync(d.params, params.data(), params.size() * 4, hipMemcpyHostToDevice, E.st), "batch upload");
   HCHK(hipStreamSynchronize(E.st), "batch upload sync");
   HCHK(hipEventRecord(E.ev0, E.st), "batch event");
-  if (graph) {
-    hipGraph_t g;
-    hipGraphExec_t ge;
-    HCHK(hipStreamBeginCapture(E.st, hipStreamCaptureModeThreadLocal), "batch capture");
-    rc = fs_batch_launch(d, nj, norder, off, pg_blocks, tile, blocks, unroll, E.st);
-    hipError_t ce = hipStreamEndCapture(E.st, &g);
-    if (rc) return rc;
-    HCHK(ce, "batch end capture");
-    hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    HCHK(ie, "batch graph instantiate");
-    hipError_t le = hipGraphLaunch(ge, E.st);
-    hipError_t se = hipStreamSynchronize(E.st);
-    (void)hipGraphExecDestroy(ge);
-    HCHK(le, "batch graph launch");
-    HCHK(se, "batch graph sync");
-  } else {
-    if ((rc = fs_batch_launch(d, nj, norder, off, pg_blocks, tile, blocks, unroll, E.st))) return rc;
-  }
+  const uint32_t pg_grid = std::min<uint32_t>(pg_blocks, fs_device_cus() * 8);
+  if ((rc = E.enqueue(d, (uint32_t)norder.size(), pg_grid, tile, blocks, unroll, graph))) return rc;
   HCHK(hipEventRecord(E.ev1, E.st), "batch event");
   HCHK(hipStreamSynchronize(E.st), "batch sync");
   if (dev_ms) HCHK(hipEventElapsedTime(dev_ms, E.ev0, E.ev1), "batch time");
@@ -513,13 +564,13 @@ extern "C" int fs_batch_autotune(const fs_job* jobs, uint32_t nj, fs_batch_optio
   uint32_t nlen_max = 0;
   for (uint32_t k = 0; k < nj; ++k) nlen_max = std::max(nlen_max, jobs[k].nmax - jobs[k].nmin);
   std::vector<fs_job_result> res(nj);
-  // one warm run, then the faster of two timed runs (single runs differ by a
-  // few per cent, as much as some of the choices below)
+  // one warm run, then the fastest of three timed runs (single runs differ by
+  // a few per cent, as much as some of the choices below)
   auto timed = [&](const fs_batch_options& o, double* sec) {
     int rc = fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, sec);
     if (rc) return rc;
     double best_s = 1e30;
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 3; ++k) {
       if ((rc = fs_batch(sample.data(), nj, &o, res.data(), nullptr, 0, nullptr, sec))) return rc;
       best_s = std::min(best_s, *sec);
     }
@@ -533,9 +584,11 @@ extern "C" int fs_batch_autotune(const fs_job* jobs, uint32_t nj, fs_batch_optio
   int rc = timed(best, &tbest);
   if (rc) return rc;
   // coordinate descent: each dimension in turn, keeping the best value found;
-  // a value replaces the current one only when it is faster by more than 1 %
+  // a value replaces the current one only when it is faster by more than 2 %
   // (the sample's timing noise), so near-ties keep the default shape, which
-  // the full-size batch prefers (a 1/8 sample has fewer tiles per workgroup)
+  // the full-size batch prefers (a 1/8 sample has fewer tiles per workgroup:
+  // with a 1 % margin, 4 workgroups per CU sometimes won on the sample and then
+  // ran the full batch 2-4 % slower than 8)
   auto dim = [&](auto set, std::initializer_list<uint32_t> vals) -> int {
     for (uint32_t v : vals) {
       fs_batch_options o = best;
@@ -543,7 +596,7 @@ extern "C" int fs_batch_autotune(const fs_job* jobs, uint32_t nj, fs_batch_optio
       double sec = 0;
       const int rc2 = timed(o, &sec);
       if (rc2) return rc2;
-      if (sec < tbest * 0.99) {
+      if (sec < tbest * 0.98) {
         tbest = sec;
         best = o;
       }

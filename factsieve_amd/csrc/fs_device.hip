@@ -429,6 +429,13 @@ extern "C" int fs_inject_fault(int idx) {
   return FS_OK;
 }
 
+// The injected fault `idx`, once: 1 (and the injection cleared) when it was set.
+int fs_take_injected_fault(int idx) {
+  if (g_inject != idx || idx == 0) return 0;
+  g_inject = 0;
+  return 1;
+}
+
 extern "C" int fs_malloc(void** d, size_t bytes) {
   int rc = ensure_init();
   if (rc) return rc;

@@ -35,3 +35,6 @@ uint32_t fs_auto_segment_candidates();
 int fs_validate_ranges(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax);
 // Upper estimate of the candidates in [lo, lo + len).
 uint64_t fs_cand_capacity(uint64_t lo, uint64_t len);
+// The fault injected with fs_inject_fault for counter idx, taken once: 1 when
+// it was set (and it is then cleared), else 0.
+int fs_take_injected_fault(int idx);

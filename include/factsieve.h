@@ -364,26 +364,37 @@ int fs_batch_autotune(const fs_job* jobs, uint32_t njobs, fs_batch_options* opt)
 /* ------------------------------------------------------ serving loop */
 /* Continuous batching of workunits (no reference counterpart: BOINC hands one
  * workunit to one process).  Admission by device memory; residues stay in HBM
- * between quanta; each result equals fs_sieve on the workunit alone. */
+ * between quanta; each result equals fs_sieve on the workunit alone.  Quanta
+ * are pipelined one deep: a step enqueues its quantum and then harvests the
+ * previous one (verification and filtering on the host) while the new one
+ * runs, so a workunit's result is published by the step after its last
+ * quantum.  A workunit whose candidates or factors overflowed a buffer starts
+ * again with a larger one (at most 3 times; then FS_E_OVERFLOW). */
 typedef void* fs_server;
 typedef struct {
   uint64_t mem_budget;      /* device bytes the server may hold for admitted work (0 = 50% free) */
   uint32_t max_slots;       /* concurrently resident workunits */
   uint32_t n_chunk;         /* n values per scheduling quantum */
   int verify_threads;
+  uint32_t use_graph;       /* 1 (default): each quantum's launches replay a captured hipGraph */
 } fs_server_options;
 
 void fs_default_server_options(fs_server_options* o);
 int fs_server_create(const fs_server_options* o, fs_server* srv);
-/* Submit a workunit; returns its id (>= 0) or FS_E_BUSY/FS_E_ARG. Never blocks. */
+/* Submit a workunit; returns its id (>= 0) or FS_E_BUSY/FS_E_ARG.  Never waits
+ * for device work: the incoming queue has its own lock, held only to append. */
 int64_t fs_server_submit(fs_server srv, const fs_job* job);
 /* Advance the device work by one scheduling quantum (admit queued work when
- * memory and slots free up, advance every active workunit by n_chunk n-values,
- * harvest finished workunits).  Returns the number finished (>= 0) or an error. */
+ * memory and slots free up, enqueue a quantum that advances every active
+ * workunit by n_chunk n-values, harvest the previous quantum).  Returns the
+ * number of workunits finished by the harvest (>= 0) or an error.  Steps of one
+ * server run one at a time. */
 int fs_server_step(fs_server srv);
 /* Fetch a finished workunit's result (1 = done and copied, 0 = not yet). */
 int fs_server_poll(fs_server srv, int64_t id, fs_job_result* out, fs_factor* factors, uint64_t cap,
                    uint64_t* nfactors);
+/* queued: submitted, not yet admitted; active: admitted, result not yet
+ * published; done: results published; bytes_in_use: candidate regions held. */
 int fs_server_stats(fs_server srv, uint32_t* queued, uint32_t* active, uint32_t* done,
                     uint64_t* bytes_in_use);
 int fs_server_destroy(fs_server srv);

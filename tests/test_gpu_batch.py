@@ -176,12 +176,28 @@ def test_batch_autotune_returns_valid_shape(gpu):
     assert [outcome(r) for r in res] == [outcome(r) for r in base]
 
 
-@pytest.mark.parametrize("budget,chunk", [(0, 700), (64 << 20, 5000)])
-def test_server_results_equal_alone(gpu, budget, chunk):
+def drain(srv, ids, limit=20000):
+    """Step the server until every id has a result; {id: JobOutcome}."""
+    got = {}
+    for _ in range(limit):
+        srv.step()
+        for i in ids:
+            if i not in got:
+                r = srv.poll(i)
+                if r is not None:
+                    got[i] = r
+        if len(got) == len(ids):
+            break
+    return got
+
+
+@pytest.mark.parametrize("budget,chunk,graph", [(0, 700, 1), (64 << 20, 5000, 1), (0, 1500, 0)])
+def test_server_results_equal_alone(gpu, budget, chunk, graph):
     """Workunits arriving while others run are admitted as slots/memory free
-    up; each result equals the same workunit run alone."""
+    up; each result equals the same workunit run alone, with the quanta's
+    launches replayed from a graph or enqueued one by one."""
     jobs = small_mix()
-    srv = gpu.Server(mem_budget=budget, n_chunk=chunk)
+    srv = gpu.Server(mem_budget=budget, n_chunk=chunk, use_graph=graph)
     try:
         ids = [srv.submit(j) for j in jobs[:10]]
         srv.step()
@@ -204,4 +220,111 @@ def test_server_results_equal_alone(gpu, budget, chunk):
             assert got[i].status == 0
             assert outcome(got[i]) == plain(gpu, job), job
     finally:
+        srv.close()
+
+
+def test_server_small_workunits_match_oracle(gpu):
+    """Seeded random workunits over every modulus size, served together in
+    quanta of 500 n: every result against the oracle."""
+    jobs = random_jobs(11, 12)
+    srv = gpu.Server(n_chunk=500)
+    try:
+        ids = [srv.submit(j) for j in jobs]
+        got = drain(srv, ids)
+        assert len(got) == len(ids)
+    finally:
+        srv.close()
+    for i, job in zip(ids, jobs):
+        want = ol.sieve(*job, threads=8)
+        assert got[i].status == 0, job
+        assert (got[i].factorcount, got[i].primecount, got[i].checksum) == (want["factorcount"], want["primecount"],
+                                                                          want["checksum"]), job
+        assert got[i].factors == want["factors"], job
+
+
+def test_server_submit_does_not_wait_for_a_step(gpu):
+    """Submissions go to a queue with its own lock: while another thread runs
+    steps whose quanta take tens of milliseconds, every submit returns within
+    a few milliseconds, and all the work still comes out right."""
+    import threading
+    import time
+    heavy = [(2 ** 62 + k * 2 ** 26, 2 ** 62 + k * 2 ** 26 + 2 ** 24, 500000, 900000) for k in range(4)]
+    light = small_mix()[:12]
+    srv = gpu.Server(n_chunk=100000)
+    ids, times, stop = [], [], threading.Event()
+    try:
+        ids += [srv.submit(j) for j in heavy]
+        step_times = []
+
+        def stepper():
+            while not stop.is_set():
+                t = time.perf_counter()
+                srv.step()
+                step_times.append(time.perf_counter() - t)
+
+        th = threading.Thread(target=stepper)
+        th.start()
+        time.sleep(0.05)
+        for j in light:
+            t = time.perf_counter()
+            ids.append(srv.submit(j))
+            times.append(time.perf_counter() - t)
+            time.sleep(0.005)
+        stop.set()
+        th.join()
+        got = drain(srv, ids)
+    finally:
+        srv.close()
+    print(f"submit max {max(times) * 1e3:.2f} ms; step max {max(step_times) * 1e3:.1f} ms", flush=True)
+    assert max(step_times) > 0.02 and max(times) < 0.005
+    assert len(got) == len(ids)
+    for i, job in zip(ids, heavy + light):
+        assert got[i].status == 0
+        assert outcome(got[i]) == plain(gpu, job), job
+
+
+@pytest.mark.parametrize("flag", ["factor", "candidate"])
+def test_server_overflow_restarts_workunits(gpu, flag):
+    """A factor-buffer or candidate-region overflow (forced from the host for
+    one harvested quantum) restarts that quantum's workunits from scratch;
+    their results still equal the workunits run alone."""
+    ctr = gpu.CTR_FACTOROVF if flag == "factor" else gpu.CTR_PRIMEOVF
+    jobs = small_mix()[:10]
+    srv = gpu.Server(n_chunk=1000)
+    try:
+        ids = [srv.submit(j) for j in jobs]
+        srv.step()                      # first quantum enqueued
+        gpu.check(gpu.lib().fs_inject_fault(ctr))
+        srv.step()                      # its harvest sees the overflow
+        got = drain(srv, ids)
+    finally:
+        gpu.lib().fs_inject_fault(0)
+        srv.close()
+    assert len(got) == len(ids)
+    for i, job in zip(ids, jobs):
+        assert got[i].status == 0, job
+        assert outcome(got[i]) == plain(gpu, job), job
+
+
+def test_server_gives_up_after_repeated_overflows(gpu):
+    """A workunit whose every quantum overflows is restarted at most three
+    times and then fails with FS_E_OVERFLOW; the server stays usable."""
+    job = (10 ** 9, 10 ** 9 + 100000, 1000, 4000)
+    srv = gpu.Server(n_chunk=1000)
+    try:
+        wid = srv.submit(job)
+        res = None
+        for _ in range(200):
+            gpu.check(gpu.lib().fs_inject_fault(gpu.CTR_FACTOROVF))
+            srv.step()
+            res = srv.poll(wid)
+            if res is not None:
+                break
+        gpu.lib().fs_inject_fault(0)
+        assert res is not None and res.status == gpu.FS_E_OVERFLOW
+        wid2 = srv.submit(job)
+        got = drain(srv, [wid2])
+        assert got[wid2].status == 0 and outcome(got[wid2]) == plain(gpu, job)
+    finally:
+        gpu.lib().fs_inject_fault(0)
         srv.close()
