@@ -63,7 +63,7 @@ class Result(ctypes.Structure):
                 ("steps", u64), ("setup_mulmods", u64), ("seconds", ctypes.c_double),
                 ("kernel_ms_iterate", ctypes.c_double), ("kernel_ms_setup", ctypes.c_double),
                 ("kernel_ms_primegen", ctypes.c_double), ("iterate_launches", u32), ("path", ctypes.c_int),
-                ("segments", u32), ("resumed", u32)]
+                ("segments", u32), ("resumed", u32), ("lanes", u32)]
 
 
 class Job(ctypes.Structure):
@@ -364,7 +364,7 @@ def hostemu_candidates(low, high):
 
 SieveResult = namedtuple("SieveResult", "factors factorcount primecount checksum raw_factors steps seconds "
                                         "kernel_ms_iterate kernel_ms_setup kernel_ms_primegen path segments "
-                                        "iterate_launches setup_mulmods resumed")
+                                        "iterate_launches setup_mulmods resumed lanes")
 
 
 def default_options(**kw):
@@ -401,7 +401,7 @@ def sieve(pmin, pmax, nmin, nmax, factor_cap=1 << 20, **opts):
     facs = [(f[i].p, f[i].nc) for i in range(min(nf.value, factor_cap))]
     return SieveResult(facs, r.factorcount, r.primecount, r.checksum, r.raw_factors, r.steps, r.seconds,
                        r.kernel_ms_iterate, r.kernel_ms_setup, r.kernel_ms_primegen, r.path, r.segments,
-                       r.iterate_launches, r.setup_mulmods, r.resumed)
+                       r.iterate_launches, r.setup_mulmods, r.resumed, r.lanes)
 
 
 cl_sieve = sieve

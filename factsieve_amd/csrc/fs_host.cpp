@@ -760,32 +760,137 @@ double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// Device buffers, stream, events and pinned read-back words of the plain
+// driver.  They outlive a call: a pool keeps one set per concurrently running
+// fs_sieve call (per device), so a call reuses the stream, its multiplier
+// table and the buffers an earlier call allocated, grows them only when it
+// needs more, and re-uploads the setup program and power table only when nmin
+// or the path changed.  The pool is never freed (the process exit reclaims it).
 struct DevBufs {
+  int device = -1;
   fs_cands c{};
   uint32_t* ctr = nullptr;
   uint64_t* sum = nullptr;
   fs_factor* fac = nullptr;
-  uint32_t fcap = 0;
+  uint32_t fcap = 0;        // this call's factor capacity (the reference's, main.cpp:193)
+  uint32_t fac_alloc = 0;   // records allocated (>= fcap)
   void* prog = nullptr;
+  size_t prog_alloc = 0;    // bytes
   uint32_t prog_terms = 0;
+  uint32_t prog_nmin = 0;
+  int prog_path = -1;       // program in prog: (prog_nmin, prog_path), -1 = none
   uint64_t* pt_bases = nullptr;
   uint32_t* pt_exps = nullptr;
+  size_t pt_alloc = 0;      // entries
   uint32_t pt_count = 0;
+  int64_t pt_nmin = -1;     // power table in pt_*: nmin, -1 = none
   void* verify = nullptr;
   fs_stream s = nullptr;
   fs_event ev[6] = {};
-  ~DevBufs() {
-    fs_cands_free(&c);
-    fs_free(ctr);
-    fs_free(sum);
-    fs_free(fac);
-    fs_free(prog);
-    fs_free(pt_bases);
-    fs_free(pt_exps);
-    fs_free(verify);
-    for (auto& e : ev) fs_event_destroy(e);
-    fs_stream_destroy(s);
+  uint32_t* h_ctr = nullptr;  // pinned read-back of the counters and sums
+  uint64_t* h_sum = nullptr;
+  // lanes 1..3 of a job split over several lanes per prime: streams beside s,
+  // their end events, and their setup programs (cached by nmin and path)
+  static constexpr int kLanes = 4;
+  hipStream_t xs[kLanes - 1] = {};
+  hipEvent_t xev[kLanes] = {};
+  void* xprog[kLanes - 1] = {};
+  size_t xprog_alloc[kLanes - 1] = {};
+  uint32_t xprog_nmin[kLanes - 1] = {};
+  int xprog_path[kLanes - 1] = {-1, -1, -1};
+  int init_lanes() {
+    for (int k = 0; k < kLanes - 1; ++k)
+      if (!xs[k] && hipStreamCreateWithFlags(&xs[k], hipStreamNonBlocking) != hipSuccess) {
+        fs_set_error_msg("lane stream creation failed");
+        return FS_E_HIP;
+      }
+    for (int k = 0; k < kLanes; ++k)
+      if (!xev[k] && hipEventCreateWithFlags(&xev[k], hipEventDisableTiming) != hipSuccess) {
+        fs_set_error_msg("lane event creation failed");
+        return FS_E_HIP;
+      }
+    return FS_OK;
   }
+  int init() {
+    int rc;
+    if (hipGetDevice(&device) != hipSuccess) device = 0;
+    if ((rc = fs_stream_create(&s))) return rc;
+    for (auto& e : ev)
+      if ((rc = fs_event_create(&e))) return rc;
+    if ((rc = fs_malloc((void**)&ctr, FS_CTR_WORDS * 4)) || (rc = fs_malloc((void**)&sum, FS_SUM_WORDS * 8)) ||
+        (rc = fs_malloc(&verify, fs_verify_bytes())))
+      return rc;
+    if (hipHostMalloc((void**)&h_ctr, FS_CTR_WORDS * 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_sum, FS_SUM_WORDS * 8, hipHostMallocDefault) != hipSuccess) {
+      fs_set_error_msg("pinned host allocation failed");
+      return FS_E_NOMEM;
+    }
+    return FS_OK;
+  }
+  // factor records for a capacity of `cap`
+  int need_factors(uint32_t cap) {
+    fcap = cap;
+    if (fac_alloc >= cap) return FS_OK;
+    fs_free(fac);
+    fac = nullptr;
+    fac_alloc = 0;
+    int rc = fs_malloc((void**)&fac, (size_t)cap * sizeof(fs_factor));
+    if (rc == FS_OK) fac_alloc = cap;
+    return rc;
+  }
+  // counters (and sums) to the host through pinned memory, waiting for them
+  int read_ctr(uint32_t* out, uint64_t* sums = nullptr) {
+    if ((sums && hipMemcpyAsync(h_sum, sum, FS_SUM_WORDS * 8, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess) ||
+        hipMemcpyAsync(h_ctr, ctr, FS_CTR_WORDS * 4, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess) {
+      fs_set_error_msg("counter read-back failed");
+      return FS_E_HIP;
+    }
+    int rc = fs_stream_sync(s);
+    if (rc) return rc;
+    memcpy(out, h_ctr, FS_CTR_WORDS * 4);
+    if (sums) memcpy(sums, h_sum, FS_SUM_WORDS * 8);
+    return FS_OK;
+  }
+};
+
+std::mutex g_pool_mu;
+std::vector<DevBufs*>* g_pool = new std::vector<DevBufs*>();
+
+// One call's use of a pooled DevBufs; the set goes back to the pool when the
+// call ends (its stream drained first, also after an error).
+struct Lease {
+  DevBufs* d = nullptr;
+  int take() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    {
+      std::lock_guard<std::mutex> lk(g_pool_mu);
+      for (size_t k = g_pool->size(); k-- > 0;) {
+        if ((*g_pool)[k]->device == dev) {
+          d = (*g_pool)[k];
+          g_pool->erase(g_pool->begin() + k);
+          return FS_OK;
+        }
+      }
+    }
+    d = new DevBufs();
+    const int rc = d->init();
+    if (rc) {
+      give_back();  // keeps what it allocated for a later call
+      return rc;
+    }
+    return FS_OK;
+  }
+  void give_back() {
+    if (!d) return;
+    for (hipStream_t x : d->xs)
+      if (x) (void)hipStreamSynchronize(x);
+    if (d->s) (void)fs_stream_sync(d->s);
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    if (d->s) g_pool->push_back(d);  // a set without a stream is never handed out again
+    d = nullptr;
+  }
+  ~Lease() { give_back(); }
 };
 
 struct Drain {
@@ -800,9 +905,7 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
   uint32_t ctr[FS_CTR_WORDS];
   uint64_t sum[FS_SUM_WORDS];
   int rc;
-  if ((rc = fs_memcpy_dtoh(sum, d.sum, sizeof sum, d.s, 0)) ||
-      (rc = fs_memcpy_dtoh(ctr, d.ctr, sizeof ctr, d.s, 1)))
-    return rc;
+  if ((rc = d.read_ctr(ctr, sum))) return rc;
   st.primecount += sum[0];
   st.checksum += sum[1];
   const bool verbose = !o.quiet;
@@ -991,6 +1094,29 @@ static void account(DevBufs& d, fs_result* r) {
   if (fs_event_elapsed_ms(d.ev[4], d.ev[5], &ms) == FS_OK) r->kernel_ms_iterate += ms;
 }
 
+// Lanes per prime for a job whose one segment holds fewer candidates than half
+// the GPU's wave slots (cus x 4 SIMDs x 8 waves x 64 lanes): the n-range is cut
+// into L subranges that run side by side on L streams, each from the setup
+// program of its own first n, as long as the costliest extra setup is at most
+// half a subrange's iterate work (VALU operations per setup term and per step
+// of each path).  1 = no split.
+static uint32_t split_lanes(uint32_t count, uint32_t nmin, uint32_t nmax, int path) {
+  const uint64_t slots = fs_auto_segment_candidates() / 8;
+  if (count == 0 || 2 * (uint64_t)count > slots || path == FS_PATH_MONT) return 1;
+  uint32_t L = (uint32_t)std::min<uint64_t>(DevBufs::kLanes, slots / count);
+  L = std::min<uint32_t>(L, (nmax - nmin) / 2048);
+  // setup program terms at n, about 0.047 n (FP64, multipliers < 2^50) or
+  // 0.037 n (hybrid, 60-bit words), and VALU operations per term and per step
+  const double terms_per_n = path == FS_PATH_FP64 ? 0.047 : 0.037;
+  const double term_ops = path == FS_PATH_FP64 ? 6.3 : 18.0;
+  const double step_ops = path == FS_PATH_FP64 ? 4.0 : 7.2;
+  for (; L > 1; --L) {
+    const double nlast = nmin + (double)(nmax - nmin) * (L - 1) / L;
+    if (2.0 * (nlast * terms_per_n + 64) * term_ops <= (double)(nmax - nmin) / L * step_ops) break;
+  }
+  return std::max<uint32_t>(L, 1);
+}
+
 extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax, const fs_options* opt,
                         fs_result* res, fs_factor* factors, uint64_t cap, uint64_t* nfactors) {
   fs_options o;
@@ -998,6 +1124,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   else fs_default_options(&o);
   fs_result r;
   memset(&r, 0, sizeof r);
+  r.lanes = 1;
   if (nfactors) *nfactors = 0;
   int rc = fs_validate_ranges(pmin, pmax, nmin, nmax);
   if (rc) return rc;
@@ -1065,37 +1192,58 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     }
   }
 
-  DevBufs d;
+  Lease lease;
+  if ((rc = lease.take())) return rc;
+  DevBufs& d = *lease.d;
   Drain dr;
-  if ((rc = fs_stream_create(&d.s))) return rc;
-  for (auto& e : d.ev)
-    if ((rc = fs_event_create(&e))) return rc;
   // factor buffer: 1e6 records, 3e7 below 2^32 (main.cpp:193, cl_sieve.cpp:593-595)
-  d.fcap = (pmin < 0xFFFFFFFFull) ? 30000000u : 1000000u;
-  if ((rc = fs_malloc((void**)&d.ctr, FS_CTR_WORDS * 4)) || (rc = fs_malloc((void**)&d.sum, FS_SUM_WORDS * 8)) ||
-      (rc = fs_malloc((void**)&d.fac, (size_t)d.fcap * sizeof(fs_factor))) ||
-      (rc = fs_malloc(&d.verify, fs_verify_bytes())))
-    return rc;
+  if ((rc = d.need_factors((pmin < 0xFFFFFFFFull) ? 30000000u : 1000000u))) return rc;
   if ((rc = fs_memset(d.ctr, 0, FS_CTR_WORDS * 4, d.s)) || (rc = fs_memset(d.sum, 0, FS_SUM_WORDS * 8, d.s)))
     return rc;
 
-  // setup program and the reference power table (verifypow, and setup on MONT)
+  // setup program and the reference power table (verifypow, and setup on MONT),
+  // uploaded when they differ from what the buffers hold
   {
     const std::shared_ptr<const fs_program> pr = prog_fut.get();
     if (!pr) return FS_E_NOMEM;
-    const std::vector<uint64_t>& w = pr->words;
-    d.prog_terms = pr->terms;
-    if ((rc = fs_malloc(&d.prog, w.size() * 8)) || (rc = fs_memcpy_htod(d.prog, w.data(), w.size() * 8, d.s, 1)))
-      return rc;
-    const int64_t m = fs_power_table(nmin, nullptr, nullptr, nullptr);
-    std::vector<uint64_t> b((size_t)m + 1);
-    std::vector<uint32_t> e((size_t)m + 1), c((size_t)m + 1);
-    fs_power_table(nmin, b.data(), e.data(), c.data());
-    d.pt_count = (uint32_t)m;
-    if ((rc = fs_malloc((void**)&d.pt_bases, b.size() * 8)) || (rc = fs_malloc((void**)&d.pt_exps, e.size() * 4)) ||
-        (rc = fs_memcpy_htod(d.pt_bases, b.data(), b.size() * 8, d.s, 0)) ||
-        (rc = fs_memcpy_htod(d.pt_exps, e.data(), e.size() * 4, d.s, 1)))
-      return rc;
+    if (d.prog_path != path || d.prog_nmin != nmin) {
+      const std::vector<uint64_t>& w = pr->words;
+      const size_t bytes = std::max<size_t>(w.size(), 1) * 8;
+      if (bytes > d.prog_alloc) {
+        fs_free(d.prog);
+        d.prog = nullptr;
+        d.prog_alloc = 0;
+        if ((rc = fs_malloc(&d.prog, bytes))) return rc;
+        d.prog_alloc = bytes;
+      }
+      d.prog_path = -1;
+      if ((rc = fs_memcpy_htod(d.prog, w.data(), w.size() * 8, d.s, 1))) return rc;
+      d.prog_terms = pr->terms;
+      d.prog_nmin = nmin;
+      d.prog_path = path;
+    }
+    if (d.pt_nmin != (int64_t)nmin) {
+      const int64_t m = fs_power_table(nmin, nullptr, nullptr, nullptr);
+      std::vector<uint64_t> b((size_t)m + 1);
+      std::vector<uint32_t> e((size_t)m + 1), c((size_t)m + 1);
+      fs_power_table(nmin, b.data(), e.data(), c.data());
+      if (b.size() > d.pt_alloc) {
+        fs_free(d.pt_bases);
+        fs_free(d.pt_exps);
+        d.pt_bases = nullptr;
+        d.pt_exps = nullptr;
+        d.pt_alloc = 0;
+        if ((rc = fs_malloc((void**)&d.pt_bases, b.size() * 8)) || (rc = fs_malloc((void**)&d.pt_exps, e.size() * 4)))
+          return rc;
+        d.pt_alloc = b.size();
+      }
+      d.pt_nmin = -1;
+      if ((rc = fs_memcpy_htod(d.pt_bases, b.data(), b.size() * 8, d.s, 0)) ||
+          (rc = fs_memcpy_htod(d.pt_exps, e.data(), e.size() * 4, d.s, 1)))
+        return rc;
+      d.pt_count = (uint32_t)m;
+      d.pt_nmin = nmin;
+    }
     fprintf(stderr, "Compressed %zu power table terms to %u\n", fs_primes_upto(nmin).size(), d.pt_count);
     if (verbose) printf("Compressed %zu power table terms to %u\n", fs_primes_upto(nmin).size(), d.pt_count);
   }
@@ -1129,7 +1277,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
           (rc = fs_getsegprimes(st.p, stop, &d.c, d.ctr, d.s)) || (rc = fs_event_record(d.ev[1], d.s)))
         return rc;
       uint32_t h[FS_CTR_WORDS];
-      if ((rc = fs_memcpy_dtoh(h, d.ctr, sizeof h, d.s, 1))) return rc;
+      if ((rc = d.read_ctr(h))) return rc;
       count = h[FS_CTR_COUNT];
       if (count <= d.c.cap && !h[FS_CTR_PRIMEOVF]) break;
       if (attempt > 2) {
@@ -1144,27 +1292,91 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
       if ((rc = fs_memcpy_htod(d.ctr + FS_CTR_PRIMEOVF, &zero, 4, d.s, 1))) return rc;
     }
     d.c.grid_hint = count ? count : 1;
-
-    // setup: nmin! mod p, in chunks of the program
+    // a one-segment job that underfills the GPU runs on several lanes per prime
+    const bool whole = (st.p == pmin && stop == st.pmax);
+    const uint32_t L = (whole && o.n_chunk == 0) ? split_lanes(count, nmin, nmax, path) : 1;
+    if (L > 1 && (uint64_t)L * count > d.c.cap) {  // room for the lanes' copies: grow, redo the segment
+      fs_cands_free(&d.c);
+      if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>((uint64_t)L * count + count / 4 + 4096,
+                                                                  0xFFFFFF00ull))))
+        return rc;
+      continue;
+    }
+    std::vector<uint32_t> nb(L + 1);  // lane k: n in [nb[k], nb[k + 1])
+    for (uint32_t k = 0; k <= L; ++k) nb[k] = nmin + (uint32_t)((uint64_t)(nmax - nmin) * k / L);
+    auto view = [&](uint32_t k) {
+      fs_cands v = d.c;
+      v.p += (size_t)k * count;
+      v.q += (size_t)k * count;
+      v.r2 += (size_t)k * count;
+      v.res += (size_t)k * count;
+      v.nnext += (size_t)k * count;
+      v.cap = count;
+      v.grid_hint = count ? count : 1;
+      return v;
+    };
+    std::vector<fs_cands> views(L);
+    for (uint32_t k = 0; k < L; ++k) views[k] = L > 1 ? view(k) : d.c;
+    std::vector<fs_stream> ls(L, d.s);
+    std::vector<const void*> lprog(L, d.prog);
+    std::vector<uint32_t> lterms(L, d.prog_terms);
+    if (L > 1) {
+      if ((rc = d.init_lanes())) return rc;
+      for (uint32_t k = 1; k < L; ++k) {
+        ls[k] = (fs_stream)d.xs[k - 1];
+        const std::shared_ptr<const fs_program> pr = fs_program_get(nb[k], path);
+        if (!pr) return FS_E_NOMEM;
+        if (d.xprog_path[k - 1] != path || d.xprog_nmin[k - 1] != nb[k]) {
+          const size_t bytes = std::max<size_t>(pr->words.size(), 1) * 8;
+          if (bytes > d.xprog_alloc[k - 1]) {
+            fs_free(d.xprog[k - 1]);
+            d.xprog[k - 1] = nullptr;
+            d.xprog_alloc[k - 1] = 0;
+            if ((rc = fs_malloc(&d.xprog[k - 1], bytes))) return rc;
+            d.xprog_alloc[k - 1] = bytes;
+          }
+          d.xprog_path[k - 1] = -1;
+          if ((rc = fs_memcpy_htod(d.xprog[k - 1], pr->words.data(), pr->words.size() * 8, d.s, 1))) return rc;
+          d.xprog_nmin[k - 1] = nb[k];
+          d.xprog_path[k - 1] = path;
+        }
+        lprog[k] = d.xprog[k - 1];
+        lterms[k] = pr->terms;
+        // lane k's copy of the candidates
+        if ((rc = fs_memcpy_dtod(views[k].p, d.c.p, (size_t)count * 8, d.s)) ||
+            (rc = fs_memcpy_dtod(views[k].q, d.c.q, (size_t)count * 8, d.s)) ||
+            (rc = fs_memcpy_dtod(views[k].r2, d.c.r2, (size_t)count * 8, d.s)))
+          return rc;
+      }
+    }
     if ((rc = fs_event_record(d.ev[2], d.s))) return rc;
-    if (d.prog_terms == 0) {
-      if ((rc = fs_setup(&d.c, d.ctr, d.prog, 0, 0, 0, nmin, path, d.s))) return rc;
+    if (L > 1) {  // the lanes start when the copies are done
+      if (hipEventRecord(d.xev[0], (hipStream_t)d.s) != hipSuccess) return FS_E_HIP;
+      for (uint32_t k = 1; k < L; ++k)
+        if (hipStreamWaitEvent(d.xs[k - 1], d.xev[0], 0) != hipSuccess) return FS_E_HIP;
     }
-    for (uint32_t s0 = 0; s0 < d.prog_terms; s0 += prog_chunk) {
-      const uint32_t s1 = std::min(d.prog_terms, s0 + prog_chunk);
-      if ((rc = fs_setup(&d.c, d.ctr, d.prog, d.prog_terms, s0, s1, nmin, path, d.s))) return rc;
+    // setup: n! mod p at each lane's first n, in chunks of the program
+    for (uint32_t k = 0; k < L; ++k) {
+      if (lterms[k] == 0) {
+        if ((rc = fs_setup(&views[k], d.ctr, lprog[k], 0, 0, 0, nb[k], path, ls[k]))) return rc;
+      }
+      for (uint32_t s0 = 0; s0 < lterms[k]; s0 += prog_chunk) {
+        const uint32_t s1 = std::min(lterms[k], s0 + prog_chunk);
+        if ((rc = fs_setup(&views[k], d.ctr, lprog[k], lterms[k], s0, s1, nb[k], path, ls[k]))) return rc;
+      }
+      if (k == 0 && (rc = fs_event_record(d.ev[3], d.s))) return rc;
     }
-    if ((rc = fs_event_record(d.ev[3], d.s))) return rc;
 
     // power-table / setup cross-check on the first candidate of the first
     // non-empty segment (verify kernels, cl_sieve.cpp:868-907)
     if (!verified && count > 0) {
-      if ((rc = fs_verifyslow(&d.c, d.ctr, d.verify, nmin, d.s)) ||
-          (rc = fs_verifypow(&d.c, d.ctr, d.pt_bases, d.pt_exps, d.pt_count, d.verify, d.s)) ||
-          (rc = fs_verifyreduce(&d.c, d.ctr, d.verify, d.s)) || (rc = fs_verifyresult(&d.c, d.ctr, d.verify, d.s)))
+      if ((rc = fs_verifyslow(&views[0], d.ctr, d.verify, nmin, d.s)) ||
+          (rc = fs_verifypow(&views[0], d.ctr, d.pt_bases, d.pt_exps, d.pt_count, d.verify, d.s)) ||
+          (rc = fs_verifyreduce(&views[0], d.ctr, d.verify, d.s)) ||
+          (rc = fs_verifyresult(&views[0], d.ctr, d.verify, d.s)))
         return rc;
       uint32_t h[FS_CTR_WORDS];
-      if ((rc = fs_memcpy_dtoh(h, d.ctr, sizeof h, d.s, 1))) return rc;
+      if ((rc = d.read_ctr(h))) return rc;
       if (h[FS_CTR_POWERTABLE]) {
         fs_set_error_msg("error: power table verification failed");
         return FS_E_POWERTABLE;
@@ -1175,20 +1387,29 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
       if (verbose) printf("Verified power table (%llu bytes) starting sieve...\n", (unsigned long long)d.pt_count * 16);
     }
 
-    // iterate over n in chunks, then fold the checksum
+    // iterate over each lane's n in chunks, then fold the checksum of the last lane
     if ((rc = fs_event_record(d.ev[4], d.s))) return rc;
     const std::vector<uint32_t> cuts = loop_boundaries(path, st.p, stop, nmin, nmax);
-    size_t ci = 0;
-    for (uint32_t n0 = nmin; n0 < nmax;) {
-      uint32_t n1 = (nmax - n0 > nchunk_max) ? n0 + nchunk_max : nmax;
-      while (ci < cuts.size() && cuts[ci] <= n0) ++ci;
-      if (ci < cuts.size() && cuts[ci] < n1) n1 = cuts[ci];
-      if ((rc = fs_iterate(&d.c, d.ctr, d.fac, d.fcap, n0, n1, path, d.s))) return rc;
-      ++r.iterate_launches;
-      n0 = n1;
+    for (uint32_t k = 0; k < L; ++k) {
+      size_t ci = 0;
+      for (uint32_t n0 = nb[k]; n0 < nb[k + 1];) {
+        uint32_t n1 = (nb[k + 1] - n0 > nchunk_max) ? n0 + nchunk_max : nb[k + 1];
+        while (ci < cuts.size() && cuts[ci] <= n0) ++ci;
+        if (ci < cuts.size() && cuts[ci] < n1) n1 = cuts[ci];
+        if ((rc = fs_iterate(&views[k], d.ctr, d.fac, d.fcap, n0, n1, path, ls[k]))) return rc;
+        ++r.iterate_launches;
+        n0 = n1;
+      }
+      if (k > 0) {  // join lane k
+        if (hipEventRecord(d.xev[k], d.xs[k - 1]) != hipSuccess ||
+            hipStreamWaitEvent((hipStream_t)d.s, d.xev[k], 0) != hipSuccess)
+          return FS_E_HIP;
+      }
     }
     if ((rc = fs_event_record(d.ev[5], d.s))) return rc;
-    if ((rc = fs_check(&d.c, d.ctr, d.sum, nmax, d.s))) return rc;
+    if ((rc = fs_check(&views[L - 1], d.ctr, d.sum, nmax, d.s))) return rc;
+    for (uint32_t k = 1; k < L; ++k) r.setup_mulmods += (uint64_t)count * lterms[k];
+    if (L > 1) r.lanes = L;
     r.steps += (uint64_t)count * (nmax - nmin);
     r.setup_mulmods += (uint64_t)count * d.prog_terms;
     ++r.segments;

@@ -289,6 +289,8 @@ typedef struct {
   int path;                 /* path used */
   uint32_t segments;
   uint32_t resumed;         /* 1 if resumed from a checkpoint */
+  uint32_t lanes;           /* lanes per prime of a one-segment job that underfills the GPU (its n-range
+                               cut into subranges run side by side); 1 otherwise */
 } fs_result;
 
 void fs_default_options(fs_options* o);
