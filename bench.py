@@ -12,18 +12,19 @@ jobs.  One JSON line is printed.
 
 roofline: the dominant kernel is the residue iteration.  Its work is counted
 in VALU lane-operations per p*n step of the compiled 8-step loop, timed with
-HIP events on the launch stream.  On the FP64 path (p < 2^53): 4.0 with paired
+HIP events on the launch stream.  On the FP64 path (p < 2^53): 3.5 with paired
 steps, used when nmax <= 2^22 and 64 nmax <= 31 p (one 6-operation product by
-(n+1)(n+2) per two steps -- 2 multiplies, 2 FMAs, 1 add, 1 round -- and two
-exact compares, |v| == 1 on the residue of n and |v'| == n+2 on the product,
-which find +-1 at n and n+1), 3.0 when also p nmax^2 <= 2^53 (exact product:
-2 multiplies, 1 FMA, 1 round); otherwise 6.5 with single steps (one compare per
-two steps: |v[k+1]| <= m[k] covers +-1 at k and k+1); 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
-2 v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, and 6 filter ops per 8 steps), 7.125
+(n+1)(n+2) per two steps -- 2 multiplies, 2 FMAs, 1 add, 1 round -- and one
+compare, |v'| <= the block's last n+2, which fires when the residue of n+1
+(then v' = +-(n+2)) or of n+2 (then v' = +-1) is +-1), 2.5 when also
+p nmax^2 <= 2^53 (exact product: 2 multiplies, 1 FMA, 1 round); otherwise 6.5
+with single steps (one compare per two steps: |v[k+1]| <= m[k] covers +-1 at k
+and k+1); 11.75 on the hybrid path of p > 2^53 (6 FP64 quotient ops,
+2 v_mad_u64_u32, 2 v_mul_lo_u32, 1 add, and 6 filter ops per 8 steps), 7.0
 with paired steps when nmax <= 2^22 and every modulus is in [2^45, 2^64 - 2^58)
 (per two steps one product by (n+1)(n+2): 2 v_mad_u64_u32, 4 v_mul_lo_u32,
 2 v_add3_u32 and 1 FMA for the quotient, 2 conversions, 1 multiply and 1 FMA
-for the next quotient estimate, 1 filter minimum; 2 more per 16 steps), 9.625
+for the next quotient estimate, 1 filter minimum; 112 per 16 steps), 9.625
 when every modulus is at least nmax * 2^34 (below 2^63; nmax * 2^39 above)
 and the quotient comes from the high word of the residue alone (4 FP64 ops
 instead of 6), and 4.5 on the FP64
@@ -61,11 +62,11 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12       # lane-ops/s, 2.4 GHz peak engine
 # per 8 steps, hybrid 94, Montgomery ~45 per step
 OPS_PER_STEP = {1: 6.5, 2: 11.75, 3: 45}
 OPS_PER_STEP_HYBRID_HI = 9.625                  # hybrid with the high-word quotient: 77 per 8 steps
-OPS_PER_STEP_HYBRID_PAIR = 7.125                # hybrid paired steps (nmax <= 2^22, 2^45 <= p < 2^64 - 2^58):
-                                                # 8 x 13 + 8 filter minimums + 2 per 16 steps
+OPS_PER_STEP_HYBRID_PAIR = 7.0                  # hybrid paired steps (nmax <= 2^22, 2^45 <= p < 2^64 - 2^58):
+                                                # 8 x 13 + 7 filter minimums + 1 compare per 16 steps
 OPS_PER_STEP_FP64_EXACT = 4.5                   # FP64 with an exact product (p * nmax <= 2^53): 36 per 8
-OPS_PER_STEP_FP64_PAIR = 4.0                    # FP64 paired steps (64 nmax <= 31 p): 4 x 6 + 8 compares per 8
-OPS_PER_STEP_FP64_PAIR_EXACT = 3.0              # paired steps with an exact product (p nmax^2 <= 2^53): 4 x 4 + 8
+OPS_PER_STEP_FP64_PAIR = 3.5                    # FP64 paired steps (64 nmax <= 31 p): 8 x (6 + 1 compare) per 16
+OPS_PER_STEP_FP64_PAIR_EXACT = 2.5              # paired steps with an exact product (p nmax^2 <= 2^53): 8 x (4 + 1)
 
 
 def fp64_ops_at(pmin, pmax, n1):

@@ -222,45 +222,45 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     double s = fs_canon_to_fp(*res, p);
     uint32_t n = n0;
     if (fs_fp_pair_ok(p, n1)) {
-      // The paired loop: one product by M = a*(a+1) per two positions, the
-      // compares |v| == 1 on its input and |v| == a+1 on its result (each
-      // pair's compares are exact on their own, so the device's block length
-      // does not matter here).  The odd positions' residues are formed here by
-      // single steps only to check that the compares fire exactly for the
-      // pairs that hold a +-1 residue.
+      // The paired loop: one product by M = a*(a+1) per two positions; a
+      // block from nb owns positions nb+1 .. nb+U and its filter is one
+      // compare per pair, |v| <= nb+U on the product.  The odd positions'
+      // residues are formed here by single steps to check each pair against
+      // two single steps and that the filter fires for every block holding a
+      // +-1; position n0 is tested at the start and the positions after the
+      // last whole block by single steps.
       const bool pexact = fs_fp_pair_exact_ok(p, n1);
       auto pstep = [&](double x, uint64_t M) {
         return pexact ? fs_fp_mulmod_exact(x, (double)M, pd, pinv) : fs_fp_mulmod(x, (double)M, pd, pinv);
       };
+      if (n0 < n1 && __builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n0 : (int32_t)n0);
       for (; n < nu; n += U) {
-        double v[U / 2 + 1];
+        double v[U / 2 + 1], u[U / 2];
         v[0] = s;
+        bool fire = false, any = false;
         for (uint32_t i = 0; i < U / 2; ++i) {
           const uint64_t a = n + 1 + 2 * i;
           v[i + 1] = pstep(v[i], a * (a + 1));
           if (__builtin_fabs(v[i + 1]) > 0.51 * pd) return FS_E_VALIDATE;  // the loop's invariant
-          const bool hit = (__builtin_fabs(v[i]) == 1.0) | (__builtin_fabs(v[i + 1]) == (double)(a + 1));
-          const double u = fs_fp_mulmod(v[i], (double)a, pd, pinv);
-          const double odd = pstep(u, a + 1);
-          if (fs_fp_to_canon(odd, p) != fs_fp_to_canon(v[i + 1], p)) return FS_E_VALIDATE;
-          const double r[2] = {v[i], u};  // residues of n + 2i and n + 2i + 1
-          bool any = false;
+          fire |= __builtin_fabs(v[i + 1]) <= (double)(n + U);
+          u[i] = fs_fp_mulmod(v[i], (double)a, pd, pinv);
+          if (fs_fp_to_canon(pstep(u[i], a + 1), p) != fs_fp_to_canon(v[i + 1], p)) return FS_E_VALIDATE;
+        }
+        for (uint32_t i = 0; i < U / 2; ++i) {
+          const double r[2] = {u[i], v[i + 1]};  // residues of n + 2i + 1 and n + 2i + 2
           for (uint32_t k = 0; k < 2; ++k) {
-            if (__builtin_fabs(r[k]) != 1.0) continue;
+            const uint32_t nn = n + 2 * i + 1 + k;
+            if (nn >= n1 || __builtin_fabs(r[k]) != 1.0) continue;
             any = true;
-            const uint32_t nn = n + 2 * i + k;
             rec(r[k] > 0.0 ? -(int32_t)nn : (int32_t)nn);
           }
-          // A +-1 residue always makes a compare fire.  A compare without one
-          // needs a+1 not invertible mod p (a composite probable prime).
-          if (any && !hit) return FS_E_VALIDATE;
-          if (hit && !any && std::gcd(a + 1, p) == 1) return FS_E_VALIDATE;
         }
+        if (any && !fire) return FS_E_VALIDATE;  // the filter would have missed it
         s = v[U / 2];
       }
       for (; n < n1; ++n) {
-        if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
         s = pstep(s, n + 1);
+        if (n + 1 < n1 && __builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)(n + 1) : (int32_t)(n + 1));
       }
       *res = fs_fp_to_canon(s, p);
       *nf = cnt;
@@ -286,52 +286,48 @@ extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1
     *res = fs_fp_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID && fs_sh_pair_ok(p, n1)) {
     // The paired loop: one product by M = a*(a+1) per two positions, each
-    // leaving the estimate y of its result over p; a block's filter is
-    // |y| <= 1.5 (n1 + 2)/p over its input and its U/2 results.  Here each
-    // pair is also checked against two single steps and the loop's residue
-    // bound, and a +-1 residue in a block whose filter does not fire is an
-    // error.
+    // leaving the estimate y of its result over p; a block from nb owns
+    // positions nb+1 .. nb+U and its filter is |y| <= 1.5 (n1 + 2)/p over its
+    // U/2 results.  Here each pair is also checked against two single steps
+    // and the loop's residue bound, and a +-1 residue in a block whose filter
+    // does not fire is an error.
     const double pinv = 1.0 / (double)p, pinv32 = pinv * 4294967296.0;
     const double thr = 1.5 * (double)(n1 + 2) * pinv;
     const uint64_t pn = 0 - p, cpair = fs_sh_pair_c(pn), c31 = p << 31;
     int64_t s = fs_canon_to_signed(*res, p);
+    if (n0 < n1 && ((s == 1) | (s == -1))) rec(s == 1 ? -(int32_t)n0 : (int32_t)n0);
     double ys = fs_sh_estimate(s, pinv, pinv32);
     uint32_t n = n0;
     for (; n < nu; n += U) {
       int64_t v[U / 2 + 1];
-      double y[U / 2 + 1];
       v[0] = s;
-      y[0] = ys;
-      bool fire = __builtin_fabs(y[0]) <= thr;
+      bool fire = false, any = false;
       for (uint32_t i = 0; i < U / 2; ++i) {
         const uint64_t a = n + 1 + 2 * i, M = a * (a + 1);
-        v[i + 1] = fs_sh_mulmod_pair(v[i], y[i], (uint32_t)M, (uint32_t)(M >> 32), (double)M, pn, cpair, pinv,
-                                     pinv32, &y[i + 1]);
+        v[i + 1] = fs_sh_mulmod_pair(v[i], ys, (uint32_t)M, (uint32_t)(M >> 32), (double)M, pn, cpair, pinv,
+                                     pinv32, &ys);
         if (__builtin_fabs((double)v[i + 1]) > 0.51 * (double)p) return FS_E_VALIDATE;  // the loop's bound
-        fire |= __builtin_fabs(y[i + 1]) <= thr;
+        fire |= __builtin_fabs(ys) <= thr;
         double yu;
         const int64_t u = fs_sh_mulmod(v[i], (uint32_t)a, (double)a, c31, pn, pinv, pinv32, &yu);
         const int64_t two = fs_sh_mulmod(u, (uint32_t)(a + 1), (double)(a + 1), c31, pn, pinv, pinv32, &yu);
         if (fs_signed_to_canon(two, p) != fs_signed_to_canon(v[i + 1], p)) return FS_E_VALIDATE;
       }
-      for (uint32_t i = 0; i < U / 2; ++i) {
-        const uint32_t nn = n + 2 * i;
+      int64_t w = v[0];
+      for (uint32_t nn = n; nn < n + U; ++nn) {  // positions n + 1 .. n + U
         double yu;
-        const int64_t u = fs_sh_mulmod(v[i], nn + 1, (double)(nn + 1), c31, pn, pinv, pinv32, &yu);
-        const int64_t r[2] = {v[i], u};  // residues of nn and nn + 1
-        for (uint32_t k = 0; k < 2; ++k) {
-          if ((r[k] != 1) & (r[k] != -1)) continue;
-          if (!fire) return FS_E_VALIDATE;  // the filter would have missed it
-          rec(r[k] == 1 ? -(int32_t)(nn + k) : (int32_t)(nn + k));
-        }
+        w = fs_sh_mulmod(w, nn + 1, (double)(nn + 1), c31, pn, pinv, pinv32, &yu);
+        if (nn + 1 >= n1 || ((w != 1) & (w != -1))) continue;
+        any = true;
+        rec(w == 1 ? -(int32_t)(nn + 1) : (int32_t)(nn + 1));
       }
+      if (any && !fire) return FS_E_VALIDATE;  // the filter would have missed it
       s = v[U / 2];
-      ys = y[U / 2];
     }
     for (; n < n1; ++n) {
-      if ((s == 1) | (s == -1)) rec(s == 1 ? -(int32_t)n : (int32_t)n);
       double y;
       s = fs_sh_mulmod(s, n + 1, (double)(n + 1), c31, pn, pinv, pinv32, &y);
+      if (n + 1 < n1 && ((s == 1) | (s == -1))) rec(s == 1 ? -(int32_t)(n + 1) : (int32_t)(n + 1));
     }
     *res = fs_signed_to_canon(s, p);
   } else if (path == FS_PATH_HYBRID) {

@@ -404,11 +404,15 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_loop(uint64_t p, uint64_t 
 // pairs) past index n1 - n0.  With v[i] the residue of nn = nb + 2i and a+1 =
 // nn+2 the pair's second multiplier, v[i+1] = v[i] * M = (residue of nn+1) *
 // (a+1); a+1 being below 0.4844 p and every residue of magnitude <= 0.503 p,
-// |v[i+1]| == a+1 exactly when the residue of nn+1 is +-1, and |v[i]| == 1
-// exactly when the residue of nn is.  Per two steps: one six-operation
-// product (four when EXACT) and two compares.  The taken branch (a +-1 residue
-// in the block) recomputes the odd positions with single steps and records
-// every +-1 in order.
+// v[i+1] == +-(a+1) exactly when the residue of nn+1 is +-1, and |v[i+1]| == 1
+// exactly when the residue of nn+2 is.  So a block of H pairs from position
+// nb owns positions nb+1 .. nb+U, and one compare per pair, |v[i+1]| <= nb+U
+// (the block's last a+1, a scalar), fires for every +-1 among them (and, with
+// probability about 2U n/p, for other small residues).  The taken branch
+// recomputes the odd positions with single steps and records every +-1 below
+// n1 in order.  Position n0 is tested once at the start; the positions after
+// the last whole block belong to the single-step tail.  Per two steps: one
+// six-operation product (four when EXACT) and one compare.
 template <int U, bool EXACT, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint64_t s_in, uint32_t n0, uint32_t n1,
                                                                 const double* __restrict__ nt,
@@ -417,6 +421,7 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint
   constexpr int H = U / 2;
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = fs_canon_to_fp(s_in, p);
+  if (n0 < n1 && __builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n0 : (int32_t)n0);
   uint32_t n = n0;
   const uint32_t nu = n0 + ((n1 - n0) & ~(uint32_t)(U - 1));
   // One block of H paired steps from position nb with the pairs in w, and its
@@ -433,15 +438,15 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint
     // wave when any lane's compare fired
     uint64_t hit = 0;
 #pragma unroll
-    for (int i = 0; i < H; ++i)
-      hit |= __ballot(__builtin_fabs(v[i]) == 1.0) | __ballot(__builtin_fabs(v[i + 1]) == w[2 * i + 1]);
+    for (int i = 0; i < H; ++i) hit |= __ballot(__builtin_fabs(v[i + 1]) <= w[U - 1]);
     if (hit) {
 #pragma unroll
       for (int i = 0; i < H; ++i) {
         const uint32_t nn = nb + 2 * i;
-        if (__builtin_fabs(v[i]) == 1.0) rec(v[i] > 0.0 ? -(int32_t)nn : (int32_t)nn);
         const double u = fs_fp_mulmod(v[i], nt[nn - n0], pd, pinv);  // residue of nn + 1
-        if (__builtin_fabs(u) == 1.0) rec(u > 0.0 ? -(int32_t)(nn + 1) : (int32_t)(nn + 1));
+        if (nn + 1 < n1 && __builtin_fabs(u) == 1.0) rec(u > 0.0 ? -(int32_t)(nn + 1) : (int32_t)(nn + 1));
+        if (nn + 2 < n1 && __builtin_fabs(v[i + 1]) == 1.0)
+          rec(v[i + 1] > 0.0 ? -(int32_t)(nn + 2) : (int32_t)(nn + 2));
       }
     }
   };
@@ -470,9 +475,9 @@ __device__ __forceinline__ uint64_t fs_cand_iter_fp64_pair_loop(uint64_t p, uint
     block(wb, n);
     n += U;
   }
-  for (; n < n1; ++n) {
-    if (__builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)n : (int32_t)n);
+  for (; n < n1; ++n) {  // positions nu + 1 .. n1 - 1
     s = EXACT ? fs_fp_mulmod_exact(s, nt[n - n0], pd, pinv) : fs_fp_mulmod(s, nt[n - n0], pd, pinv);
+    if (n + 1 < n1 && __builtin_fabs(s) == 1.0) rec(s > 0.0 ? -(int32_t)(n + 1) : (int32_t)(n + 1));
   }
   return fs_fp_to_canon(s, p);
 }
@@ -564,13 +569,14 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_loop(uint64_t p, uint64_
 // on the scalar unit from the uniform a, its double read from the FP64 pair
 // table pt (as for the FP64 path, pt[2j] = (n0+1+2j)*(n0+2+2j)), with the
 // same double-buffered scalar loads.  Each step leaves the estimate y of its
-// result over p, which the next step needs for its quotient; the block's
-// filter is |y| <= 1.5 (n1 + 2)/p over its input and all H results, which
-// holds whenever one of them is +-1 or +-(a+1), i.e. whenever a residue of
-// positions nb - 1 .. nb + U is +-1 (by the estimate's error bound).  The
-// taken branch tests positions nb .. nb + U - 1 exactly, the odd ones by one
-// single step each; the positions just outside are the neighbouring blocks'.
-// Per two steps: thirteen operations and one of the filter, plus two per block.
+// result over p, which the next step needs for its quotient.  As on the FP64
+// path a block of H pairs from nb owns positions nb+1 .. nb+U: its results
+// are +-1 or +-(a+1) whenever one of those residues is +-1, and its filter
+// |y| <= 1.5 (n1 + 2)/p over the H result estimates then holds (by the
+// estimate's error bound).  The taken branch re-tests the block's positions
+// below n1 exactly with single steps; position n0 is tested once at the start
+// and the positions after the last whole block by the single-step tail.  Per
+// two steps: thirteen operations and one of the filter, plus one per block.
 template <int U, class Rec>
 __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_pair_loop(uint64_t p, uint64_t s_in, uint32_t n0,
                                                                   uint32_t n1, const double* __restrict__ nt,
@@ -581,6 +587,7 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_pair_loop(uint64_t p, ui
   const double thr = 1.5 * (double)(n1 + 2) * pinv;
   const uint64_t pn = 0 - p, c31 = p << 31, cpair = fs_sh_pair_c(pn);
   int64_t s = fs_canon_to_signed(s_in, p);
+  if (n0 < n1 && ((s == 1) | (s == -1))) rec(s == 1 ? -(int32_t)n0 : (int32_t)n0);
   double ys = fs_sh_estimate(s, pinv, pinv32);
   uint32_t n = n0;
   const uint32_t nu = n0 + ((n1 - n0) & ~(uint32_t)(U - 1));
@@ -593,21 +600,21 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_pair_loop(uint64_t p, ui
     const int64_t sb = s;
     uint32_t a = nb + 1;
     asm volatile("" : "+s"(a));  // a uniform 32-bit scalar: the words of M on the scalar unit
-    double ymin = __builtin_fabs(ys);
+    double ymin = 1.0;
 #pragma unroll
     for (int i = 0; i < H; ++i) {
       const uint32_t ai = a + 2 * i;
       const uint64_t M = (uint64_t)ai * (ai + 1);
       s = fs_sh_mulmod_pair(s, ys, (uint32_t)M, (uint32_t)(M >> 32), w[2 * i], pn, cpair, pinv, pinv32, &ys);
-      ymin = __builtin_fmin(ymin, __builtin_fabs(ys));
+      ymin = i == 0 ? __builtin_fabs(ys) : __builtin_fmin(ymin, __builtin_fabs(ys));
     }
     if (__ballot(ymin <= thr)) {  // whole wave: exact tests where any lane's filter passed
       int64_t v = sb;
 #pragma unroll 1
       for (uint32_t nn = nb; nn < nb + U; ++nn) {
-        if ((v == 1) | (v == -1)) rec(v == 1 ? -(int32_t)nn : (int32_t)nn);
         double yu;
-        v = fs_sh_mulmod(v, nn + 1, nt[nn - n0], c31, pn, pinv, pinv32, &yu);
+        v = fs_sh_mulmod(v, nn + 1, nt[nn - n0], c31, pn, pinv, pinv32, &yu);  // residue of nn + 1
+        if (nn + 1 < n1 && ((v == 1) | (v == -1))) rec(v == 1 ? -(int32_t)(nn + 1) : (int32_t)(nn + 1));
       }
     }
   };
@@ -631,10 +638,10 @@ __device__ __forceinline__ uint64_t fs_cand_iter_hybrid_pair_loop(uint64_t p, ui
     block(wb, n);
     n += U;
   }
-  for (; n < n1; ++n) {
-    if ((s == 1) | (s == -1)) rec(s == 1 ? -(int32_t)n : (int32_t)n);
+  for (; n < n1; ++n) {  // positions nu + 1 .. n1 - 1
     double y;
     s = fs_sh_mulmod(s, n + 1, nt[n - n0], c31, pn, pinv, pinv32, &y);
+    if (n + 1 < n1 && ((s == 1) | (s == -1))) rec(s == 1 ? -(int32_t)(n + 1) : (int32_t)(n + 1));
   }
   return fs_signed_to_canon(s, p);
 }

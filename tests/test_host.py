@@ -363,8 +363,8 @@ def test_fp64_pair_loop_matches_oracle(fs):
     above p, from random start residues and from residues that put +-1 at
     every position of an 8-step block.  The host execution also returns an
     error if one paired step ever differs from two single steps, leaves
-    |s| > 0.51p, or if a pair's compares (|v| == 1 on its input, |v'| == a+1 on
-    its product) disagree with an exact test of its residues."""
+    |s| > 0.51p, or if a block holds a +-1 residue and its filter (one compare
+    per pair, |v'| <= the block's last a+1) does not fire."""
     rng = random.Random(43)
     cases = []
     for n1 in (1040, 5003, 20000, 300000, 2 ** 22):
