@@ -44,12 +44,25 @@ class Cands(ctypes.Structure):
                 ("grid_hint", u32)]
 
 
+FRACTION_FN = ctypes.CFUNCTYPE(None, ctypes.c_double, ctypes.c_void_p)
+SECTION_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+TRICKLE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p)
+
+
+class BoincHooks(ctypes.Structure):
+    """fs_boinc_hooks: the BOINC API calls of a run (cl_sieve.cpp:47-69, 1119-1139)."""
+    _fields_ = [("standalone", ctypes.c_int), ("fraction_done", FRACTION_FN),
+                ("begin_critical_section", SECTION_FN), ("end_critical_section", SECTION_FN),
+                ("checkpoint_completed", SECTION_FN), ("send_trickle_up", TRICKLE_FN),
+                ("starting_elapsed_time", ctypes.c_double), ("user", ctypes.c_void_p)]
+
+
 class Options(ctypes.Structure):
     _fields_ = [("path", ctypes.c_int), ("verify_threads", ctypes.c_int), ("write_files", ctypes.c_int),
                 ("resume", ctypes.c_int), ("quiet", ctypes.c_int), ("test_mode", ctypes.c_int),
                 ("seg_candidates", u32), ("n_chunk", u32),
                 ("checkpoint_secs", u32), ("workdir", ctypes.c_char_p), ("stop_after_segments", u64),
-                ("verify_device", ctypes.c_int)]
+                ("verify_device", ctypes.c_int), ("boinc", ctypes.POINTER(BoincHooks))]
 
 
 class WorkStatus(ctypes.Structure):
@@ -94,6 +107,10 @@ class FactSieveError(RuntimeError):
 # (name, restype, argtypes) of every C-ABI entry point used from Python
 _SIGS = [
     ("fs_init", ctypes.c_int, [ctypes.c_int]),
+    ("fs_trickle_up_text", ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p,
+                                          ctypes.c_size_t]),
+    ("fs_read_init_data", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
+                                         ctypes.c_size_t]),
     ("fs_device_info", ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(u32),
                                       ctypes.POINTER(u64), ctypes.POINTER(u64)]),
     ("fs_last_error", ctypes.c_char_p, []),
@@ -263,6 +280,23 @@ def write_checkpoint(directory, st, which):
     return st
 
 
+def trickle_up_text(progress, cputime, runtime):
+    """The daily trickle-up message (cl_sieve.cpp:59-64)."""
+    buf = ctypes.create_string_buffer(512)
+    check(lib().fs_trickle_up_text(progress, cputime, runtime, buf, 512))
+    return buf.value.decode()
+
+
+def read_init_data(directory):
+    """(gpu_device_num, gpu_type) from init_data.xml in `directory`, or None
+    when there is no such file; FactSieveError when it has no valid
+    <gpu_device_num>."""
+    dev = ctypes.c_int(-1)
+    typ = ctypes.create_string_buffer(64)
+    r = check(lib().fs_read_init_data(str(directory).encode(), ctypes.byref(dev), typ, 64))
+    return (dev.value, typ.value.decode()) if r == 1 else None
+
+
 def parse_uint64(s, lo, hi):
     """(status, value): status 0 ok, -1 cannot parse, -2 out of range (putil.c)."""
     out = u64(0)
@@ -373,6 +407,8 @@ def default_options(**kw):
     for k, v in kw.items():
         if k == "workdir" and v is not None:
             v = v.encode() if isinstance(v, str) else v
+        if k == "boinc" and isinstance(v, BoincHooks):
+            v = ctypes.pointer(v)  # the caller keeps the hooks alive for the call
         setattr(o, k, v)
     return o
 

@@ -180,6 +180,19 @@ __global__ void k_fill_ntab(double* t, double* pt, uint32_t half, uint32_t first
   }
 }
 
+// Lane copies of a segment's candidates (p, q, r2): entry i of lane k is
+// entry i; grid-stride over (lanes - 1) * count entries.
+__global__ void k_replicate(fs_cands c, uint32_t count, uint32_t lanes) {
+  const uint64_t total = (uint64_t)(lanes - 1) * count;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t i = (uint32_t)(t % count);
+    const uint64_t dst = t + count;
+    c.p[dst] = c.p[i];
+    c.q[dst] = c.q[i];
+    c.r2[dst] = c.r2[i];
+  }
+}
+
 // ---- check: checksum fold + count + validation (check.cl:50-109)
 
 __global__ __launch_bounds__(FS_BLOCK) void k_check(fs_cands c, uint32_t* ctr, uint64_t* sum,
@@ -426,6 +439,18 @@ extern "C" int fs_inject_fault(int idx) {
     return FS_E_ARG;
   }
   g_inject = idx;
+  return FS_OK;
+}
+
+int fs_cands_replicate(const fs_cands* c, uint32_t count, uint32_t lanes, fs_stream s) {
+  if (lanes < 2 || count == 0) return FS_OK;
+  if ((uint64_t)lanes * count > c->cap) {
+    g_last_error = "cands_replicate: capacity below lanes * count";
+    return FS_E_ARG;
+  }
+  const uint64_t total = (uint64_t)(lanes - 1) * count;
+  k_replicate<<<(uint32_t)std::min<uint64_t>((total + 255) / 256, 4096), 256, 0, S(s)>>>(*c, count, lanes);
+  HIPCHK(hipGetLastError(), "cands_replicate");
   return FS_OK;
 }
 

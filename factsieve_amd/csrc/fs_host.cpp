@@ -12,6 +12,7 @@
 #include <errno.h>
 #include <math.h>
 #include <time.h>
+#include <sys/resource.h>
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -285,10 +286,22 @@ static int64_t build_program(uint32_t nmin, int path, std::vector<uint64_t>* out
   return (int64_t)prog.size();
 }
 
+namespace {
+std::mutex g_prog_mu;
+std::map<std::pair<uint32_t, int>, std::shared_ptr<const fs_program>> g_prog_cache;
+size_t g_prog_words = 0;
+}  // namespace
+
+std::shared_ptr<const fs_program> fs_program_cached(uint32_t nmin, int path) {
+  std::lock_guard<std::mutex> lk(g_prog_mu);
+  auto it = g_prog_cache.find(std::make_pair(nmin, path));
+  return it != g_prog_cache.end() ? it->second : nullptr;
+}
+
 std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path) {
-  static std::mutex mu;
-  static std::map<std::pair<uint32_t, int>, std::shared_ptr<const fs_program>> cache;
-  static size_t cached_words = 0;
+  std::mutex& mu = g_prog_mu;
+  auto& cache = g_prog_cache;
+  size_t& cached_words = g_prog_words;
   const auto key = std::make_pair(nmin, path);
   {
     std::lock_guard<std::mutex> lk(mu);
@@ -1038,8 +1051,12 @@ int fs_validate_ranges(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nma
 }
 
 uint32_t fs_auto_segment_candidates() {
-  uint32_t cus = 0;
-  if (fs_device_info(nullptr, 0, &cus, nullptr, nullptr) != FS_OK || cus == 0) cus = 256;
+  static std::atomic<uint32_t> cached{0};  // the compute units of the device (one device per process here)
+  uint32_t cus = cached.load();
+  if (!cus) {
+    if (fs_device_info(nullptr, 0, &cus, nullptr, nullptr) != FS_OK || cus == 0) cus = 256;
+    cached.store(cus);
+  }
   // 8 waves of 64 lanes on each of the 4 SIMDs of every CU, times 8 rounds
   return cus * 4u * 8u * 64u * 8u;
 }
@@ -1117,6 +1134,97 @@ static uint32_t split_lanes(uint32_t count, uint32_t nmin, uint32_t nmax, int pa
   return std::max<uint32_t>(L, 1);
 }
 
+extern "C" int fs_trickle_up_text(double progress, double cputime, double runtime, char* buf, size_t len) {
+  if (!buf) return FS_E_ARG;
+  const int n = snprintf(buf, len,
+                         "<trickle_up>\n"
+                         "   <progress>%lf</progress>\n"
+                         "   <cputime>%lf</cputime>\n"
+                         "   <runtime>%lf</runtime>\n"
+                         "</trickle_up>\n",
+                         progress, cputime, runtime);
+  if (n < 0 || (size_t)n >= len) return FS_E_ARG;
+  return n;
+}
+
+extern "C" int fs_read_init_data(const char* dir, int* gpu_device_num, char* gpu_type, size_t gpu_type_len) {
+  if (!gpu_device_num) return FS_E_ARG;
+  FILE* f = fopen(in_dir(dir, "init_data.xml").c_str(), "rb");
+  if (!f) return 0;
+  std::string text;
+  char buf[4096];
+  size_t got;
+  while ((got = fread(buf, 1, sizeof buf, f)) > 0 && text.size() < (1u << 20)) text.append(buf, got);
+  fclose(f);
+  auto field = [&](const char* tag, std::string* out) {
+    const std::string open = std::string("<") + tag + ">", close = std::string("</") + tag + ">";
+    const size_t a = text.find(open);
+    if (a == std::string::npos) return false;
+    const size_t b = text.find(close, a + open.size());
+    if (b == std::string::npos) return false;
+    *out = text.substr(a + open.size(), b - a - open.size());
+    return true;
+  };
+  std::string num, type;
+  if (gpu_type && gpu_type_len) {
+    gpu_type[0] = 0;
+    if (field("gpu_type", &type)) snprintf(gpu_type, gpu_type_len, "%s", type.c_str());
+  }
+  if (!field("gpu_device_num", &num)) {
+    fs_set_error_msg("init_data.xml: no <gpu_device_num>");
+    return FS_E_ARG;
+  }
+  size_t a = num.find_first_not_of(" \t\r\n"), b = num.find_last_not_of(" \t\r\n");
+  if (a == std::string::npos) {
+    fs_set_error_msg("init_data.xml: empty <gpu_device_num>");
+    return FS_E_ARG;
+  }
+  num = num.substr(a, b - a + 1);
+  if (num.size() > 2 || num.find_first_not_of("0123456789") != std::string::npos || std::stoi(num) > 63) {
+    fs_set_error_msg("init_data.xml: <gpu_device_num> is not a device index 0..63");
+    return FS_E_ARG;
+  }
+  *gpu_device_num = std::stoi(num);
+  return 1;
+}
+
+namespace {
+// The BOINC hooks of one run (all optional).
+struct Boinc {
+  const fs_boinc_hooks* h;
+  double last_fraction = 0.0;
+  explicit Boinc(const fs_boinc_hooks* hooks) : h(hooks) {}
+  void fraction(double fd) {
+    last_fraction = fd;
+    if (h && h->fraction_done) h->fraction_done(fd, h->user);
+  }
+  void begin() {
+    if (h && h->begin_critical_section) h->begin_critical_section(h->user);
+  }
+  void end() {
+    if (h && h->end_critical_section) h->end_critical_section(h->user);
+  }
+  void checkpoint_done() {
+    if (h && h->checkpoint_completed) h->checkpoint_completed(h->user);
+  }
+  // handle_trickle_up (cl_sieve.cpp:47-69): once a day, unless standalone
+  void trickle(WorkStatus& st, double run_s) {
+    if (!h || h->standalone || !h->send_trickle_up) return;
+    const uint64_t now = (uint64_t)time(nullptr);
+    if (now - st.last_trickle <= 86400) return;
+    st.last_trickle = now;
+    struct rusage ru;
+    double cpu = 0.0;
+    if (getrusage(RUSAGE_SELF, &ru) == 0)
+      cpu = (double)ru.ru_utime.tv_sec + ru.ru_utime.tv_usec * 1e-6 + (double)ru.ru_stime.tv_sec +
+            ru.ru_stime.tv_usec * 1e-6;
+    char msg[512];
+    if (fs_trickle_up_text(last_fraction, cpu, run_s + h->starting_elapsed_time, msg, sizeof msg) > 0)
+      h->send_trickle_up("factsieve_progress", msg, h->user);
+  }
+};
+}  // namespace
+
 extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax, const fs_options* opt,
                         fs_result* res, fs_factor* factors, uint64_t cap, uint64_t* nfactors) {
   fs_options o;
@@ -1140,13 +1248,16 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     return FS_E_ARG;
   }
   r.path = path;
-  // The setup program is built on another host thread while the checkpoint
-  // state, the stream and the buffers are prepared.
+  // The setup program, when not cached, is built on another host thread while
+  // the checkpoint state, the stream and the buffers are prepared.
+  const std::shared_ptr<const fs_program> prog_ready = fs_program_cached(nmin, path);
   std::future<std::shared_ptr<const fs_program>> prog_fut;
-  try {
-    prog_fut = std::async(std::launch::async, [nmin, path] { return fs_program_get(nmin, path); });
-  } catch (const std::system_error&) {
-    prog_fut = std::async(std::launch::deferred, [nmin, path] { return fs_program_get(nmin, path); });
+  if (!prog_ready) {
+    try {
+      prog_fut = std::async(std::launch::async, [nmin, path] { return fs_program_get(nmin, path); });
+    } catch (const std::system_error&) {
+      prog_fut = std::async(std::launch::deferred, [nmin, path] { return fs_program_get(nmin, path); });
+    }
   }
 
   WorkStatus st;
@@ -1204,7 +1315,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   // setup program and the reference power table (verifypow, and setup on MONT),
   // uploaded when they differ from what the buffers hold
   {
-    const std::shared_ptr<const fs_program> pr = prog_fut.get();
+    const std::shared_ptr<const fs_program> pr = prog_ready ? prog_ready : prog_fut.get();
     if (!pr) return FS_E_NOMEM;
     if (d.prog_path != path || d.prog_nmin != nmin) {
       const std::vector<uint64_t>& w = pr->words;
@@ -1252,6 +1363,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   const uint32_t nchunk_max = o.n_chunk ? o.n_chunk : (1u << 22);
   bool verified = false;
   double last_ckpt = now_s(), last_prog = now_s();
+  Boinc bx(o.boinc);
   const double irsize = 1.0 / (double)(pmax - pmin ? pmax - pmin : 1);
   uint32_t prog_chunk = 1u << 16;
 
@@ -1342,12 +1454,9 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
         }
         lprog[k] = d.xprog[k - 1];
         lterms[k] = pr->terms;
-        // lane k's copy of the candidates
-        if ((rc = fs_memcpy_dtod(views[k].p, d.c.p, (size_t)count * 8, d.s)) ||
-            (rc = fs_memcpy_dtod(views[k].q, d.c.q, (size_t)count * 8, d.s)) ||
-            (rc = fs_memcpy_dtod(views[k].r2, d.c.r2, (size_t)count * 8, d.s)))
-          return rc;
       }
+      // the lanes' copies of the candidates (p, q, r2), in one launch
+      if ((rc = fs_cands_replicate(&d.c, count, L, d.s))) return rc;
     }
     if ((rc = fs_event_record(d.ev[2], d.s))) return rc;
     if (L > 1) {  // the lanes start when the copies are done
@@ -1417,21 +1526,31 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     const uint64_t nextp = st.p + len;
     const bool last = (nextp >= st.pmax || nextp < st.p);
     st.p = last ? st.pmax : nextp;
-    if ((rc = fs_stream_sync(d.s))) return rc;
+    bx.begin();  // the wait for the segment's kernels (waitOnEvent, cl_sieve.cpp:237-285)
+    rc = fs_stream_sync(d.s);
+    bx.end();
+    if (rc) return rc;
     account(d, &r);
 
     const double tn = now_s();
-    if (verbose && tn - last_prog > 2.0) {
-      printf("Sieve Progress: %.1f%%\n", (double)(st.p - pmin) * irsize * 100.0);
+    if (tn - last_prog > 2.0) {  // fraction done every 2 s (cl_sieve.cpp:1119-1124)
+      const double fd = (double)(st.p - pmin) * irsize;
+      bx.fraction(fd);
+      if (verbose) printf("Sieve Progress: %.1f%%\n", fd * 100.0);
       last_prog = tn;
     }
     const bool stop_now = o.stop_after_segments && r.segments >= o.stop_after_segments && !last;
     if (!last && (tn - last_ckpt > (double)o.checkpoint_secs || stop_now)) {
-      if ((rc = drain(d, st, o, dr, &r))) return rc;
-      if (o.write_files && !o.test_mode) {
+      bx.begin();  // drain + checkpoint in one critical section (cl_sieve.cpp:1131-1135)
+      rc = drain(d, st, o, dr, &r);
+      if (rc == FS_OK && o.write_files && !o.test_mode) {
+        bx.trickle(st, tn - t0);
         write_state(st, write_a_next, o.workdir);
         if (verbose) printf("Checkpoint, current p: %llu\n", (unsigned long long)st.p);
+        bx.checkpoint_done();
       }
+      bx.end();
+      if (rc) return rc;
       last_ckpt = tn;
     }
     if (stop_now) {
@@ -1445,12 +1564,20 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   }
 
   st.p = st.pmax;
+  bx.begin();  // the last drain and the results file (cl_sieve.cpp:1249-1256)
+  bx.fraction(1.0);
   if (verbose) printf("Sieve Progress: %.1f%%\n", 100.0);
-  if ((rc = drain(d, st, o, dr, &r))) return rc;
-  if (o.write_files) {
-    if (!o.test_mode) write_state(st, write_a_next, o.workdir);
-    if ((rc = finalize_file(st, o))) return rc;
+  rc = drain(d, st, o, dr, &r);
+  if (rc == FS_OK && o.write_files) {
+    if (!o.test_mode) {
+      bx.trickle(st, now_s() - t0);
+      write_state(st, write_a_next, o.workdir);
+      bx.checkpoint_done();
+    }
+    rc = finalize_file(st, o);
   }
+  bx.end();
+  if (rc) return rc;
   r.seconds = now_s() - t0;
   r.factorcount = st.factorcount;
   r.primecount = st.primecount;

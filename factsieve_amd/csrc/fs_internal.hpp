@@ -20,6 +20,8 @@ struct fs_program {
   std::vector<uint64_t> words;
 };
 std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path);
+// The cached program of (nmin, path), or null when it is not cached.
+std::shared_ptr<const fs_program> fs_program_cached(uint32_t nmin, int path);
 // Sort by (p, |n|); verify on CPU threads (index of first failure or -1);
 // drop composite p and fold kept factors into *checksum (returns kept count).
 void fs_sort_factors(fs_factor* f, size_t n);
@@ -38,3 +40,7 @@ uint64_t fs_cand_capacity(uint64_t lo, uint64_t len);
 // The fault injected with fs_inject_fault for counter idx, taken once: 1 when
 // it was set (and it is then cleared), else 0.
 int fs_take_injected_fault(int idx);
+// Copies p, q and r2 of the first `count` candidates of c to positions
+// k * count .. (k + 1) * count - 1 for k = 1 .. lanes - 1 (one launch on s);
+// c->cap must hold lanes * count.
+int fs_cands_replicate(const fs_cands* c, uint32_t count, uint32_t lanes, fs_stream s);

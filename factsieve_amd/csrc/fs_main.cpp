@@ -2,7 +2,9 @@
 // executable (main.cpp:27-399): same options -n -N -p -P -v -s -d -h (plus
 // --test and --device), same number syntax (putil.c), same stdout/stderr lines,
 // factors.txt and stateA/B.ckp in the working directory, exit(EXIT_FAILURE) on
-// any error.  BOINC is not linked: the program always runs standalone.
+// any error.  BOINC is not linked: the program runs standalone, and takes its
+// GPU from -d, else from an init_data.xml in the working directory
+// (<gpu_device_num>, README.md:42-56), else device 0.
 #include <getopt.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -34,6 +36,7 @@ struct Args {
   uint64_t nmin = 0, nmax = 0;
   uint64_t threads = 2;
   uint64_t device = 0;
+  bool device_set = false;
   int verify_device = FS_VERIFY_AUTO;
   bool test = false;
 };
@@ -46,7 +49,9 @@ static int parse_option(int opt, const char* arg, Args& a) {
     case 'n': return fs_parse_uint64(&a.nmin, arg, 127, 0x7FFFFFFFull - 1);
     case 'N': return fs_parse_uint64(&a.nmax, arg, 128, 0x7FFFFFFFull);
     case 'v': return fs_parse_uint64(&a.threads, arg, 1, 128);
-    case 'd': return arg ? fs_parse_uint64(&a.device, arg, 0, 63) : 0;
+    case 'd':
+      a.device_set = arg != nullptr;
+      return arg ? fs_parse_uint64(&a.device, arg, 0, 63) : 0;
     case 'V':
       if (!strcmp(arg, "cpu")) a.verify_device = FS_VERIFY_CPU;
       else if (!strcmp(arg, "gpu")) a.verify_device = FS_VERIFY_GPU;
@@ -106,6 +111,21 @@ int main(int argc, char** argv) {
     ind = -1;
   }
 
+  if (!a.device_set) {  // boinc_get_opencl_ids, stand-alone form (main.cpp:230-247)
+    int dev = 0;
+    char type[64];
+    const int found = fs_read_init_data("", &dev, type, sizeof type);
+    if (found == 1) {
+      a.device = (uint64_t)dev;
+      printf("init_data.xml: using GPU device %d (%s).\n", dev, type[0] ? type : "gpu_type not given");
+      fprintf(stderr, "init_data.xml: using GPU device %d (%s).\n", dev, type[0] ? type : "gpu_type not given");
+    } else if (found == 0) {
+      printf("init_data.xml not found, using device 0.\n");
+    } else {
+      printf("%s, using device 0.\n", fs_last_error());
+      fprintf(stderr, "%s, using device 0.\n", fs_last_error());
+    }
+  }
   int rc = fs_init((int)a.device);
   if (rc) die(fs_last_error());
   char name[256];

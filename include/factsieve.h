@@ -252,6 +252,44 @@ int fs_hostemu_sh_pair(uint64_t p, int64_t s, uint32_t a, int64_t* raw, uint64_t
 int fs_hostemu_setup_mul(uint64_t p, uint64_t s, uint64_t m, int square, uint64_t* raw, uint64_t* res,
                          uint32_t* shift);
 
+/* ------------------------------------------------ BOINC client glue (shim) */
+/* The calls the reference makes into the BOINC API during a run, as hooks a
+ * BOINC wrapper points at boinc_* (BOINC itself is not linked here); a NULL
+ * hook is skipped, and a NULL fs_options.boinc is a standalone run:
+ *   fraction_done           boinc_fraction_done (cl_sieve.cpp:1119-1124): about
+ *                           every 2 s of the segment loop, and 1.0 at the end
+ *                           (cl_sieve.cpp:1251)
+ *   begin/end_critical_...  boinc_begin/end_critical_section: around each wait
+ *                           for the GPU and each drain + checkpoint
+ *                           (cl_sieve.cpp:248-280, 1131-1135, 1249-1256)
+ *   checkpoint_completed    boinc_checkpoint_completed after a checkpoint write
+ *                           (cl_sieve.cpp:225-232)
+ *   send_trickle_up         boinc_send_trickle_up (cl_sieve.cpp:47-69): at a
+ *                           checkpoint, once a day (workStatus.last_trickle),
+ *                           unless standalone; variety "factsieve_progress",
+ *                           text as fs_trickle_up_text writes it with the last
+ *                           fraction done, the process CPU time and the run's
+ *                           elapsed time plus starting_elapsed_time. */
+typedef struct {
+  int standalone;                 /* 1: boinc_is_standalone(), no trickle-up */
+  void (*fraction_done)(double fraction, void* user);
+  void (*begin_critical_section)(void* user);
+  void (*end_critical_section)(void* user);
+  void (*checkpoint_completed)(void* user);
+  int (*send_trickle_up)(const char* variety, const char* text, void* user);
+  double starting_elapsed_time;   /* APP_INIT_DATA.starting_elapsed_time, seconds */
+  void* user;
+} fs_boinc_hooks;
+/* The trickle-up message of cl_sieve.cpp:59-64 into buf; returns its length, or
+ * FS_E_ARG when it does not fit. */
+int fs_trickle_up_text(double progress, double cputime, double runtime, char* buf, size_t len);
+/* init_data.xml in `dir` ("" = current directory), the stand-alone device choice
+ * (README.md:42-56, boinc_get_opencl_ids in main.cpp:230-247): 1 and
+ * *gpu_device_num (and the <gpu_type> text into gpu_type, when given) when the
+ * file has a <gpu_device_num>; 0 when there is no such file; FS_E_ARG when it
+ * has no valid <gpu_device_num> (0..63). */
+int fs_read_init_data(const char* dir, int* gpu_device_num, char* gpu_type, size_t gpu_type_len);
+
 /* ------------------------------------------------------- whole sieve */
 typedef struct {
   int path;                 /* FS_PATH_* (AUTO = by pmax) */
@@ -269,6 +307,7 @@ typedef struct {
                                GPU when they need more than 2^24 multiplications, else on the CPU;
                                FS_VERIFY_CPU: always on verify_threads CPU threads (the reference);
                                FS_VERIFY_GPU: always on the GPU (fs_verify_factors_gpu) */
+  const fs_boinc_hooks* boinc; /* BOINC client hooks (NULL = standalone, no hooks) */
 } fs_options;
 #define FS_VERIFY_AUTO 0
 #define FS_VERIFY_CPU 1

@@ -566,3 +566,60 @@ def test_factor_lists_in_reference_order(gpu):
     assert r.factorcount > 1000 and reference_order(r.factors)
     res, _ = gpu.batch([(127, 20000, 127, 30000), (10 ** 8, 10 ** 8 + 300000, 1000, 3000)])
     assert all(reference_order(o.factors) for o in res)
+
+
+@pytest.mark.parametrize("standalone", [0, 1])
+def test_boinc_hooks(gpu, tmp_path, standalone):
+    """A run with BOINC hooks (cl_sieve.cpp:47-69, 225-232, 1119-1139,
+    1249-1256): fraction done never decreases and ends at 1.0, critical
+    sections nest properly and hold every checkpoint, and a run resumed from a
+    checkpoint whose last trickle is over a day old sends exactly one
+    trickle-up (none when standalone).  The results equal the plain run's."""
+    import re
+    g = C1["range"]
+    calls = []
+
+    def fraction(fd, user):
+        calls.append(("fraction", fd))
+
+    def begin(user):
+        calls.append(("begin",))
+
+    def end(user):
+        calls.append(("end",))
+
+    def done(user):
+        calls.append(("checkpoint",))
+
+    def trickle(variety, text, user):
+        calls.append(("trickle", variety.decode(), text.decode()))
+        return 0
+
+    cbs = (gpu.FRACTION_FN(fraction), gpu.SECTION_FN(begin), gpu.SECTION_FN(end), gpu.SECTION_FN(done),
+           gpu.TRICKLE_FN(trickle))
+    hooks = gpu.BoincHooks(standalone, *cbs, 3600.0, None)
+    st = gpu.WorkStatus(pmin=g[0], pmax=g[1], nmin=g[2], nmax=g[3], p=g[0], last_trickle=0)
+    gpu.write_checkpoint(tmp_path, st, 0)
+    opts = dict(write_files=1, resume=1, workdir=str(tmp_path), checkpoint_secs=0, seg_candidates=8000)
+    r = gpu.sieve(*g, boinc=hooks, **opts)
+    assert r.resumed == 1 and r.segments > 2
+    assert (r.factorcount, r.primecount, f"{r.checksum:016X}") == (25, 108529, C1["checksum"])
+    fr = [c[1] for c in calls if c[0] == "fraction"]
+    assert fr and fr[-1] == 1.0 and all(a <= b for a, b in zip(fr, fr[1:]))
+    depth = 0
+    for c in calls:
+        depth += {"begin": 1, "end": -1}.get(c[0], 0)
+        assert 0 <= depth <= 1
+        if c[0] in ("checkpoint", "trickle"):
+            assert depth == 1
+    assert depth == 0 and sum(1 for c in calls if c[0] == "checkpoint") >= 2
+    tr = [c for c in calls if c[0] == "trickle"]
+    if standalone:
+        assert not tr
+    else:
+        assert len(tr) == 1 and tr[0][1] == "factsieve_progress"
+        m = re.fullmatch(r"<trickle_up>\n   <progress>([0-9.]+)</progress>\n   <cputime>([0-9.]+)</cputime>\n"
+                         r"   <runtime>([0-9.]+)</runtime>\n</trickle_up>\n", tr[0][2])
+        assert m and 0.0 <= float(m.group(1)) <= 1.0 and float(m.group(3)) >= 3600.0
+    last, _ = gpu.read_checkpoint(tmp_path, *g)
+    assert (last.last_trickle > 0) == (not standalone)

@@ -653,3 +653,53 @@ def test_checkpoint_selection_and_rejection(fs, tmp_path, capfd):
 def test_checkpoint_write_error_reported(fs, tmp_path):
     with pytest.raises(fs.FactSieveError):
         fs.write_checkpoint(tmp_path / "no" / "such" / "dir", _state(fs, 10 ** 8), 0)
+
+
+def test_trickle_up_text_is_the_reference_message(fs):
+    """The daily trickle-up record, character for character as
+    cl_sieve.cpp:59-64 formats it (%lf fields)."""
+    assert fs.trickle_up_text(0.25, 1234.5, 86400.125) == (
+        "<trickle_up>\n"
+        "   <progress>0.250000</progress>\n"
+        "   <cputime>1234.500000</cputime>\n"
+        "   <runtime>86400.125000</runtime>\n"
+        "</trickle_up>\n")
+
+
+def test_read_init_data(fs, tmp_path):
+    """The stand-alone device choice from init_data.xml (README.md:42-56):
+    both documented formats, a missing file, and files without a valid
+    <gpu_device_num>."""
+    assert fs.read_init_data(tmp_path) is None
+    for gpu_type, num in (("NVIDIA", 0), ("ATI", 2), ("ATI", 63)):
+        (tmp_path / "init_data.xml").write_text(
+            f"<app_init_data>\n<gpu_type>{gpu_type}</gpu_type>\n<gpu_device_num>{num}</gpu_device_num>\n"
+            "</app_init_data>\n")
+        assert fs.read_init_data(tmp_path) == (num, gpu_type)
+    (tmp_path / "init_data.xml").write_text("<app_init_data><gpu_device_num> 5 </gpu_device_num></app_init_data>")
+    assert fs.read_init_data(tmp_path) == (5, "")
+    for bad in ("<app_init_data></app_init_data>", "<gpu_device_num>x</gpu_device_num>",
+                "<gpu_device_num>64</gpu_device_num>", "<gpu_device_num>-1</gpu_device_num>",
+                "<gpu_device_num>1"):
+        (tmp_path / "init_data.xml").write_text(bad)
+        with pytest.raises(fs.FactSieveError):
+            fs.read_init_data(tmp_path)
+
+
+def test_cli_takes_the_device_from_init_data(fs, tmp_path):
+    """Without -d the executable names the device it takes from
+    init_data.xml, or device 0 when there is no file; with -d it reads no
+    file.  (Only the device choice is checked: the run itself needs a GPU.)"""
+    import subprocess
+    args = [fs.EXE_PATH, "-n", "1000", "-N", "1100", "-p", "100000000", "-P", "100010000"]
+
+    def first_lines():
+        r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        return r.stdout
+
+    assert "init_data.xml not found, using device 0." in first_lines()
+    (tmp_path / "init_data.xml").write_text(
+        "<app_init_data>\n<gpu_type>ATI</gpu_type>\n<gpu_device_num>0</gpu_device_num>\n</app_init_data>\n")
+    assert "init_data.xml: using GPU device 0 (ATI)." in first_lines()
+    args += ["-d", "0"]
+    assert "init_data.xml" not in first_lines()
