@@ -112,10 +112,10 @@ def hybrid_hi(pmin, pmax, nmax):
 INT32_MUL_PER_STEP = 11                         # 64-bit Montgomery step: 4 + 3 + 4 partial products
 INT32_MUL_PEAK = VALU_PEAK_TOPS * 1e12          # full-rate v_mad_u64_u32 / v_mul_*_u32 (measured)
 # HBM bytes per candidate per iterate launch, measured with rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE (k_iterate_fp64 on c2: 23.2 B read + 12.0 B written; k_iterate_hybrid
+# and WRITE_SIZE (k_iterate_fp64 on c2: 26.2 B read + 12.0 B written; k_iterate_hybrid
 # on c3: 20.4 + 12.0), each scaled by the counter's reading for a 1 GiB device copy
 # (x2.0 fetch, x1.0 write).
-TRAFFIC_BYTES_PER_CAND = {1: 35.2, 2: 32.4}
+TRAFFIC_BYTES_PER_CAND = {1: 38.2, 2: 32.4}
 
 
 def cpu_baseline(pmin, nmin, nmax, target_s=15.0):
@@ -123,7 +123,20 @@ def cpu_baseline(pmin, nmin, nmax, target_s=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol  # the oracle is used here only as the CPU baseline
 
-    threads = min(os.cpu_count() or 1, 16)
+    # the host cores this process may run on; a GPU machine's share of them is
+    # what OMP_NUM_THREADS names there (16 per GPU), and the oracle runs on all of it
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(avail, share) if share > 0 else avail
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     span = 2 * 10 ** 5
     t = time.time()
     ol.sieve(pmin, pmin + span, nmin, nmax, threads=threads)
@@ -134,6 +147,8 @@ def cpu_baseline(pmin, nmin, nmax, target_s=15.0):
     dt = time.time() - t
     steps = r["primecount"] * (nmax - nmin)
     return {"value": steps / dt, "unit": "p*n steps/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "nproc": os.cpu_count(), "cores_available": avail,
+            "omp_num_threads": share or None,
             "sample": f"oracle (C/OpenMP port of the reference algorithm) on p in [{pmin}, {pmin + span}), "
                       f"n in [{nmin}, {nmax}): {r['primecount']} candidates, {steps:.3e} p*n steps in {dt:.1f} s"}
 
@@ -143,13 +158,16 @@ def bench_batched(fs, njobs=64, seed=None):
     64 plain single-job runs; results must agree job by job."""
     from factsieve_amd import workloads
     jobs = workloads.mixed_jobs(njobs, seed=workloads.DEFAULT_SEED if seed is None else seed)
+    print("bench: autotuning the batch", file=sys.stderr, flush=True)
     opts = fs.batch_autotune(jobs)
+    print(f"bench: autotuned {opts}", file=sys.stderr, flush=True)
     # Both forms are timed after one untimed pass over the same jobs (setup
     # programs cached by nmin, device buffers sized, code objects loaded).
     fs.batch(jobs, **opts)
     t = time.perf_counter()
     res, _ = fs.batch(jobs, **opts)
     t_batch = time.perf_counter() - t
+    print(f"bench: batched {t_batch:.3f} s; plain jobs next", file=sys.stderr, flush=True)
     for j in jobs:
         fs.sieve(*j, verify_threads=16)
     t = time.perf_counter()
@@ -237,14 +255,77 @@ def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192, use_graph=
             "makespan_s": makespan, "steps_per_s": steps / makespan, "bit_exact_vs_batched_alone": same}
 
 
+def plain_leg(fs, workload, steps, warmup):
+    """`steps` timed whole jobs of a plain workload after `warmup` untimed ones:
+    (whole-job p*n steps/s, the wall time, the job results, roofline and kernel
+    times of its iterate launches)."""
+    pmin, pmax, nmin, nmax, desc = WORKLOADS[workload]
+    print(f"bench: plain {workload}, {steps} timed jobs", file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        fs.sieve(pmin, pmax, nmin, nmax, verify_threads=16)
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        results.append(fs.sieve(pmin, pmax, nmin, nmax, verify_threads=16))
+    elapsed = time.perf_counter() - t0
+    ref = results[0]
+    for r in results[1:]:
+        assert (r.checksum, r.factorcount, r.primecount) == (ref.checksum, ref.factorcount, ref.primecount)
+    steps_total = sum(r.steps for r in results)
+    it_ms = sum(r.kernel_ms_iterate for r in results)
+    launches = sum(r.iterate_launches for r in results)
+    ops = OPS_PER_STEP[ref.path]
+    if ref.path == 2 and hybrid_pair(pmin, pmax, nmax):
+        ops = OPS_PER_STEP_HYBRID_PAIR
+    elif ref.path == 2 and hybrid_hi(pmin, pmax, nmax):
+        ops = OPS_PER_STEP_HYBRID_HI
+    if ref.path == 1:
+        ops = fp64_ops(pmin, pmax, nmin, nmax)
+    achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
+    traffic = None
+    if ref.path in TRAFFIC_BYTES_PER_CAND:
+        traffic = TRAFFIC_BYTES_PER_CAND[ref.path] * ref.primecount * max(ref.lanes, 1) / max(ref.iterate_launches, 1)
+    leg = {
+        "workload": desc, "pmin": pmin, "pmax": pmax, "nmin": nmin, "nmax": nmax,
+        "candidates": ref.primecount, "pn_steps_per_job": ref.steps, "factors": ref.factorcount,
+        "checksum": f"{ref.checksum:016X}", "path": {1: "fp64", 2: "hybrid", 3: "montgomery"}[ref.path],
+        "segments": ref.segments, "lanes_per_prime": ref.lanes,
+        "steps_per_s": steps_total / elapsed, "ms_per_job": elapsed / steps * 1e3,
+        "roofline": {"bound": "valu", "kernel": "k_iterate_" + {1: "fp64", 2: "hybrid", 3: "mont"}[ref.path],
+                     "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s (VALU lane-ops)",
+                     "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic, "traffic_unit": "bytes per launch",
+                     "ops_per_step": ops, "avg_launch_ms": it_ms / max(launches, 1),
+                     "kernel_steps_per_s": steps_total / (it_ms * 1e-3)},
+        "int32_mul_equiv": {"frac": steps_total / (it_ms * 1e-3) * INT32_MUL_PER_STEP / INT32_MUL_PEAK,
+                            "muls_per_step": INT32_MUL_PER_STEP, "peak_ops_per_s": INT32_MUL_PEAK},
+        "kernel_ms_per_job": {"iterate": it_ms / steps,
+                              "setup": sum(r.kernel_ms_setup for r in results) / steps,
+                              "primegen": sum(r.kernel_ms_primegen for r in results) / steps},
+    }
+    return leg, elapsed, ref
+
+
+def valu_peaks(fs):
+    """Measured issue rate of each VALU instruction kind the kernels use."""
+    out = {}
+    for kind, name in fs.VALU_PEAK_KINDS.items():
+        rate, ms = fs.valu_peak(kind)
+        out[name] = {"lane_ops_per_s": rate, "frac_of_2p4ghz_issue": rate / (VALU_PEAK_TOPS * 1e12), "ms": ms}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c4"],
+                    help="plain workload of the headline line (c4: only the batched leg, for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched and serving legs")
+    ap.add_argument("--no-extra", action="store_true", help="skip the c1 and c3 plain legs")
+    ap.add_argument("--valu-peak", action="store_true",
+                    help="only measure the VALU issue rate of each instruction kind and exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run one device-to-device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -267,35 +348,17 @@ def main():
 
     import factsieve_amd as fs
     fs.init(0)
-    pmin, pmax, nmin, nmax, desc = WORKLOADS[args.workload]
-    for _ in range(args.warmup):
-        fs.sieve(pmin, pmax, nmin, nmax, verify_threads=16)
-    results = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        results.append(fs.sieve(pmin, pmax, nmin, nmax, verify_threads=16))
-    t1 = time.perf_counter()
-    ref = results[0]
-    for r in results[1:]:
-        assert (r.checksum, r.factorcount, r.primecount) == (ref.checksum, ref.factorcount, ref.primecount)
-    elapsed = t1 - t0
-    steps_total = sum(r.steps for r in results)
-    value = steps_total / elapsed
-
-    it_ms = sum(r.kernel_ms_iterate for r in results)
-    launches = sum(r.iterate_launches for r in results)
-    avg_launch_ms = it_ms / max(launches, 1)
-    ops = OPS_PER_STEP[ref.path]
-    if ref.path == 2 and hybrid_pair(pmin, pmax, nmax):
-        ops = OPS_PER_STEP_HYBRID_PAIR
-    elif ref.path == 2 and hybrid_hi(pmin, pmax, nmax):
-        ops = OPS_PER_STEP_HYBRID_HI
-    if ref.path == 1:
-        ops = fp64_ops(pmin, pmax, nmin, nmax)
-    achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
-    traffic = None
-    if ref.path in TRAFFIC_BYTES_PER_CAND:
-        traffic = TRAFFIC_BYTES_PER_CAND[ref.path] * ref.primecount / max(ref.iterate_launches, 1)
+    if args.valu_peak:
+        print(json.dumps({"valu_peak": valu_peaks(fs), "peak_2p4ghz_lane_ops_per_s": VALU_PEAK_TOPS * 1e12}),
+              flush=True)
+        return
+    if args.workload == "c4":
+        bat, _ = bench_batched(fs)
+        print(json.dumps({"metric": "p*n residue steps/s (batched, 64 mixed jobs)",
+                          "value": bat["batched_steps_per_s"], "unit": "p*n steps/s", "batched": bat}), flush=True)
+        return
+    leg, elapsed, ref = plain_leg(fs, args.workload, args.steps, args.warmup)
+    value = leg["steps_per_s"]
     line = {
         "metric": "p*n residue steps/s (plain single job)",
         "value": value,
@@ -309,29 +372,27 @@ def main():
         "vs_baseline": None,
         "dtype": {1: "f64 (exact integer residues)", 2: "u64+f64", 3: "u64"}[ref.path],
         "data": "synthetic (the workload is a numeric range; no data set)",
-        "config": {"workload": desc, "pmin": pmin, "pmax": pmax, "nmin": nmin, "nmax": nmax,
-                   "candidates": ref.primecount, "pn_steps_per_job": ref.steps, "factors": ref.factorcount,
-                   "checksum": f"{ref.checksum:016X}", "path": {1: "fp64", 2: "hybrid", 3: "montgomery"}[ref.path],
-                   "segments": ref.segments, "parallelism": "single GPU"},
-        "roofline": {"bound": "valu", "kernel": "k_iterate_" + {1: "fp64", 2: "hybrid", 3: "mont"}[ref.path],
-                     "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s (VALU lane-ops)",
-                     "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic, "traffic_unit": "bytes per launch",
-                     "ops_per_step": ops, "avg_launch_ms": avg_launch_ms,
-                     "kernel_steps_per_s": steps_total / (it_ms * 1e-3)},
-        "int32_mul_equiv": {"frac": steps_total / (it_ms * 1e-3) * INT32_MUL_PER_STEP / INT32_MUL_PEAK,
-                            "muls_per_step": INT32_MUL_PER_STEP, "peak_ops_per_s": INT32_MUL_PEAK},
-        "kernel_ms_per_job": {"iterate": it_ms / args.steps,
-                              "setup": sum(r.kernel_ms_setup for r in results) / args.steps,
-                              "primegen": sum(r.kernel_ms_primegen for r in results) / args.steps},
+        "config": {k: leg[k] for k in ("workload", "pmin", "pmax", "nmin", "nmax", "candidates",
+                                       "pn_steps_per_job", "factors", "checksum", "path", "segments",
+                                       "lanes_per_prime")},
+        "roofline": leg["roofline"],
+        "int32_mul_equiv": leg["int32_mul_equiv"],
+        "kernel_ms_per_job": leg["kernel_ms_per_job"],
     }
+    line["config"]["parallelism"] = "single GPU"
+    if not args.no_extra:
+        for w, steps, warm in (("c1", 30, 3), ("c3", 3, 1)):
+            if w != args.workload:
+                line["plain_" + w] = plain_leg(fs, w, steps, warm)[0]
     if not args.no_batch:
-        bat, ref = bench_batched(fs)
+        bat, ref_jobs = bench_batched(fs)
         line["batched"] = bat
         bench_serving.capacity_steps_per_s = bat["batched_steps_per_s"]
-        line["serving"] = bench_serving(fs, ref)
-        line["serving_without_graph"] = bench_serving(fs, ref, use_graph=0)
+        line["serving"] = bench_serving(fs, ref_jobs)
+        line["serving_without_graph"] = bench_serving(fs, ref_jobs, use_graph=0)
         line["graph_replay"] = bench_graph_replay(fs)
     if not args.no_cpu_baseline:
+        pmin, _, nmin, nmax, _ = WORKLOADS[args.workload]
         line["cpu_baseline"] = cpu_baseline(pmin, nmin, nmax)
         line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)

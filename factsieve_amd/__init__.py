@@ -107,6 +107,8 @@ class FactSieveError(RuntimeError):
 # (name, restype, argtypes) of every C-ABI entry point used from Python
 _SIGS = [
     ("fs_init", ctypes.c_int, [ctypes.c_int]),
+    ("fs_valu_peak", ctypes.c_int, [ctypes.c_int, u32, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double)]),
     ("fs_trickle_up_text", ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_char_p,
                                           ctypes.c_size_t]),
     ("fs_read_init_data", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
@@ -278,6 +280,17 @@ def write_checkpoint(directory, st, which):
     """write_state into stateA.ckp (which=0) or stateB.ckp (which=1); sets st.state_sum."""
     check(lib().fs_write_checkpoint(str(directory).encode(), ctypes.byref(st), which))
     return st
+
+
+VALU_PEAK_KINDS = {0: "v_mad_u64_u32", 1: "v_mul_lo_u32", 2: "v_mul_hi_u32", 3: "v_fma_f64"}
+
+
+def valu_peak(kind, iters=1 << 16):
+    """(lane-operations per second, launch ms) of one VALU instruction kind
+    (VALU_PEAK_KINDS) at 8 waves per SIMD."""
+    rate, ms = ctypes.c_double(), ctypes.c_double()
+    check(lib().fs_valu_peak(kind, iters, ctypes.byref(rate), ctypes.byref(ms)))
+    return rate.value, ms.value
 
 
 def trickle_up_text(progress, cputime, runtime):

@@ -29,7 +29,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["fs_device.hip", "fs_batch.hip", "fs_host.cpp", "fs_hostemu.cpp", "fs_batch_host.cpp", "fs_server.cpp"]
+SOURCES = ["fs_device.hip", "fs_batch.hip", "fs_peak.hip", "fs_host.cpp", "fs_hostemu.cpp", "fs_batch_host.cpp",
+           "fs_server.cpp"]
 
 
 def _run(cmd, cwd=None):

@@ -214,7 +214,9 @@ int fs_batch_run_device(fs_batch_bufs& E, const std::vector<fs_bjob>& bj, const 
   HCHK(hipMemcpyAsync(d.params, params.data(), params.size() * 4, hipMemcpyHostToDevice, E.st), "batch upload");
   HCHK(hipStreamSynchronize(E.st), "batch upload sync");
   HCHK(hipEventRecord(E.ev0, E.st), "batch event");
-  const uint32_t pg_grid = std::min<uint32_t>(pg_blocks, fs_device_cus() * 8);
+  // one workgroup per virtual block: the dispatcher balances jobs of unequal
+  // cost better than a grid-stride over a fixed grid
+  const uint32_t pg_grid = pg_blocks;
   if ((rc = E.enqueue(d, (uint32_t)norder.size(), pg_grid, tile, blocks, unroll, graph))) return rc;
   HCHK(hipEventRecord(E.ev1, E.st), "batch event");
   HCHK(hipStreamSynchronize(E.st), "batch sync");

@@ -402,6 +402,14 @@ int fs_batch(const fs_job* jobs, uint32_t njobs, const fs_batch_options* opt, fs
  * per prime and n-chunk length. */
 int fs_batch_autotune(const fs_job* jobs, uint32_t njobs, fs_batch_options* opt);
 
+/* ------------------------------------------------ VALU issue-rate probe */
+/* No reference counterpart: the measured peak the benchmark prices the
+ * kernels against.  Lane-operations per second (and the launch's ms) of one
+ * instruction kind -- 0 v_mad_u64_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32,
+ * 3 v_fma_f64 -- in `iters` iterations of 8 independent chains per lane, 8 waves
+ * on every SIMD, timed over one launch after a warm-up launch. */
+int fs_valu_peak(int kind, uint32_t iters, double* lane_ops_per_s, double* ms);
+
 /* ------------------------------------------------------ serving loop */
 /* Continuous batching of workunits (no reference counterpart: BOINC hands one
  * workunit to one process).  Admission by device memory; residues stay in HBM
