@@ -19,8 +19,10 @@ import os
 from collections import namedtuple
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfactsieve.so")
-EXE_PATH = os.path.join(_HERE, "factsieve")
+# FACTSIEVE_LIB names another build of the same C ABI (the host-sanitizer
+# build of factsieve_amd.build.build_sanitized, for instance).
+LIB_PATH = os.environ.get("FACTSIEVE_LIB") or os.path.join(_HERE, "libfactsieve.so")
+EXE_PATH = os.path.join(os.path.dirname(LIB_PATH), "factsieve")  # built beside the library
 
 FS_OK = 0
 FS_E_ARG, FS_E_HIP, FS_E_NOMEM, FS_E_OVERFLOW, FS_E_VALIDATE = -1, -2, -3, -4, -5

@@ -10,9 +10,11 @@ Outputs, all in-tree so they travel with the repository to a GPU machine:
     oracle/_ref/libfsref.so         reference verifyprime.c + putil.c (only where
                                     the reference checkout is present)
 """
+import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -61,6 +63,49 @@ def build_library(verbose=False):
     _run([HIPCC, "-O2", "-std=c++17", "-o", EXE, os.path.join(CSRC, "fs_main.cpp"), "-L", HERE,
           "-lfactsieve", "-Wl,-rpath,$ORIGIN"])
     return LIB
+
+
+# AddressSanitizer and UndefinedBehaviorSanitizer on the host code only: each
+# -fsanitize= goes after -Xarch_host, so the gfx950 code is built as usual.
+SAN_DIR = os.path.join(HERE, "_build", "sanitized")
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def sanitizer_runtime():
+    """Path of the clang AddressSanitizer runtime (it must be the first library
+    of a process that loads the sanitized build, through LD_PRELOAD)."""
+    llvm = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "llvm")
+    found = sorted(glob.glob(os.path.join(llvm, "lib", "clang", "*", "lib", "*", "libclang_rt.asan-x86_64.so")))
+    if not found:
+        raise RuntimeError("no libclang_rt.asan-x86_64.so under " + llvm)
+    return found[-1]
+
+
+def build_sanitized(jobs=8):
+    """The library and the command line with the host code under ASan + UBSan,
+    into factsieve_amd/_build/sanitized/ (load it with FACTSIEVE_LIB and the
+    runtime of sanitizer_runtime() in LD_PRELOAD).  Returns the library path."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    flags = [f for f in HIP_FLAGS if f != "-O3"] + ["-O1", "-g"] + SAN_HOST
+
+    def one(src):
+        obj = os.path.join(SAN_DIR, os.path.splitext(src)[0] + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        _run([HIPCC] + flags + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(one, SOURCES))
+    rt_dir = os.path.dirname(sanitizer_runtime())
+    lib = os.path.join(SAN_DIR, "libfactsieve.so")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-shared-libsan", "-o", lib + ".tmp"] + objs
+         + ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-Wl,-rpath," + rt_dir,
+            "-lpthread"])
+    os.replace(lib + ".tmp", lib)
+    _run([HIPCC, "-O1", "-g", "-std=c++17", "-o", os.path.join(SAN_DIR, "factsieve"), os.path.join(CSRC, "fs_main.cpp"),
+          "-L", SAN_DIR, "-lfactsieve", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + rt_dir])
+    return lib
 
 
 def build_oracle():

@@ -197,6 +197,8 @@ extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* r
 extern "C" int fs_hostemu_iterate(int path, uint64_t p, uint32_t n0, uint32_t n1, uint64_t* res,
                                   fs_factor* fac, uint32_t cap, uint32_t* nf) {
   if ((p & 1) == 0 || p < 3 || n0 > n1 || *res >= p) return FS_E_ARG;
+  // fs_sh_mulmod's multiplier bound m < 2p (n1 < 2^32 <= 2p for p >= 2^31).
+  if (path == FS_PATH_HYBRID && *res != 0 && p < ((uint64_t)1 << 31) && (uint64_t)n1 >= 2 * p) return FS_E_ARG;
   const uint64_t q = fs_invert(p), one = fs_r64_mod(p), r2 = fs_r2_from_one(one, p, q);
   uint32_t cnt = 0;
   auto rec = [&](int32_t nc) {

@@ -215,6 +215,9 @@ int fs_parse_uint64(uint64_t* result, const char* str, uint64_t lo, uint64_t hi)
  * the CPU with the same inline arithmetic, for testing the three formulations
  * without a GPU.  No reference counterpart; the sieve driver never calls them. */
 int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* res);
+/* Residue *res at startN advanced to endN.  The HYBRID path needs its multipliers
+ * below 2p, as in the sieve (a residue is 0 from n = p on): endN < 2p unless
+ * *res == 0 (FS_E_ARG otherwise). */
 int fs_hostemu_iterate(int path, uint64_t p, uint32_t startN, uint32_t endN, uint64_t* res,
                        fs_factor* factors, uint32_t cap, uint32_t* nfactors);
 int64_t fs_hostemu_candidates(uint64_t low, uint64_t high, uint64_t* out, uint64_t cap);

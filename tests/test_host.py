@@ -242,6 +242,12 @@ def test_host_arithmetic_matches_oracle(fs, lo, hi, paths):
                 if nmin < 10 ** 6:
                     start = fs.hostemu_setup(path, p, nmin)
                     assert start == want, (path, p, nmin)
+                if path == fs.PATH_HYBRID and start and nmin + span >= 2 * p:
+                    # outside the hybrid step's multiplier bound m < 2p, which
+                    # the sieve keeps (its residues are 0 from n = p on)
+                    with pytest.raises(fs.FactSieveError):
+                        fs.hostemu_iterate(path, p, nmin, nmin + span, start)
+                    continue
                 r, facs, nf = fs.hostemu_iterate(path, p, nmin, nmin + span, start)
                 M = ol.mont(p)
                 s6 = ol.u64(O.ora_mmul(start, M.r2, p, M.q))
@@ -556,7 +562,8 @@ def test_hybrid_quotient_filter_sound(fs):
     """The hybrid iterate tests s == +-1 only where the quotient estimate
     passes |y| <= 1.5/p; the host execution returns an error if a residue of
     +-1 ever fails that filter.  Start residues +-1 (and +-1 times a few
-    multipliers) over odd moduli from 3 to the top of the hybrid range."""
+    multipliers) over odd moduli from 3 to the top of the hybrid range, with
+    every multiplier below 2p as in the sieve (the residue is 0 from n = p)."""
     rng = random.Random(5)
     ps = [3, 5, 7, 127, 65537, 2 ** 31 - 1, 2 ** 32 + 15, 2 ** 52 - 1, 2 ** 52 + 1, 2 ** 63 - 25, 2 ** 63 + 29,
           2 ** 64 - 2 ** 46 - 1, 2 ** 64 - 2 ** 46 - 3]
@@ -565,7 +572,7 @@ def test_hybrid_quotient_filter_sound(fs):
         if p > 2 ** 64 - 2 ** 46:
             continue
         for start in (1, p - 1):
-            n0 = rng.randrange(1, 2 ** 31 - 10)
+            n0 = rng.randrange(1, min(2 ** 31 - 10, 2 * p - 4))
             _, facs, nf = fs.hostemu_iterate(fs.PATH_HYBRID, p, n0, n0 + 3, start)
             assert nf >= 1 and facs[0] == (p, (-1 if start == 1 else 1) * n0), p
 
