@@ -369,22 +369,35 @@ extern "C" int fs_is_prime(uint64_t p) {
 
 // O(n) recomputation of n! mod p; true iff n! = +1 (c = -1) or -1 (c = +1)
 // (verify, verifyprime.c:137-164).
-// lo * (lo+1) * ... * hi mod p for odd p >= 3 and 2 <= lo <= hi < p: plain
-// 64-bit remainders below 2^32, Montgomery products with a Montgomery counter
-// above (the verify loop of verifyprime.c:137-164).
+// a*b*2^-64 mod p for a, b < p, odd p (q = p^-1 mod 2^64), with the host's
+// 64x64->128 multiply: m*p matches a*b in the low word, so the high words'
+// difference is the exact quotient of a*b - m*p by 2^64, in (-p, p).
+static inline uint64_t mont_host(uint64_t a, uint64_t b, uint64_t p, uint64_t q) {
+  const u128 t = (u128)a * b;
+  const uint64_t m = (uint64_t)t * q;
+  const uint64_t mp = (uint64_t)(((u128)m * p) >> 64), hi = (uint64_t)(t >> 64);
+  return hi >= mp ? hi - mp : hi - mp + p;
+}
+
+// lo * (lo+1) * ... * hi mod p for odd p >= 3 and 2 <= lo <= hi < p (the
+// verify loop of verifyprime.c:137-164): four interleaved chains of
+// Montgomery products, chain j taking the multipliers lo + j + 4k, each kept in
+// Montgomery form and advanced by 4 with a modular add.
 static uint64_t range_product_mod(uint64_t p, uint64_t lo, uint64_t hi) {
-  if (p < 0xFFFFFFFFull) {
-    uint64_t r = 1;
-    for (uint64_t i = lo; i <= hi; ++i) r = r * i % p;
-    return r;
-  }
   const uint64_t q = fs_invert(p), one = (0 - p) % p, r2 = fs_r2_from_one(one, p, q);
-  uint64_t rm = one, im = fs_mont_mul(lo, r2, p, q);  // Montgomery forms of 1 and lo
-  for (uint64_t i = lo; i <= hi; ++i) {
-    rm = fs_mont_mul(rm, im, p, q);
-    im = fs_add_mod(im, one, p);
+  const uint64_t four = fs_add_mod(fs_add_mod(one, one, p), fs_add_mod(one, one, p), p);
+  uint64_t r[4] = {one, one, one, one}, im[4];
+  for (int j = 0; j < 4; ++j) im[j] = mont_host((lo + j) % p, r2, p, q);
+  const uint64_t count = hi - lo + 1;  // hi < p, so no wrap
+  for (uint64_t k = count / 4; k; --k) {
+    for (int j = 0; j < 4; ++j) {
+      r[j] = mont_host(r[j], im[j], p, q);
+      im[j] = fs_add_mod(im[j], four, p);
+    }
   }
-  return fs_mont_mul(rm, 1, p, q);
+  for (uint64_t j = 0; j < count % 4; ++j) r[0] = mont_host(r[0], im[j], p, q);  // the last 0..3 multipliers
+  const uint64_t prod = mont_host(mont_host(r[0], r[1], p, q), mont_host(r[2], r[3], p, q), p, q);
+  return mont_host(prod, 1, p, q);
 }
 
 // n! mod p (0 when p <= n); p odd, >= 3.
@@ -802,6 +815,12 @@ struct DevBufs {
   fs_event ev[6] = {};
   uint32_t* h_ctr = nullptr;  // pinned read-back of the counters and sums
   uint64_t* h_sum = nullptr;
+  // The first factor records, read back with the counters at the end of each
+  // segment (one wait covers the segment and its read-back); `staged` holds
+  // while nothing has been enqueued on s since that wait.
+  static constexpr uint32_t kStagedFactors = 1024;
+  fs_factor* h_fac = nullptr;
+  bool staged = false;
   // lanes 1..3 of a job split over several lanes per prime: streams beside s,
   // their end events, and their setup programs (cached by nmin and path)
   static constexpr int kLanes = 4;
@@ -834,7 +853,8 @@ struct DevBufs {
         (rc = fs_malloc(&verify, fs_verify_bytes())))
       return rc;
     if (hipHostMalloc((void**)&h_ctr, FS_CTR_WORDS * 4, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_sum, FS_SUM_WORDS * 8, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void**)&h_sum, FS_SUM_WORDS * 8, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&h_fac, kStagedFactors * sizeof(fs_factor), hipHostMallocDefault) != hipSuccess) {
       fs_set_error_msg("pinned host allocation failed");
       return FS_E_NOMEM;
     }
@@ -851,8 +871,23 @@ struct DevBufs {
     if (rc == FS_OK) fac_alloc = cap;
     return rc;
   }
+  // counters, sums and the first factor records to pinned memory on s, read
+  // after the next wait for s
+  int stage() {
+    staged = false;
+    const size_t nrec = std::min(kStagedFactors, fcap);
+    if (hipMemcpyAsync(h_ctr, ctr, FS_CTR_WORDS * 4, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess ||
+        hipMemcpyAsync(h_sum, sum, FS_SUM_WORDS * 8, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess ||
+        (nrec && hipMemcpyAsync(h_fac, fac, nrec * sizeof(fs_factor), hipMemcpyDeviceToHost, (hipStream_t)s) !=
+                     hipSuccess)) {
+      fs_set_error_msg("counter read-back failed");
+      return FS_E_HIP;
+    }
+    return FS_OK;
+  }
   // counters (and sums) to the host through pinned memory, waiting for them
   int read_ctr(uint32_t* out, uint64_t* sums = nullptr) {
+    staged = false;
     if ((sums && hipMemcpyAsync(h_sum, sum, FS_SUM_WORDS * 8, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess) ||
         hipMemcpyAsync(h_ctr, ctr, FS_CTR_WORDS * 4, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess) {
       fs_set_error_msg("counter read-back failed");
@@ -918,7 +953,13 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
   uint32_t ctr[FS_CTR_WORDS];
   uint64_t sum[FS_SUM_WORDS];
   int rc;
-  if ((rc = d.read_ctr(ctr, sum))) return rc;
+  const bool staged = d.staged;  // counters, sums and the first records already on the host
+  if (staged) {
+    memcpy(ctr, d.h_ctr, sizeof ctr);
+    memcpy(sum, d.h_sum, sizeof sum);
+  } else if ((rc = d.read_ctr(ctr, sum))) {
+    return rc;
+  }
   st.primecount += sum[0];
   st.checksum += sum[1];
   const bool verbose = !o.quiet;
@@ -943,7 +984,11 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
   }
   if (nf > 0) {
     std::vector<fs_factor> h(nf);
-    if ((rc = fs_memcpy_dtoh(h.data(), d.fac, (size_t)nf * sizeof(fs_factor), d.s, 1))) return rc;
+    const uint32_t have = staged ? std::min(nf, DevBufs::kStagedFactors) : 0;
+    memcpy(h.data(), d.h_fac, (size_t)have * sizeof(fs_factor));
+    if (have < nf && (rc = fs_memcpy_dtoh(h.data() + have, d.fac + have, (size_t)(nf - have) * sizeof(fs_factor),
+                                          d.s, 1)))
+      return rc;
     fs_sort_factors(h.data(), nf);
     res->raw_factors += nf;
     const char* where = fs_verify_uses_gpu(h.data(), nf, o.verify_device) ? "GPU" : "CPU";
@@ -984,6 +1029,7 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
     }
     out.kept.insert(out.kept.end(), h.begin(), h.begin() + kept);
   }
+  d.staged = false;
   return fs_clearresult(d.ctr, d.sum, d.s);
 }
 
@@ -1309,8 +1355,8 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   Drain dr;
   // factor buffer: 1e6 records, 3e7 below 2^32 (main.cpp:193, cl_sieve.cpp:593-595)
   if ((rc = d.need_factors((pmin < 0xFFFFFFFFull) ? 30000000u : 1000000u))) return rc;
-  if ((rc = fs_memset(d.ctr, 0, FS_CTR_WORDS * 4, d.s)) || (rc = fs_memset(d.sum, 0, FS_SUM_WORDS * 8, d.s)))
-    return rc;
+  d.staged = false;
+  if ((rc = fs_clearresult(d.ctr, d.sum, d.s))) return rc;  // the flags and sums; clearn zeroes the count
 
   // setup program and the reference power table (verifypow, and setup on MONT),
   // uploaded when they differ from what the buffers hold
@@ -1361,7 +1407,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
 
   const uint32_t target = o.seg_candidates ? o.seg_candidates : fs_auto_segment_candidates();
   const uint32_t nchunk_max = o.n_chunk ? o.n_chunk : (1u << 22);
-  bool verified = false;
+  bool verified = false, verify_pending = false;
   double last_ckpt = now_s(), last_prog = now_s();
   Boinc bx(o.boinc);
   const double irsize = 1.0 / (double)(pmax - pmin ? pmax - pmin : 1);
@@ -1484,16 +1530,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
           (rc = fs_verifyreduce(&views[0], d.ctr, d.verify, d.s)) ||
           (rc = fs_verifyresult(&views[0], d.ctr, d.verify, d.s)))
         return rc;
-      uint32_t h[FS_CTR_WORDS];
-      if ((rc = d.read_ctr(h))) return rc;
-      if (h[FS_CTR_POWERTABLE]) {
-        fs_set_error_msg("error: power table verification failed");
-        return FS_E_POWERTABLE;
-      }
-      verified = true;
-      fprintf(stderr, "Verified power table (%llu bytes) starting sieve...\n",
-              (unsigned long long)d.pt_count * 16);
-      if (verbose) printf("Verified power table (%llu bytes) starting sieve...\n", (unsigned long long)d.pt_count * 16);
+      verify_pending = true;  // its flag is read with the segment's counters, before any result is taken
     }
 
     // iterate over each lane's n in chunks, then fold the checksum of the last lane
@@ -1517,6 +1554,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     }
     if ((rc = fs_event_record(d.ev[5], d.s))) return rc;
     if ((rc = fs_check(&views[L - 1], d.ctr, d.sum, nmax, d.s))) return rc;
+    if ((rc = d.stage())) return rc;
     for (uint32_t k = 1; k < L; ++k) r.setup_mulmods += (uint64_t)count * lterms[k];
     if (L > 1) r.lanes = L;
     r.steps += (uint64_t)count * (nmax - nmin);
@@ -1530,6 +1568,17 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     rc = fs_stream_sync(d.s);
     bx.end();
     if (rc) return rc;
+    d.staged = true;
+    if (verify_pending) {  // the power-table cross-check of the first segment (cl_sieve.cpp:903-907)
+      if (d.h_ctr[FS_CTR_POWERTABLE]) {
+        fs_set_error_msg("error: power table verification failed");
+        return FS_E_POWERTABLE;
+      }
+      verify_pending = false;
+      verified = true;
+      fprintf(stderr, "Verified power table (%llu bytes) starting sieve...\n", (unsigned long long)d.pt_count * 16);
+      if (verbose) printf("Verified power table (%llu bytes) starting sieve...\n", (unsigned long long)d.pt_count * 16);
+    }
     account(d, &r);
 
     const double tn = now_s();

@@ -155,8 +155,8 @@ def test_is_prime_and_verify_match_oracle(fs):
     for p, nc in json.load(open(os.path.join(GOLDEN, "c1.json")))["factors"]:
         n, c = abs(nc), (-1 if nc < 0 else 1)
         assert fs.verify_factor(p, n, c) and not fs.verify_factor(p, n, -c) and not fs.verify_factor(p, n + 1, c)
-    for p in [2 ** 62 + 135, 2 ** 64 - 59]:
-        for n in [34, 35, 200]:
+    for p in [2 ** 62 + 135, 2 ** 64 - 59, 1000003, 4294967311]:
+        for n in [5, 6, 33, 34, 35, 36, 200]:  # 2..n holds 4k, 4k+1, 4k+2 and 4k+3 multipliers
             for c in (-1, 1):
                 assert fs.verify_factor(p, n, c) == bool(O.ora_verify(p, n, c))
 
