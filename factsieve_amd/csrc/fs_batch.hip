@@ -86,7 +86,7 @@ __global__ __launch_bounds__(FS_BLOCK) void k_bprimegen(const fs_bjob* __restric
     const fs_bjob J = jobs[j];
     if (J.flags & FS_BJ_NEW) {  // uniform per workgroup
       fs_primegen_block(
-          J.pmin | 1, J.pmax, b - J.pg_first, s_tab, s_mask, s_excl, s_wsum,
+          J.pmin | 1, J.pmax, b - J.pg_first, 0, 1, s_tab, s_mask, s_excl, s_wsum,
           [&](uint32_t n) { return atomicAdd(&jcnt[j], n); },
           [&](uint32_t idx, uint64_t N, uint64_t q, uint64_t r2) {
             if (idx < J.cand_cap) {

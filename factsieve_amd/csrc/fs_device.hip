@@ -76,15 +76,15 @@ __global__ void k_set_flag(uint32_t* ctr, uint32_t idx) {
 
 // Segmented candidate generation over [low, high) (getsegprimes.cl:170-239);
 // the block routine is fs_primegen_block.
-__global__ __launch_bounds__(FS_BLOCK) void k_getsegprimes(uint64_t base, uint64_t high, fs_cands c,
-                                                           uint32_t* ctr) {
+__global__ __launch_bounds__(FS_BLOCK) void k_getsegprimes(uint64_t base, uint64_t high, uint32_t split,
+                                                           fs_cands c, uint32_t* ctr) {
   __shared__ uint64_t s_tab[FS_NMASK];
   __shared__ uint64_t s_mask[FS_BLOCK];
   __shared__ uint32_t s_excl[FS_BLOCK];
   __shared__ uint32_t s_wsum[FS_BLOCK / 64 + 1];
   fs_sieve_table_init(s_tab);
   fs_primegen_block(
-      base, high, blockIdx.x, s_tab, s_mask, s_excl, s_wsum,
+      base, high, blockIdx.x / split, blockIdx.x % split, split, s_tab, s_mask, s_excl, s_wsum,
       [&](uint32_t n) { return atomicAdd(&ctr[FS_CTR_COUNT], n); },
       [&](uint32_t idx, uint64_t N, uint64_t q, uint64_t r2) {
         if (idx < c.cap) {
@@ -250,6 +250,9 @@ __global__ __launch_bounds__(FS_BLOCK) void k_verifyslow(fs_cands c, const uint3
     acc = fs_mont_mul(acc, fs_mont_mul(n, r2, p, q), p, q);
   const uint64_t b = fs_block_prod(acc, p, q, sh);
   if (threadIdx.x == 0) ver[blockIdx.x].slow = b;
+  // slots of blocks not launched hold 1 (Montgomery form) for verifyreduce
+  const uint64_t one = fs_mont_mul(1, r2, p, q);
+  for (uint32_t k = gridDim.x + blockIdx.x * FS_BLOCK + threadIdx.x; k < FS_VERIFY_BLOCKS; k += gs) ver[k].slow = one;
 }
 
 __global__ __launch_bounds__(FS_BLOCK) void k_verifypow(fs_cands c, const uint32_t* ctr,
@@ -271,6 +274,8 @@ __global__ __launch_bounds__(FS_BLOCK) void k_verifypow(fs_cands c, const uint32
   }
   const uint64_t b = fs_block_prod(acc, p, q, sh);
   if (threadIdx.x == 0) ver[blockIdx.x].pow = b;
+  const uint64_t one = fs_mont_mul(1, r2, p, q);
+  for (uint32_t k = gridDim.x + blockIdx.x * FS_BLOCK + threadIdx.x; k < FS_VERIFY_BLOCKS; k += gs) ver[k].pow = one;
 }
 
 __global__ __launch_bounds__(FS_BLOCK) void k_verifyreduce(fs_cands c, const uint32_t* ctr,
@@ -666,7 +671,10 @@ extern "C" int fs_getsegprimes(uint64_t low, uint64_t high, const fs_cands* c, u
     g_last_error = "getsegprimes: range too large for one launch";
     return FS_E_ARG;
   }
-  k_getsegprimes<<<(uint32_t)blocks, FS_BLOCK, 0, S(s)>>>(base, high, *c, d_ctr);
+  // a range of fewer than 2,048 tiles shares each tile's tests among up to 16
+  // blocks, so the chip holds at least ~2,048 workgroups
+  const uint32_t split = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16, 2048 / blocks));
+  k_getsegprimes<<<(uint32_t)(blocks * split), FS_BLOCK, 0, S(s)>>>(base, high, split, *c, d_ctr);
   LAUNCHCHK("getsegprimes");
   if (g_inject == FS_CTR_PRIMEOVF) {
     k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_PRIMEOVF);
@@ -880,7 +888,9 @@ extern "C" int fs_verifyslow(const fs_cands* c, const uint32_t* d_ctr, void* d_v
                              fs_stream s) {
   int rc = check_cands(c, "verifyslow");
   if (rc) return rc;
-  k_verifyslow<<<FS_VERIFY_BLOCKS, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (VerifyPair*)d_verify, nmin);
+  // about 16 multipliers per thread, at most FS_VERIFY_BLOCKS blocks
+  const uint32_t g = (uint32_t)std::min<uint64_t>(FS_VERIFY_BLOCKS, std::max<uint64_t>(1, ((uint64_t)nmin + 4095) / 4096));
+  k_verifyslow<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (VerifyPair*)d_verify, nmin);
   LAUNCHCHK("verifyslow");
   return FS_OK;
 }
@@ -889,8 +899,9 @@ extern "C" int fs_verifypow(const fs_cands* c, const uint32_t* d_ctr, const uint
                             const uint32_t* d_exps, uint32_t count, void* d_verify, fs_stream s) {
   int rc = check_cands(c, "verifypow");
   if (rc) return rc;
-  k_verifypow<<<FS_VERIFY_BLOCKS, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_bases, d_exps, count,
-                                                       (VerifyPair*)d_verify);
+  // one power per thread, at most FS_VERIFY_BLOCKS blocks
+  const uint32_t g = std::min<uint32_t>(FS_VERIFY_BLOCKS, std::max<uint32_t>(1, (count + FS_BLOCK - 1) / FS_BLOCK));
+  k_verifypow<<<g, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_bases, d_exps, count, (VerifyPair*)d_verify);
   LAUNCHCHK("verifypow");
   return FS_OK;
 }

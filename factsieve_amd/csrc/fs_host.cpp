@@ -795,6 +795,9 @@ double now_s() {
 struct DevBufs {
   int device = -1;
   fs_cands c{};
+  static constexpr size_t kSumOff = 64, kHdr = 128;  // byte offsets of the sums and the records
+  static_assert(FS_CTR_WORDS * 4 <= kSumOff && kSumOff + FS_SUM_WORDS * 8 <= kHdr, "header layout");
+  uint8_t* blk = nullptr;   // counters | sums | factor records
   uint32_t* ctr = nullptr;
   uint64_t* sum = nullptr;
   fs_factor* fac = nullptr;
@@ -813,7 +816,8 @@ struct DevBufs {
   void* verify = nullptr;
   fs_stream s = nullptr;
   fs_event ev[6] = {};
-  uint32_t* h_ctr = nullptr;  // pinned read-back of the counters and sums
+  uint8_t* h_blk = nullptr;   // pinned read-back, laid out as blk
+  uint32_t* h_ctr = nullptr;
   uint64_t* h_sum = nullptr;
   // The first factor records, read back with the counters at the end of each
   // segment (one wait covers the segment and its read-back); `staged` holds
@@ -849,37 +853,44 @@ struct DevBufs {
     if ((rc = fs_stream_create(&s))) return rc;
     for (auto& e : ev)
       if ((rc = fs_event_create(&e))) return rc;
-    if ((rc = fs_malloc((void**)&ctr, FS_CTR_WORDS * 4)) || (rc = fs_malloc((void**)&sum, FS_SUM_WORDS * 8)) ||
-        (rc = fs_malloc(&verify, fs_verify_bytes())))
-      return rc;
-    if (hipHostMalloc((void**)&h_ctr, FS_CTR_WORDS * 4, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_sum, FS_SUM_WORDS * 8, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void**)&h_fac, kStagedFactors * sizeof(fs_factor), hipHostMallocDefault) != hipSuccess) {
+    if ((rc = fs_malloc(&verify, fs_verify_bytes())) || (rc = need_factors(kStagedFactors))) return rc;
+    if (hipHostMalloc((void**)&h_blk, kHdr + kStagedFactors * sizeof(fs_factor), hipHostMallocDefault) !=
+        hipSuccess) {
       fs_set_error_msg("pinned host allocation failed");
       return FS_E_NOMEM;
     }
+    h_ctr = (uint32_t*)h_blk;
+    h_sum = (uint64_t*)(h_blk + kSumOff);
+    h_fac = (fs_factor*)(h_blk + kHdr);
     return FS_OK;
   }
-  // factor records for a capacity of `cap`
+  // factor records for a capacity of `cap`, behind the counters and sums in
+  // one allocation (ctr at 0, sum at kSumOff, records at kHdr), so that one
+  // copy reads back all three
   int need_factors(uint32_t cap) {
     fcap = cap;
     if (fac_alloc >= cap) return FS_OK;
-    fs_free(fac);
+    fs_free(blk);
+    blk = nullptr;
+    ctr = nullptr;
+    sum = nullptr;
     fac = nullptr;
     fac_alloc = 0;
-    int rc = fs_malloc((void**)&fac, (size_t)cap * sizeof(fs_factor));
-    if (rc == FS_OK) fac_alloc = cap;
-    return rc;
+    int rc = fs_malloc((void**)&blk, kHdr + (size_t)cap * sizeof(fs_factor));
+    if (rc) return rc;
+    ctr = (uint32_t*)blk;
+    sum = (uint64_t*)(blk + kSumOff);
+    fac = (fs_factor*)(blk + kHdr);
+    fac_alloc = cap;
+    return FS_OK;
   }
-  // counters, sums and the first factor records to pinned memory on s, read
-  // after the next wait for s
+  // counters, sums and the first factor records to pinned memory on s (one
+  // copy), read after the next wait for s
   int stage() {
     staged = false;
     const size_t nrec = std::min(kStagedFactors, fcap);
-    if (hipMemcpyAsync(h_ctr, ctr, FS_CTR_WORDS * 4, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess ||
-        hipMemcpyAsync(h_sum, sum, FS_SUM_WORDS * 8, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess ||
-        (nrec && hipMemcpyAsync(h_fac, fac, nrec * sizeof(fs_factor), hipMemcpyDeviceToHost, (hipStream_t)s) !=
-                     hipSuccess)) {
+    if (hipMemcpyAsync(h_blk, blk, kHdr + nrec * sizeof(fs_factor), hipMemcpyDeviceToHost, (hipStream_t)s) !=
+        hipSuccess) {
       fs_set_error_msg("counter read-back failed");
       return FS_E_HIP;
     }
@@ -888,8 +899,7 @@ struct DevBufs {
   // counters (and sums) to the host through pinned memory, waiting for them
   int read_ctr(uint32_t* out, uint64_t* sums = nullptr) {
     staged = false;
-    if ((sums && hipMemcpyAsync(h_sum, sum, FS_SUM_WORDS * 8, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess) ||
-        hipMemcpyAsync(h_ctr, ctr, FS_CTR_WORDS * 4, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess) {
+    if (hipMemcpyAsync(h_blk, blk, kHdr, hipMemcpyDeviceToHost, (hipStream_t)s) != hipSuccess) {
       fs_set_error_msg("counter read-back failed");
       return FS_E_HIP;
     }

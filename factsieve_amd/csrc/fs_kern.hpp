@@ -67,7 +67,9 @@ __device__ __forceinline__ void fs_sieve_table_init(uint64_t* s_tab) {
 // per round.  The block then reserves room for all its passers with a single
 // reserve(total) (one atomic per block, not per candidate), and each lane
 // stores its passers as store(index, N, q = N^-1 mod 2^64, r2 = 2^128 mod N).
-// All threads must call it.
+// With split > 1, `split` blocks share one range tile: each builds the same
+// masks and tests only the rounds r with r % split == sub, so a short range
+// still spreads its tests over many waves.  All threads must call it.
 __device__ __forceinline__ uint64_t fs_primegen_survivor(uint64_t base, uint64_t tw, uint32_t wbase, uint32_t k,
                                                          const uint64_t* s_mask, const uint32_t* s_excl) {
   uint32_t lo = 0, hi = 63;  // owner lane: last L with excl[L] <= k
@@ -83,9 +85,9 @@ __device__ __forceinline__ uint64_t fs_primegen_survivor(uint64_t base, uint64_t
 }
 
 template <class Reserve, class Store>
-__device__ __forceinline__ void fs_primegen_block(uint64_t base, uint64_t high, uint64_t blk, const uint64_t* s_tab,
-                                                  uint64_t* s_mask, uint32_t* s_excl, uint32_t* s_wsum,
-                                                  Reserve reserve, Store store) {
+__device__ __forceinline__ void fs_primegen_block(uint64_t base, uint64_t high, uint64_t blk, uint32_t sub,
+                                                  uint32_t split, const uint64_t* s_tab, uint64_t* s_mask,
+                                                  uint32_t* s_excl, uint32_t* s_wsum, Reserve reserve, Store store) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = threadIdx.x & ~63u;
   const uint64_t t = blk * FS_BLOCK + threadIdx.x;
@@ -115,10 +117,11 @@ __device__ __forceinline__ void fs_primegen_block(uint64_t base, uint64_t high, 
   s_excl[threadIdx.x] = incl - cnt;
   const uint32_t total = __shfl(incl, 63, 64);
   __syncthreads();
-  // 1. test: round r handles survivor k = lane + 64 r of this wave (r < 64)
+  // 1. test: round r handles survivor k = lane + 64 r of this wave (r < 64);
+  // this block takes the rounds sub, sub + split, ...
   const uint64_t tw = blk * FS_BLOCK + wbase;
   uint64_t passed = 0;
-  for (uint32_t k = lane, r = 0; k < total; k += 64, ++r) {
+  for (uint32_t k = lane + 64 * sub, r = sub; k < total; k += 64 * split, r += split) {
     const uint64_t N = fs_primegen_survivor(base, tw, wbase, k, s_mask, s_excl);
     if (fs_candidate_probable_prime(N)) passed |= (uint64_t)1 << r;
   }
