@@ -291,7 +291,9 @@ def plain_leg(fs, workload, steps, warmup):
         "checksum": f"{ref.checksum:016X}", "path": {1: "fp64", 2: "hybrid", 3: "montgomery"}[ref.path],
         "segments": ref.segments, "lanes_per_prime": ref.lanes,
         "steps_per_s": steps_total / elapsed, "ms_per_job": elapsed / steps * 1e3,
-        "roofline": {"bound": "valu", "kernel": "k_iterate_" + {1: "fp64", 2: "hybrid", 3: "mont"}[ref.path],
+        "roofline": {"bound": "valu",
+                     "kernel": ("k_iterate_lanes<%s>" % {1: "FP64", 2: "HYBRID"}[ref.path]) if ref.lanes > 1
+                     else "k_iterate_" + {1: "fp64", 2: "hybrid", 3: "mont"}[ref.path],
                      "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s (VALU lane-ops)",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic, "traffic_unit": "bytes per launch",
                      "ops_per_step": ops, "avg_launch_ms": it_ms / max(launches, 1),

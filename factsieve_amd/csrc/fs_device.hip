@@ -19,6 +19,7 @@
 #include <vector>
 #include "fs_arith.hpp"
 #include "fs_kern.hpp"
+#include "fs_internal.hpp"
 #include "../../include/factsieve.h"
 
 // ===================================================================== errors
@@ -181,16 +182,97 @@ __global__ void k_fill_ntab(double* t, double* pt, uint32_t half, uint32_t first
 }
 
 // Lane copies of a segment's candidates (p, q, r2): entry i of lane k is
-// entry i; grid-stride over (lanes - 1) * count entries.
-__global__ void k_replicate(fs_cands c, uint32_t count, uint32_t lanes) {
+// entry i, at k * stride + i; grid-stride over (lanes - 1) * count entries.
+__global__ void k_replicate(fs_cands c, uint32_t count, uint32_t stride, uint32_t lanes) {
   const uint64_t total = (uint64_t)(lanes - 1) * count;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t i = (uint32_t)(t % count);
-    const uint64_t dst = t + count;
+    const uint64_t dst = (t / count + 1) * stride + i;
     c.p[dst] = c.p[i];
     c.q[dst] = c.q[i];
     c.r2[dst] = c.r2[i];
   }
+}
+
+// ---- lanes per prime in one launch: lane k of a segment is the copy of its
+// `count` candidates at k * stride (stride a multiple of FS_BLOCK, so a block
+// lies in one lane and the lane's n-range, program and table pointers are
+// uniform over the block), covering n in [nb[k], nb[k + 1]).
+struct LaneArgs {
+  uint32_t lanes, count, stride, ncuts;
+  uint32_t nb[FS_LANES + 1];
+  uint32_t terms[FS_LANES + 1];    // setup program words of each lane (entry FS_LANES unused:
+  uint32_t poff[FS_LANES + 1];     // fs_pick reads FS_LANES + 1 entries) and their offsets
+  uint32_t cuts[4];                // loop boundaries: iterate launches of one lane are cut there
+  uint32_t tbase, thalf;           // first n of the multiplier table, half length of its pair table
+};
+
+// This thread's lane (a scalar: the same for the whole block) and global
+// index; false when it holds no candidate.
+__device__ __forceinline__ bool fs_lane_of(const LaneArgs& a, uint32_t* lane, uint32_t* i) {
+  const uint32_t b0 = blockIdx.x * FS_BLOCK;
+  *lane = __builtin_amdgcn_readfirstlane(b0 / a.stride);
+  *i = b0 + threadIdx.x;
+  return *lane < a.lanes && *i - *lane * a.stride < a.count;
+}
+// Entry `lane` of a per-lane argument array, selected without indexing (so a
+// scalar lane keeps the value in scalar registers, as the scalar program and
+// table loads need).
+template <class T>
+__device__ __forceinline__ T fs_pick(const T* v, uint32_t lane) {
+  T r = v[0];
+#pragma unroll
+  for (uint32_t k = 1; k <= FS_LANES; ++k)
+    if (lane == k) r = v[k];
+  return r;
+}
+
+// Setup chunk [start, end) of every lane's program (a lane whose program ended
+// in an earlier chunk is left alone); each lane's residue becomes nb[k]! mod p.
+// The programs (progs + poff[k]) and tables are __restrict__ kernel arguments
+// so that their loads stay scalar.
+template <int PATH>
+__global__ __launch_bounds__(FS_BLOCK) void k_setup_lanes(fs_cands c, LaneArgs a, const uint64_t* __restrict__ progs,
+                                                          uint32_t start, uint32_t end) {
+  uint32_t lane, i;
+  if (!fs_lane_of(a, &lane, &i)) return;
+  const uint32_t terms = fs_pick(a.terms, lane), e = end < terms ? end : terms;
+  if (start > 0 && start >= e) return;
+  const uint64_t* prog = progs + fs_pick(a.poff, lane);
+  if (PATH == FS_PATH_FP64)
+    c.res[i] = fs_cand_setup_fp64(c.p[i], c.q[i], c.r2[i], c.res[i], (const double*)prog, start, e);
+  else
+    c.res[i] = fs_cand_setup_hybrid(c.p[i], c.q[i], c.r2[i], c.res[i], prog, terms, start, e);
+  c.nnext[i] = fs_pick(a.nb, lane);
+}
+
+// Every lane's whole n-range, as one launch per loop interval would run it.
+template <int PATH>
+__global__ __launch_bounds__(FS_BLOCK) void k_iterate_lanes(fs_cands c, uint32_t* ctr, fs_factor* fac, uint32_t fcap,
+                                                            LaneArgs a, const double* __restrict__ ntab,
+                                                            const double* __restrict__ pairs) {
+  uint32_t lane, i;
+  if (!fs_lane_of(a, &lane, &i)) return;
+  const uint64_t p = c.p[i];
+  const uint32_t hi = fs_pick(a.nb, lane + 1);
+  uint32_t lo = fs_pick(a.nb, lane);
+  if (c.nnext[i] != lo) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
+  uint64_t s = c.res[i];
+  while (lo < hi) {
+    uint32_t b = hi;
+    for (uint32_t k = 0; k < a.ncuts; ++k)
+      if (a.cuts[k] > lo && a.cuts[k] < b) b = a.cuts[k];
+    const uint32_t j = lo + 1 - a.tbase;
+    const double* nt = ntab + j;
+    const double* pt = pairs + (size_t)(j & 1) * 2 * a.thalf + 2 * (size_t)(j >> 1);
+    if (PATH == FS_PATH_FP64)
+      s = fs_cand_iter_fp64<FS_UNROLL_DEFAULT, 16>(p, s, lo, b, nt, pt, true, FS_REC(p));
+    else
+      s = fs_cand_iter_hybrid<FS_UNROLL_DEFAULT, 16>(p, s, lo, b, nt, pt, true, FS_REC(p));
+    lo = b;
+  }
+  c.res[i] = s;
+  c.nnext[i] = hi;
 }
 
 // ---- check: checksum fold + count + validation (check.cl:50-109)
@@ -447,14 +529,14 @@ extern "C" int fs_inject_fault(int idx) {
   return FS_OK;
 }
 
-int fs_cands_replicate(const fs_cands* c, uint32_t count, uint32_t lanes, fs_stream s) {
+int fs_cands_replicate(const fs_cands* c, uint32_t count, uint32_t stride, uint32_t lanes, fs_stream s) {
   if (lanes < 2 || count == 0) return FS_OK;
-  if ((uint64_t)lanes * count > c->cap) {
-    g_last_error = "cands_replicate: capacity below lanes * count";
+  if (stride < count || (uint64_t)(lanes - 1) * stride + count > c->cap) {
+    g_last_error = "cands_replicate: capacity below the lanes' copies";
     return FS_E_ARG;
   }
   const uint64_t total = (uint64_t)(lanes - 1) * count;
-  k_replicate<<<(uint32_t)std::min<uint64_t>((total + 255) / 256, 4096), 256, 0, S(s)>>>(*c, count, lanes);
+  k_replicate<<<(uint32_t)std::min<uint64_t>((total + 255) / 256, 4096), 256, 0, S(s)>>>(*c, count, stride, lanes);
   HIPCHK(hipGetLastError(), "cands_replicate");
   return FS_OK;
 }
@@ -815,6 +897,101 @@ extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_facto
       return FS_E_ARG;
   }
   LAUNCHCHK("iterate");
+  if (g_inject == FS_CTR_FACTOROVF) {
+    k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_FACTOROVF);
+    g_inject = 0;
+  }
+  return FS_OK;
+}
+
+static int lane_args(const fs_cands* c, const fs_lane_set* ls, int path, LaneArgs* a, const char* who) {
+  int rc = check_cands(c, who);
+  if (rc) return rc;
+  if (path != FS_PATH_FP64 && path != FS_PATH_HYBRID) {
+    g_last_error = std::string(who) + ": lanes run on the FP64 and hybrid paths only";
+    return FS_E_ARG;
+  }
+  if (ls->lanes < 1 || ls->lanes > FS_LANES || ls->stride % FS_BLOCK || ls->stride < ls->count ||
+      (uint64_t)(ls->lanes - 1) * ls->stride + ls->count > c->cap) {
+    g_last_error = std::string(who) + ": bad lane layout";
+    return FS_E_ARG;
+  }
+  memset(a, 0, sizeof *a);
+  a->lanes = ls->lanes;
+  a->count = ls->count;
+  a->stride = ls->stride;
+  for (uint32_t k = 0; k <= ls->lanes; ++k) a->nb[k] = ls->nb[k];
+  for (uint32_t k = 0; k < ls->lanes; ++k) {
+    if (k && a->nb[k] < a->nb[k - 1]) {
+      g_last_error = std::string(who) + ": lane n-ranges out of order";
+      return FS_E_ARG;
+    }
+    a->poff[k] = ls->poff[k];
+    a->terms[k] = ls->terms[k];
+  }
+  if (a->nb[ls->lanes] < a->nb[ls->lanes - 1] || a->nb[ls->lanes] > 0x7fffffffu) {
+    g_last_error = std::string(who) + ": need n < 2^31";
+    return FS_E_ARG;
+  }
+  return FS_OK;
+}
+
+static uint32_t lane_grid(const fs_lane_set* ls) {
+  return (uint32_t)(((uint64_t)(ls->lanes - 1) * ls->stride + ls->count + FS_BLOCK - 1) / FS_BLOCK);
+}
+
+int fs_setup_lanes(const fs_cands* c, const fs_lane_set* ls, uint32_t start, uint32_t end, int path, fs_stream s) {
+  LaneArgs a;
+  int rc = lane_args(c, ls, path, &a, "setup_lanes");
+  if (rc) return rc;
+  if ((start | end) & 3) {
+    g_last_error = "setup_lanes: program chunks must start and end on 4-term blocks";
+    return FS_E_ARG;
+  }
+  for (uint32_t k = 0; k < ls->lanes; ++k)
+    if ((ls->terms[k] & 3) || (ls->poff[k] & 3) || !ls->progs) {
+      g_last_error = "setup_lanes: bad program";
+      return FS_E_ARG;
+    }
+  if (ls->count == 0) return FS_OK;
+  const uint64_t* pg = (const uint64_t*)ls->progs;
+  if (path == FS_PATH_FP64)
+    k_setup_lanes<FS_PATH_FP64><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, a, pg, start, end);
+  else
+    k_setup_lanes<FS_PATH_HYBRID><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, a, pg, start, end);
+  LAUNCHCHK("setup_lanes");
+  return FS_OK;
+}
+
+int fs_iterate_lanes(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t fcap, const fs_lane_set* ls,
+                     const uint32_t* cuts, uint32_t ncuts, int path, fs_stream s) {
+  LaneArgs a;
+  int rc = lane_args(c, ls, path, &a, "iterate_lanes");
+  if (rc) return rc;
+  if (ncuts > 4) {
+    g_last_error = "iterate_lanes: at most 4 loop boundaries";
+    return FS_E_ARG;
+  }
+  a.ncuts = ncuts;
+  for (uint32_t k = 0; k < ncuts; ++k) a.cuts[k] = cuts[k];
+  const uint32_t n0 = ls->nb[0], n1 = ls->nb[ls->lanes];
+  const double *nt = nullptr, *pt = nullptr, *tab = nullptr, *pairs = nullptr;
+  if ((rc = ntab_for(S(s), n0 + 1, n1 - n0, &nt, &pt))) return rc;
+  {
+    std::lock_guard<std::mutex> lk(g_ntab_mu);
+    const NTab& t = g_ntab[S(s)];
+    tab = t.d;
+    pairs = t.pairs();
+    a.tbase = t.base;
+    a.thalf = t.half();
+  }
+  if (ls->count) {
+    if (path == FS_PATH_FP64)
+      k_iterate_lanes<FS_PATH_FP64><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, a, tab, pairs);
+    else
+      k_iterate_lanes<FS_PATH_HYBRID><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, a, tab, pairs);
+    LAUNCHCHK("iterate_lanes");
+  }
   if (g_inject == FS_CTR_FACTOROVF) {
     k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_FACTOROVF);
     g_inject = 0;

@@ -825,26 +825,25 @@ struct DevBufs {
   static constexpr uint32_t kStagedFactors = 1024;
   fs_factor* h_fac = nullptr;
   bool staged = false;
-  // lanes 1..3 of a job split over several lanes per prime: streams beside s,
-  // their end events, and their setup programs (cached by nmin and path)
-  static constexpr int kLanes = 4;
-  hipStream_t xs[kLanes - 1] = {};
-  hipEvent_t xev[kLanes] = {};
-  void* xprog[kLanes - 1] = {};
-  size_t xprog_alloc[kLanes - 1] = {};
-  uint32_t xprog_nmin[kLanes - 1] = {};
-  int xprog_path[kLanes - 1] = {-1, -1, -1};
-  int init_lanes() {
-    for (int k = 0; k < kLanes - 1; ++k)
-      if (!xs[k] && hipStreamCreateWithFlags(&xs[k], hipStreamNonBlocking) != hipSuccess) {
-        fs_set_error_msg("lane stream creation failed");
-        return FS_E_HIP;
-      }
-    for (int k = 0; k < kLanes; ++k)
-      if (!xev[k] && hipEventCreateWithFlags(&xev[k], hipEventDisableTiming) != hipSuccess) {
-        fs_set_error_msg("lane event creation failed");
-        return FS_E_HIP;
-      }
+  // setup programs of the lanes of a job split over several lanes per prime,
+  // one after another in lprog (cached by path and the lanes' first n)
+  static constexpr int kLanes = FS_LANES;
+  void* lprog = nullptr;
+  size_t lprog_alloc = 0;  // bytes
+  std::vector<uint32_t> lprog_key;
+  uint32_t lpoff[kLanes] = {}, lterms[kLanes] = {};
+  // side stream of the verify products, and its end event
+  hipStream_t vs = nullptr;
+  hipEvent_t vev = nullptr;
+  int init_side() {
+    if (!vs && hipStreamCreateWithFlags(&vs, hipStreamNonBlocking) != hipSuccess) {
+      fs_set_error_msg("side stream creation failed");
+      return FS_E_HIP;
+    }
+    if (!vev && hipEventCreateWithFlags(&vev, hipEventDisableTiming) != hipSuccess) {
+      fs_set_error_msg("side event creation failed");
+      return FS_E_HIP;
+    }
     return FS_OK;
   }
   int init() {
@@ -941,8 +940,7 @@ struct Lease {
   }
   void give_back() {
     if (!d) return;
-    for (hipStream_t x : d->xs)
-      if (x) (void)hipStreamSynchronize(x);
+    if (d->vs) (void)hipStreamSynchronize(d->vs);
     if (d->s) (void)fs_stream_sync(d->s);
     std::lock_guard<std::mutex> lk(g_pool_mu);
     if (d->s) g_pool->push_back(d);  // a set without a stream is never handed out again
@@ -1460,112 +1458,126 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
       if ((rc = fs_memcpy_htod(d.ctr + FS_CTR_PRIMEOVF, &zero, 4, d.s, 1))) return rc;
     }
     d.c.grid_hint = count ? count : 1;
-    // a one-segment job that underfills the GPU runs on several lanes per prime
+    // a one-segment job that underfills the GPU runs on several lanes per
+    // prime: lane k is a copy of the candidates at k * stride (a multiple of
+    // the 256-thread block) covering n in [nb[k], nb[k + 1]), all lanes in one
+    // setup and one iterate launch
     const bool whole = (st.p == pmin && stop == st.pmax);
     const uint32_t L = (whole && o.n_chunk == 0) ? split_lanes(count, nmin, nmax, path) : 1;
-    if (L > 1 && (uint64_t)L * count > d.c.cap) {  // room for the lanes' copies: grow, redo the segment
+    const uint32_t stride = L > 1 ? (count + 255u) & ~255u : count;
+    if (L > 1 && (uint64_t)(L - 1) * stride + count > d.c.cap) {  // room for the copies: grow, redo the segment
       fs_cands_free(&d.c);
-      if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>((uint64_t)L * count + count / 4 + 4096,
+      if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>((uint64_t)L * stride + count / 4 + 4096,
                                                                   0xFFFFFF00ull))))
         return rc;
       continue;
     }
-    std::vector<uint32_t> nb(L + 1);  // lane k: n in [nb[k], nb[k + 1])
-    for (uint32_t k = 0; k <= L; ++k) nb[k] = nmin + (uint32_t)((uint64_t)(nmax - nmin) * k / L);
-    auto view = [&](uint32_t k) {
-      fs_cands v = d.c;
-      v.p += (size_t)k * count;
-      v.q += (size_t)k * count;
-      v.r2 += (size_t)k * count;
-      v.res += (size_t)k * count;
-      v.nnext += (size_t)k * count;
-      v.cap = count;
-      v.grid_hint = count ? count : 1;
-      return v;
-    };
-    std::vector<fs_cands> views(L);
-    for (uint32_t k = 0; k < L; ++k) views[k] = L > 1 ? view(k) : d.c;
-    std::vector<fs_stream> ls(L, d.s);
-    std::vector<const void*> lprog(L, d.prog);
-    std::vector<uint32_t> lterms(L, d.prog_terms);
+    fs_lane_set lanes{};
+    lanes.lanes = L;
+    lanes.count = count;
+    lanes.stride = stride;
+    for (uint32_t k = 0; k <= L; ++k) lanes.nb[k] = nmin + (uint32_t)((uint64_t)(nmax - nmin) * k / L);
+    fs_cands first = d.c;  // lane 0, and the candidates the checks read
+    if (L > 1) first.cap = count;
+    fs_cands top = first;  // the last lane: its residues are nmax! at the end
+    top.p += (size_t)(L - 1) * stride;
+    top.q += (size_t)(L - 1) * stride;
+    top.r2 += (size_t)(L - 1) * stride;
+    top.res += (size_t)(L - 1) * stride;
+    top.nnext += (size_t)(L - 1) * stride;
     if (L > 1) {
-      if ((rc = d.init_lanes())) return rc;
-      for (uint32_t k = 1; k < L; ++k) {
-        ls[k] = (fs_stream)d.xs[k - 1];
-        const std::shared_ptr<const fs_program> pr = fs_program_get(nb[k], path);
-        if (!pr) return FS_E_NOMEM;
-        if (d.xprog_path[k - 1] != path || d.xprog_nmin[k - 1] != nb[k]) {
-          const size_t bytes = std::max<size_t>(pr->words.size(), 1) * 8;
-          if (bytes > d.xprog_alloc[k - 1]) {
-            fs_free(d.xprog[k - 1]);
-            d.xprog[k - 1] = nullptr;
-            d.xprog_alloc[k - 1] = 0;
-            if ((rc = fs_malloc(&d.xprog[k - 1], bytes))) return rc;
-            d.xprog_alloc[k - 1] = bytes;
-          }
-          d.xprog_path[k - 1] = -1;
-          if ((rc = fs_memcpy_htod(d.xprog[k - 1], pr->words.data(), pr->words.size() * 8, d.s, 1))) return rc;
-          d.xprog_nmin[k - 1] = nb[k];
-          d.xprog_path[k - 1] = path;
+      std::vector<uint32_t> key{(uint32_t)path, L};
+      key.insert(key.end(), lanes.nb, lanes.nb + L);
+      if (key != d.lprog_key) {
+        std::vector<uint64_t> all;
+        for (uint32_t k = 0; k < L; ++k) {
+          const std::shared_ptr<const fs_program> pr = fs_program_get(lanes.nb[k], path);
+          if (!pr) return FS_E_NOMEM;
+          d.lpoff[k] = (uint32_t)all.size();
+          d.lterms[k] = pr->terms;
+          all.insert(all.end(), pr->words.begin(), pr->words.end());
+          all.resize((all.size() + 3) & ~(size_t)3, 0);  // the next program starts on a 4-word block
         }
-        lprog[k] = d.xprog[k - 1];
-        lterms[k] = pr->terms;
+        const size_t bytes = std::max<size_t>(all.size(), 4) * 8;
+        if (bytes > d.lprog_alloc) {
+          fs_free(d.lprog);
+          d.lprog = nullptr;
+          d.lprog_alloc = 0;
+          if ((rc = fs_malloc(&d.lprog, bytes))) return rc;
+          d.lprog_alloc = bytes;
+        }
+        d.lprog_key.clear();
+        if ((rc = fs_memcpy_htod(d.lprog, all.data(), all.size() * 8, d.s, 1))) return rc;
+        d.lprog_key = key;
       }
-      // the lanes' copies of the candidates (p, q, r2), in one launch
-      if ((rc = fs_cands_replicate(&d.c, count, L, d.s))) return rc;
-    }
-    if ((rc = fs_event_record(d.ev[2], d.s))) return rc;
-    if (L > 1) {  // the lanes start when the copies are done
-      if (hipEventRecord(d.xev[0], (hipStream_t)d.s) != hipSuccess) return FS_E_HIP;
-      for (uint32_t k = 1; k < L; ++k)
-        if (hipStreamWaitEvent(d.xs[k - 1], d.xev[0], 0) != hipSuccess) return FS_E_HIP;
-    }
-    // setup: n! mod p at each lane's first n, in chunks of the program
-    for (uint32_t k = 0; k < L; ++k) {
-      if (lterms[k] == 0) {
-        if ((rc = fs_setup(&views[k], d.ctr, lprog[k], 0, 0, 0, nb[k], path, ls[k]))) return rc;
+      lanes.progs = d.lprog;
+      for (uint32_t k = 0; k < L; ++k) {
+        lanes.poff[k] = d.lpoff[k];
+        lanes.terms[k] = d.lterms[k];
       }
-      for (uint32_t s0 = 0; s0 < lterms[k]; s0 += prog_chunk) {
-        const uint32_t s1 = std::min(lterms[k], s0 + prog_chunk);
-        if ((rc = fs_setup(&views[k], d.ctr, lprog[k], lterms[k], s0, s1, nb[k], path, ls[k]))) return rc;
-      }
-      if (k == 0 && (rc = fs_event_record(d.ev[3], d.s))) return rc;
+      if ((rc = fs_cands_replicate(&d.c, count, stride, L, d.s))) return rc;  // the lanes' p, q, r2
     }
 
+    // setup: n! mod p at each lane's first n, in chunks of the program
+    if ((rc = fs_event_record(d.ev[2], d.s))) return rc;
+    if (L > 1) {
+      uint32_t most = 0;
+      for (uint32_t k = 0; k < L; ++k) most = std::max(most, lanes.terms[k]);
+      for (uint32_t s0 = 0; s0 == 0 || s0 < most; s0 += prog_chunk)
+        if ((rc = fs_setup_lanes(&d.c, &lanes, s0, s0 + prog_chunk, path, d.s))) return rc;
+    } else {
+      if (d.prog_terms == 0 && (rc = fs_setup(&first, d.ctr, d.prog, 0, 0, 0, nmin, path, d.s))) return rc;
+      for (uint32_t s0 = 0; s0 < d.prog_terms; s0 += prog_chunk) {
+        const uint32_t s1 = std::min(d.prog_terms, s0 + prog_chunk);
+        if ((rc = fs_setup(&first, d.ctr, d.prog, d.prog_terms, s0, s1, nmin, path, d.s))) return rc;
+      }
+    }
     // power-table / setup cross-check on the first candidate of the first
-    // non-empty segment (verify kernels, cl_sieve.cpp:868-907)
-    if (!verified && count > 0) {
-      if ((rc = fs_verifyslow(&views[0], d.ctr, d.verify, nmin, d.s)) ||
-          (rc = fs_verifypow(&views[0], d.ctr, d.pt_bases, d.pt_exps, d.pt_count, d.verify, d.s)) ||
-          (rc = fs_verifyreduce(&views[0], d.ctr, d.verify, d.s)) ||
-          (rc = fs_verifyresult(&views[0], d.ctr, d.verify, d.s)))
+    // non-empty segment (verify kernels, cl_sieve.cpp:868-907): the products
+    // run on a side stream beside setup, enqueued after it (they read only p,
+    // q and r2, which the wait for the candidate count has made final), and
+    // the comparison with the setup residue follows setup
+    const bool verify_now = !verified && count > 0;
+    if (verify_now) {
+      if ((rc = d.init_side())) return rc;
+      const fs_stream vs = (fs_stream)d.vs;
+      if ((rc = fs_verifyslow(&first, d.ctr, d.verify, nmin, vs)) ||
+          (rc = fs_verifypow(&first, d.ctr, d.pt_bases, d.pt_exps, d.pt_count, d.verify, vs)) ||
+          (rc = fs_verifyreduce(&first, d.ctr, d.verify, vs)))
         return rc;
+      if (hipEventRecord(d.vev, d.vs) != hipSuccess) return FS_E_HIP;
+    }
+
+    if ((rc = fs_event_record(d.ev[3], d.s))) return rc;
+    if (verify_now) {
+      if (hipStreamWaitEvent((hipStream_t)d.s, d.vev, 0) != hipSuccess) return FS_E_HIP;
+      if ((rc = fs_verifyresult(&first, d.ctr, d.verify, d.s))) return rc;
       verify_pending = true;  // its flag is read with the segment's counters, before any result is taken
     }
 
-    // iterate over each lane's n in chunks, then fold the checksum of the last lane
+    // iterate over the n-range in chunks cut where the loops change, then fold
+    // the checksum of the last lane
     if ((rc = fs_event_record(d.ev[4], d.s))) return rc;
     const std::vector<uint32_t> cuts = loop_boundaries(path, st.p, stop, nmin, nmax);
-    for (uint32_t k = 0; k < L; ++k) {
+    if (L > 1) {
+      if ((rc = fs_iterate_lanes(&d.c, d.ctr, d.fac, d.fcap, &lanes, cuts.data(), (uint32_t)cuts.size(), path, d.s)))
+        return rc;
+      ++r.iterate_launches;
+    } else {
       size_t ci = 0;
-      for (uint32_t n0 = nb[k]; n0 < nb[k + 1];) {
-        uint32_t n1 = (nb[k + 1] - n0 > nchunk_max) ? n0 + nchunk_max : nb[k + 1];
+      for (uint32_t n0 = nmin; n0 < nmax;) {
+        uint32_t n1 = (nmax - n0 > nchunk_max) ? n0 + nchunk_max : nmax;
         while (ci < cuts.size() && cuts[ci] <= n0) ++ci;
         if (ci < cuts.size() && cuts[ci] < n1) n1 = cuts[ci];
-        if ((rc = fs_iterate(&views[k], d.ctr, d.fac, d.fcap, n0, n1, path, ls[k]))) return rc;
+        if ((rc = fs_iterate(&first, d.ctr, d.fac, d.fcap, n0, n1, path, d.s))) return rc;
         ++r.iterate_launches;
         n0 = n1;
       }
-      if (k > 0) {  // join lane k
-        if (hipEventRecord(d.xev[k], d.xs[k - 1]) != hipSuccess ||
-            hipStreamWaitEvent((hipStream_t)d.s, d.xev[k], 0) != hipSuccess)
-          return FS_E_HIP;
-      }
     }
     if ((rc = fs_event_record(d.ev[5], d.s))) return rc;
-    if ((rc = fs_check(&views[L - 1], d.ctr, d.sum, nmax, d.s))) return rc;
+    if ((rc = fs_check(&top, d.ctr, d.sum, nmax, d.s))) return rc;
     if ((rc = d.stage())) return rc;
-    for (uint32_t k = 1; k < L; ++k) r.setup_mulmods += (uint64_t)count * lterms[k];
+    for (uint32_t k = 1; k < L; ++k) r.setup_mulmods += (uint64_t)count * lanes.terms[k];
     if (L > 1) r.lanes = L;
     r.steps += (uint64_t)count * (nmax - nmin);
     r.setup_mulmods += (uint64_t)count * d.prog_terms;

@@ -275,6 +275,28 @@ def test_edge_ranges_match_oracle(gpu, args):
     assert r.factors == o["factors"]
 
 
+@pytest.mark.parametrize("args", [
+    (10 ** 8, 10 ** 8 + 2 * 10 ** 5, 1000, 10000),               # FP64: loop cut at 9,487 inside the last lane
+    (360287970 - 100000, 360287970 + 100000, 1000, 9000),         # FP64: exact/inexact pairs cut inside a lane
+    (4 * 10 ** 6, 5 * 10 ** 6, 11, 9000),                         # FP64 exact pairs, many factors
+    (2 ** 62, 2 ** 62 + 2 * 10 ** 5, 5000, 15000),                 # hybrid pairs
+    ((16385 << 39) - 100000, (16385 << 39) + 100000, 2000, 12000),  # hybrid just above 2^53
+    (2 ** 64 - 2 ** 58 - 100000, 2 ** 64 - 2 ** 58 + 100000, 3000, 13000),  # top of the high-word quotient
+])
+def test_lanes_per_prime_match_oracle(gpu, args):
+    """One-segment jobs that underfill the GPU run as several lanes per prime
+    (one setup and one iterate launch over all lanes, each lane from its own
+    setup program, loop cuts taken inside the kernel): the factor list and
+    checksum equal the single-lane run's and the oracle's."""
+    print("lanes", args, flush=True)
+    r = gpu.sieve(*args)
+    one = gpu.sieve(*args, n_chunk=1 << 22)  # an explicit n-chunk keeps one lane
+    o = ol.sieve(*args, threads=8)
+    assert r.lanes > 1 and one.lanes <= 1
+    assert (r.factorcount, r.primecount, r.checksum) == (o["factorcount"], o["primecount"], o["checksum"])
+    assert r.factors == o["factors"] == one.factors and one.checksum == r.checksum
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 def test_random_jobs_match_oracle(gpu, seed):
     """Seeded random jobs over every modulus size from 2^17 to 2^64 (every
