@@ -67,6 +67,10 @@ OPS_PER_STEP_HYBRID_PAIR = 7.0                  # hybrid paired steps (nmax <= 2
 OPS_PER_STEP_FP64_EXACT = 4.5                   # FP64 with an exact product (p * nmax <= 2^53): 36 per 8
 OPS_PER_STEP_FP64_PAIR = 3.5                    # FP64 paired steps (64 nmax <= 31 p): 8 x (6 + 1 compare) per 16
 OPS_PER_STEP_FP64_PAIR_EXACT = 2.5              # paired steps with an exact product (p nmax^2 <= 2^53): 8 x (4 + 1)
+# VALU lane-ops per setup program term (PMC of k_setup_fp64 / k_setup_hybrid):
+# counted in the iterate window of a job run as lanes per prime, whose one
+# launch (k_lanes) sets up and iterates
+SETUP_OPS_PER_TERM = {1: 6.3, 2: 18.1}
 
 
 def fp64_ops_at(pmin, pmax, n1):
@@ -281,7 +285,11 @@ def plain_leg(fs, workload, steps, warmup):
         ops = OPS_PER_STEP_HYBRID_HI
     if ref.path == 1:
         ops = fp64_ops(pmin, pmax, nmin, nmax)
-    achieved = steps_total * ops / (it_ms * 1e-3) / 1e12
+    window_ops = steps_total * ops
+    if ref.lanes > 1:  # the lanes' launch also runs every lane's setup
+        window_ops += sum(r.setup_mulmods for r in results) * SETUP_OPS_PER_TERM[ref.path]
+        ops = window_ops / steps_total
+    achieved = window_ops / (it_ms * 1e-3) / 1e12
     traffic = None
     if ref.path in TRAFFIC_BYTES_PER_CAND:
         traffic = TRAFFIC_BYTES_PER_CAND[ref.path] * ref.primecount * max(ref.lanes, 1) / max(ref.iterate_launches, 1)
@@ -292,7 +300,7 @@ def plain_leg(fs, workload, steps, warmup):
         "segments": ref.segments, "lanes_per_prime": ref.lanes,
         "steps_per_s": steps_total / elapsed, "ms_per_job": elapsed / steps * 1e3,
         "roofline": {"bound": "valu",
-                     "kernel": ("k_iterate_lanes<%s>" % {1: "FP64", 2: "HYBRID"}[ref.path]) if ref.lanes > 1
+                     "kernel": ("k_lanes<%s>" % {1: "FP64", 2: "HYBRID"}[ref.path]) if ref.lanes > 1
                      else "k_iterate_" + {1: "fp64", 2: "hybrid", 3: "mont"}[ref.path],
                      "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "TOP/s (VALU lane-ops)",
                      "frac": achieved / VALU_PEAK_TOPS, "traffic": traffic, "traffic_unit": "bytes per launch",

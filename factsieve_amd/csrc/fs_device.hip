@@ -54,7 +54,11 @@ static hipStream_t S(fs_stream s) { return (hipStream_t)s; }
 
 // ============================================================ device helpers
 
+// Candidates of the segment the kernels work on: the count getsegprimes
+// reserved, at most cap; none after a candidate-buffer overflow (the driver
+// then redoes the segment with room for the count).
 __device__ __forceinline__ uint32_t fs_live(const uint32_t* ctr, uint32_t cap) {
+  if (ctr[FS_CTR_PRIMEOVF]) return 0;
   const uint32_t n = ctr[FS_CTR_COUNT];
   return n < cap ? n : cap;
 }
@@ -182,8 +186,10 @@ __global__ void k_fill_ntab(double* t, double* pt, uint32_t half, uint32_t first
 }
 
 // Lane copies of a segment's candidates (p, q, r2): entry i of lane k is
-// entry i, at k * stride + i; grid-stride over (lanes - 1) * count entries.
-__global__ void k_replicate(fs_cands c, uint32_t count, uint32_t stride, uint32_t lanes) {
+// entry i, at k * stride + i; grid-stride over (lanes - 1) * count entries,
+// count = the segment's candidates (at most stride).
+__global__ void k_replicate(fs_cands c, const uint32_t* ctr, uint32_t stride, uint32_t lanes) {
+  const uint32_t count = fs_live(ctr, stride);
   const uint64_t total = (uint64_t)(lanes - 1) * count;
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
     const uint32_t i = (uint32_t)(t % count);
@@ -195,11 +201,11 @@ __global__ void k_replicate(fs_cands c, uint32_t count, uint32_t stride, uint32_
 }
 
 // ---- lanes per prime in one launch: lane k of a segment is the copy of its
-// `count` candidates at k * stride (stride a multiple of FS_BLOCK, so a block
-// lies in one lane and the lane's n-range, program and table pointers are
-// uniform over the block), covering n in [nb[k], nb[k + 1]).
+// candidates (at most stride of them) at k * stride (stride a multiple of
+// FS_BLOCK, so a block lies in one lane and the lane's n-range, program and
+// table pointers are uniform over the block), covering n in [nb[k], nb[k + 1]).
 struct LaneArgs {
-  uint32_t lanes, count, stride, ncuts;
+  uint32_t lanes, stride, ncuts;
   uint32_t nb[FS_LANES + 1];
   uint32_t terms[FS_LANES + 1];    // setup program words of each lane (entry FS_LANES unused:
   uint32_t poff[FS_LANES + 1];     // fs_pick reads FS_LANES + 1 entries) and their offsets
@@ -209,11 +215,11 @@ struct LaneArgs {
 
 // This thread's lane (a scalar: the same for the whole block) and global
 // index; false when it holds no candidate.
-__device__ __forceinline__ bool fs_lane_of(const LaneArgs& a, uint32_t* lane, uint32_t* i) {
+__device__ __forceinline__ bool fs_lane_of(const LaneArgs& a, const uint32_t* ctr, uint32_t* lane, uint32_t* i) {
   const uint32_t b0 = blockIdx.x * FS_BLOCK;
   *lane = __builtin_amdgcn_readfirstlane(b0 / a.stride);
   *i = b0 + threadIdx.x;
-  return *lane < a.lanes && *i - *lane * a.stride < a.count;
+  return *lane < a.lanes && *i - *lane * a.stride < fs_live(ctr, a.stride);
 }
 // Entry `lane` of a per-lane argument array, selected without indexing (so a
 // scalar lane keeps the value in scalar registers, as the scalar program and
@@ -227,37 +233,25 @@ __device__ __forceinline__ T fs_pick(const T* v, uint32_t lane) {
   return r;
 }
 
-// Setup chunk [start, end) of every lane's program (a lane whose program ended
-// in an earlier chunk is left alone); each lane's residue becomes nb[k]! mod p.
-// The programs (progs + poff[k]) and tables are __restrict__ kernel arguments
-// so that their loads stay scalar.
+// Every lane: setup (its whole program: the residue becomes nb[k]! mod p),
+// then its n-range as one launch per loop interval would run it.  A lane with
+// a short program starts iterating while longer ones still set up.  The first
+// candidate's setup residue is also written to *res0 for the power-table
+// cross-check (the iterate then moves the residue on).
 template <int PATH>
-__global__ __launch_bounds__(FS_BLOCK) void k_setup_lanes(fs_cands c, LaneArgs a, const uint64_t* __restrict__ progs,
-                                                          uint32_t start, uint32_t end) {
+__global__ __launch_bounds__(FS_BLOCK) void k_lanes(fs_cands c, uint32_t* ctr, fs_factor* fac, uint32_t fcap, LaneArgs a,
+                                                    const uint64_t* __restrict__ progs, const double* __restrict__ ntab,
+                                                    const double* __restrict__ pairs, uint64_t* res0) {
   uint32_t lane, i;
-  if (!fs_lane_of(a, &lane, &i)) return;
-  const uint32_t terms = fs_pick(a.terms, lane), e = end < terms ? end : terms;
-  if (start > 0 && start >= e) return;
-  const uint64_t* prog = progs + fs_pick(a.poff, lane);
-  if (PATH == FS_PATH_FP64)
-    c.res[i] = fs_cand_setup_fp64(c.p[i], c.q[i], c.r2[i], c.res[i], (const double*)prog, start, e);
-  else
-    c.res[i] = fs_cand_setup_hybrid(c.p[i], c.q[i], c.r2[i], c.res[i], prog, terms, start, e);
-  c.nnext[i] = fs_pick(a.nb, lane);
-}
-
-// Every lane's whole n-range, as one launch per loop interval would run it.
-template <int PATH>
-__global__ __launch_bounds__(FS_BLOCK) void k_iterate_lanes(fs_cands c, uint32_t* ctr, fs_factor* fac, uint32_t fcap,
-                                                            LaneArgs a, const double* __restrict__ ntab,
-                                                            const double* __restrict__ pairs) {
-  uint32_t lane, i;
-  if (!fs_lane_of(a, &lane, &i)) return;
+  if (!fs_lane_of(a, ctr, &lane, &i)) return;
   const uint64_t p = c.p[i];
+  const uint32_t terms = fs_pick(a.terms, lane);
+  const uint64_t* prog = progs + fs_pick(a.poff, lane);
+  uint64_t s = PATH == FS_PATH_FP64 ? fs_cand_setup_fp64(p, c.q[i], c.r2[i], 0, (const double*)prog, 0, terms)
+                                    : fs_cand_setup_hybrid(p, c.q[i], c.r2[i], 0, prog, terms, 0, terms);
+  if (i == 0) *res0 = s;
   const uint32_t hi = fs_pick(a.nb, lane + 1);
   uint32_t lo = fs_pick(a.nb, lane);
-  if (c.nnext[i] != lo) atomicOr(&ctr[FS_CTR_VALIDATE], 1u);
-  uint64_t s = c.res[i];
   while (lo < hi) {
     uint32_t b = hi;
     for (uint32_t k = 0; k < a.ncuts; ++k)
@@ -301,7 +295,7 @@ __global__ __launch_bounds__(FS_BLOCK) void k_check(fs_cands c, uint32_t* ctr, u
     if (t) atomicAdd((unsigned long long*)&sum[1], (unsigned long long)t);
     if (blockIdx.x == 0) {
       atomicAdd((unsigned long long*)&sum[0], (unsigned long long)live);
-      atomicMax(&ctr[FS_CTR_MAXCOUNT], ctr[FS_CTR_COUNT]);
+      if (!ctr[FS_CTR_PRIMEOVF]) atomicMax(&ctr[FS_CTR_MAXCOUNT], ctr[FS_CTR_COUNT]);
     }
   }
 }
@@ -377,7 +371,9 @@ __global__ __launch_bounds__(FS_BLOCK) void k_verifyreduce(fs_cands c, const uin
   }
 }
 
-__global__ __launch_bounds__(FS_BLOCK) void k_verifyresult(fs_cands c, uint32_t* ctr, VerifyPair* ver) {
+// res0: the setup residue of the first candidate, or null for c.res[0].
+__global__ __launch_bounds__(FS_BLOCK) void k_verifyresult(fs_cands c, uint32_t* ctr, VerifyPair* ver,
+                                                           const uint64_t* res0) {
   __shared__ uint64_t sh[FS_BLOCK];
   if (fs_live(ctr, c.cap) == 0) return;
   const uint64_t p = c.p[0], q = c.q[0], r2 = c.r2[0];
@@ -389,7 +385,7 @@ __global__ __launch_bounds__(FS_BLOCK) void k_verifyresult(fs_cands c, uint32_t*
   if (threadIdx.x == 0) {
     // slow product == power table product == the setup kernel's residue
     const uint64_t plain = fs_mont_mul(a, 1, p, q);
-    if (a != b || plain != c.res[0]) atomicOr(&ctr[FS_CTR_POWERTABLE], 1u);
+    if (a != b || plain != (res0 ? *res0 : c.res[0])) atomicOr(&ctr[FS_CTR_POWERTABLE], 1u);
   }
 }
 
@@ -529,14 +525,14 @@ extern "C" int fs_inject_fault(int idx) {
   return FS_OK;
 }
 
-int fs_cands_replicate(const fs_cands* c, uint32_t count, uint32_t stride, uint32_t lanes, fs_stream s) {
-  if (lanes < 2 || count == 0) return FS_OK;
-  if (stride < count || (uint64_t)(lanes - 1) * stride + count > c->cap) {
+int fs_cands_replicate(const fs_cands* c, const uint32_t* d_ctr, uint32_t stride, uint32_t lanes, fs_stream s) {
+  if (lanes < 2 || stride == 0) return FS_OK;
+  if ((uint64_t)lanes * stride > c->cap) {
     g_last_error = "cands_replicate: capacity below the lanes' copies";
     return FS_E_ARG;
   }
-  const uint64_t total = (uint64_t)(lanes - 1) * count;
-  k_replicate<<<(uint32_t)std::min<uint64_t>((total + 255) / 256, 4096), 256, 0, S(s)>>>(*c, count, stride, lanes);
+  const uint64_t total = (uint64_t)(lanes - 1) * stride;
+  k_replicate<<<(uint32_t)std::min<uint64_t>((total + 255) / 256, 4096), 256, 0, S(s)>>>(*c, d_ctr, stride, lanes);
   HIPCHK(hipGetLastError(), "cands_replicate");
   return FS_OK;
 }
@@ -904,6 +900,11 @@ extern "C" int fs_iterate(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_facto
   return FS_OK;
 }
 
+// The last slot of the verify buffer (past the FS_VERIFY_L2 partial products)
+// holds the setup residue of the first candidate when lanes run setup and
+// iterate in one launch.
+static uint64_t* fs_verify_res0(void* d_verify) { return &((VerifyPair*)d_verify)[FS_VERIFY_BLOCKS + 63].slow; }
+
 static int lane_args(const fs_cands* c, const fs_lane_set* ls, int path, LaneArgs* a, const char* who) {
   int rc = check_cands(c, who);
   if (rc) return rc;
@@ -911,14 +912,13 @@ static int lane_args(const fs_cands* c, const fs_lane_set* ls, int path, LaneArg
     g_last_error = std::string(who) + ": lanes run on the FP64 and hybrid paths only";
     return FS_E_ARG;
   }
-  if (ls->lanes < 1 || ls->lanes > FS_LANES || ls->stride % FS_BLOCK || ls->stride < ls->count ||
-      (uint64_t)(ls->lanes - 1) * ls->stride + ls->count > c->cap) {
+  if (ls->lanes < 1 || ls->lanes > FS_LANES || ls->stride == 0 || ls->stride % FS_BLOCK ||
+      (uint64_t)ls->lanes * ls->stride > c->cap) {
     g_last_error = std::string(who) + ": bad lane layout";
     return FS_E_ARG;
   }
   memset(a, 0, sizeof *a);
   a->lanes = ls->lanes;
-  a->count = ls->count;
   a->stride = ls->stride;
   for (uint32_t k = 0; k <= ls->lanes; ++k) a->nb[k] = ls->nb[k];
   for (uint32_t k = 0; k < ls->lanes; ++k) {
@@ -936,40 +936,20 @@ static int lane_args(const fs_cands* c, const fs_lane_set* ls, int path, LaneArg
   return FS_OK;
 }
 
-static uint32_t lane_grid(const fs_lane_set* ls) {
-  return (uint32_t)(((uint64_t)(ls->lanes - 1) * ls->stride + ls->count + FS_BLOCK - 1) / FS_BLOCK);
-}
+static uint32_t lane_grid(const fs_lane_set* ls) { return (uint32_t)((uint64_t)ls->lanes * ls->stride / FS_BLOCK); }
 
-int fs_setup_lanes(const fs_cands* c, const fs_lane_set* ls, uint32_t start, uint32_t end, int path, fs_stream s) {
+int fs_lanes(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t fcap, const fs_lane_set* ls,
+             const uint32_t* cuts, uint32_t ncuts, void* d_verify, int path, fs_stream s) {
   LaneArgs a;
-  int rc = lane_args(c, ls, path, &a, "setup_lanes");
+  int rc = lane_args(c, ls, path, &a, "lanes");
   if (rc) return rc;
-  if ((start | end) & 3) {
-    g_last_error = "setup_lanes: program chunks must start and end on 4-term blocks";
-    return FS_E_ARG;
-  }
   for (uint32_t k = 0; k < ls->lanes; ++k)
     if ((ls->terms[k] & 3) || (ls->poff[k] & 3) || !ls->progs) {
-      g_last_error = "setup_lanes: bad program";
+      g_last_error = "lanes: bad setup program";
       return FS_E_ARG;
     }
-  if (ls->count == 0) return FS_OK;
-  const uint64_t* pg = (const uint64_t*)ls->progs;
-  if (path == FS_PATH_FP64)
-    k_setup_lanes<FS_PATH_FP64><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, a, pg, start, end);
-  else
-    k_setup_lanes<FS_PATH_HYBRID><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, a, pg, start, end);
-  LAUNCHCHK("setup_lanes");
-  return FS_OK;
-}
-
-int fs_iterate_lanes(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t fcap, const fs_lane_set* ls,
-                     const uint32_t* cuts, uint32_t ncuts, int path, fs_stream s) {
-  LaneArgs a;
-  int rc = lane_args(c, ls, path, &a, "iterate_lanes");
-  if (rc) return rc;
-  if (ncuts > 4) {
-    g_last_error = "iterate_lanes: at most 4 loop boundaries";
+  if (ncuts > 4 || !d_verify) {
+    g_last_error = "lanes: at most 4 loop boundaries, and a verify buffer";
     return FS_E_ARG;
   }
   a.ncuts = ncuts;
@@ -985,15 +965,27 @@ int fs_iterate_lanes(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, ui
     a.tbase = t.base;
     a.thalf = t.half();
   }
-  if (ls->count) {
-    if (path == FS_PATH_FP64)
-      k_iterate_lanes<FS_PATH_FP64><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, a, tab, pairs);
-    else
-      k_iterate_lanes<FS_PATH_HYBRID><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, a, tab, pairs);
-    LAUNCHCHK("iterate_lanes");
-  }
+  const uint64_t* pg = (const uint64_t*)ls->progs;
+  uint64_t* res0 = fs_verify_res0(d_verify);
+  if (path == FS_PATH_FP64)
+    k_lanes<FS_PATH_FP64><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, a, pg, tab, pairs, res0);
+  else
+    k_lanes<FS_PATH_HYBRID><<<lane_grid(ls), FS_BLOCK, 0, S(s)>>>(*c, d_ctr, d_factor, fcap, a, pg, tab, pairs, res0);
+  LAUNCHCHK("lanes");
   if (g_inject == FS_CTR_FACTOROVF) {
     k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_FACTOROVF);
+    g_inject = 0;
+  }
+  return FS_OK;
+}
+
+int fs_verifyresult_res0(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_stream s) {
+  int rc = check_cands(c, "verifyresult");
+  if (rc) return rc;
+  k_verifyresult<<<1, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (VerifyPair*)d_verify, fs_verify_res0(d_verify));
+  LAUNCHCHK("verifyresult");
+  if (g_inject == FS_CTR_POWERTABLE) {
+    k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_POWERTABLE);
     g_inject = 0;
   }
   return FS_OK;
@@ -1094,7 +1086,7 @@ extern "C" int fs_verifyreduce(const fs_cands* c, const uint32_t* d_ctr, void* d
 extern "C" int fs_verifyresult(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_stream s) {
   int rc = check_cands(c, "verifyresult");
   if (rc) return rc;
-  k_verifyresult<<<1, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (VerifyPair*)d_verify);
+  k_verifyresult<<<1, FS_BLOCK, 0, S(s)>>>(*c, d_ctr, (VerifyPair*)d_verify, nullptr);
   LAUNCHCHK("verifyresult");
   if (g_inject == FS_CTR_POWERTABLE) {
     k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_POWERTABLE);

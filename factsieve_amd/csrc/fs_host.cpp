@@ -331,9 +331,15 @@ std::shared_ptr<const fs_program> fs_program_get(uint32_t nmin, int path) {
 
 // Deterministic primality for odd 64-bit p: strong probable-prime tests to the
 // seven bases {2, 325, 9375, 28178, 450775, 9780504, 1795265022} (Jim Sinclair's
-// set, as used by isPrime, verifyprime.c:96-132), bases reduced mod p.
+// set, as used by isPrime, verifyprime.c:96-132), bases reduced mod p; below
+// 4,759,123,141 the three bases {2, 7, 61} suffice (Jaeschke 1993: the least
+// strong pseudoprime to all three is 4,759,123,141).
 extern "C" int fs_is_prime(uint64_t p) {
-  static const uint32_t bases[7] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+  static const uint32_t sinclair[7] = {2, 325, 9375, 28178, 450775, 9780504, 1795265022};
+  static const uint32_t small[3] = {2, 7, 61};
+  const bool below = p < 4759123141ull;
+  const uint32_t* bases = below ? small : sinclair;
+  const int nb = below ? 3 : 7;
   if (p < 2) return 0;
   if (p < 4) return 1;
   if ((p & 1) == 0) return 0;
@@ -344,7 +350,7 @@ extern "C" int fs_is_prime(uint64_t p) {
   uint64_t d = p - 1;
   const int t = __builtin_ctzll(d);
   d >>= t;
-  for (int i = 0; i < 7; ++i) {
+  for (int i = 0; i < nb; ++i) {
     uint64_t b = bases[i];
     if (b >= p) {
       b %= p;
@@ -1171,21 +1177,45 @@ static void account(DevBufs& d, fs_result* r) {
 // program of its own first n, as long as the costliest extra setup is at most
 // half a subrange's iterate work (VALU operations per setup term and per step
 // of each path).  1 = no split.
+// Setup program terms at n, about 0.047 n (FP64, multipliers < 2^50) or 0.037 n
+// (hybrid, 60-bit words), and VALU operations per term and per step.
+static void lane_costs(int path, double* terms_per_n, double* term_ops, double* step_ops) {
+  *terms_per_n = path == FS_PATH_FP64 ? 0.047 : 0.037;
+  *term_ops = path == FS_PATH_FP64 ? 6.3 : 18.0;
+  *step_ops = path == FS_PATH_FP64 ? 4.0 : 7.2;
+}
+
 static uint32_t split_lanes(uint32_t count, uint32_t nmin, uint32_t nmax, int path) {
   const uint64_t slots = fs_auto_segment_candidates() / 8;
   if (count == 0 || 2 * (uint64_t)count > slots || path == FS_PATH_MONT) return 1;
   uint32_t L = (uint32_t)std::min<uint64_t>(DevBufs::kLanes, slots / count);
   L = std::min<uint32_t>(L, (nmax - nmin) / 2048);
-  // setup program terms at n, about 0.047 n (FP64, multipliers < 2^50) or
-  // 0.037 n (hybrid, 60-bit words), and VALU operations per term and per step
-  const double terms_per_n = path == FS_PATH_FP64 ? 0.047 : 0.037;
-  const double term_ops = path == FS_PATH_FP64 ? 6.3 : 18.0;
-  const double step_ops = path == FS_PATH_FP64 ? 4.0 : 7.2;
+  double terms_per_n, term_ops, step_ops;
+  lane_costs(path, &terms_per_n, &term_ops, &step_ops);
   for (; L > 1; --L) {
     const double nlast = nmin + (double)(nmax - nmin) * (L - 1) / L;
     if (2.0 * (nlast * terms_per_n + 64) * term_ops <= (double)(nmax - nmin) / L * step_ops) break;
   }
   return std::max<uint32_t>(L, 1);
+}
+
+// Lane boundaries nb[0] = nmin < ... < nb[L] = nmax with equal work per lane,
+// the setup of nb[k]! (a n per n) plus the lane's steps (c per n):
+// a nb[k] + c (nb[k+1] - nb[k]) = W, so nb[k+1] = r nb[k] + W / c with
+// r = 1 - a / c, and W follows from nb[L] = nmax.
+static void lane_bounds(uint32_t nmin, uint32_t nmax, uint32_t L, int path, uint32_t* nb) {
+  double terms_per_n, term_ops, step_ops;
+  lane_costs(path, &terms_per_n, &term_ops, &step_ops);
+  const double r = 1.0 - terms_per_n * term_ops / step_ops, rl = pow(r, (double)L);
+  const double w = (1.0 - r) * ((double)nmax - rl * nmin) / (1.0 - rl);  // W / c
+  nb[0] = nmin;
+  double x = nmin;
+  for (uint32_t k = 1; k < L; ++k) {
+    x = r * x + w;
+    const uint32_t v = (uint32_t)std::llround(x);
+    nb[k] = std::min(std::max(v, nb[k - 1] + 1), nmax - (L - k));  // strictly increasing, inside the range
+  }
+  nb[L] = nmax;
 }
 
 extern "C" int fs_trickle_up_text(double progress, double cputime, double runtime, char* buf, size_t len) {
@@ -1416,6 +1446,8 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   const uint32_t target = o.seg_candidates ? o.seg_candidates : fs_auto_segment_candidates();
   const uint32_t nchunk_max = o.n_chunk ? o.n_chunk : (1u << 22);
   bool verified = false, verify_pending = false;
+  uint64_t cand_floor = 0;  // room for candidates at least this large after an overflow
+  int overflows = 0;
   double last_ckpt = now_s(), last_prog = now_s();
   Boinc bx(o.boinc);
   const double irsize = 1.0 / (double)(pmax - pmin ? pmax - pmin : 1);
@@ -1430,55 +1462,38 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     uint64_t stop = st.p + len;
     if (stop > st.pmax || stop < st.p) stop = st.pmax;
     len = stop - st.p;
-    const uint64_t need = fs_cand_capacity(st.p, len);
-    if (need > d.c.cap) {
-      fs_cands_free(&d.c);
-      if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>(need + need / 4, 0xFFFFFF00ull)))) return rc;
-    }
-    // candidates of the segment; exact count read back to size the grids
-    uint32_t count = 0;
-    for (int attempt = 0;; ++attempt) {
-      d.c.grid_hint = 0;
-      if ((rc = fs_clearn(d.ctr, d.s)) || (rc = fs_event_record(d.ev[0], d.s)) ||
-          (rc = fs_getsegprimes(st.p, stop, &d.c, d.ctr, d.s)) || (rc = fs_event_record(d.ev[1], d.s)))
-        return rc;
-      uint32_t h[FS_CTR_WORDS];
-      if ((rc = d.read_ctr(h))) return rc;
-      count = h[FS_CTR_COUNT];
-      if (count <= d.c.cap && !h[FS_CTR_PRIMEOVF]) break;
-      if (attempt > 2) {
-        fs_set_error_msg("error: gpu prime array overflow");
-        return FS_E_OVERFLOW;
-      }
-      // grow and redo the segment (nothing of it was folded yet)
-      const uint64_t ncap = std::min<uint64_t>((uint64_t)count + count / 2 + 65536, 0xFFFFFF00ull);
-      fs_cands_free(&d.c);
-      if ((rc = fs_cands_alloc(&d.c, (uint32_t)ncap))) return rc;
-      uint32_t zero = 0;
-      if ((rc = fs_memcpy_htod(d.ctr + FS_CTR_PRIMEOVF, &zero, 4, d.s, 1))) return rc;
-    }
-    d.c.grid_hint = count ? count : 1;
+    // Room for the segment's candidates from an upper estimate (the count is
+    // not read back: every kernel reads it on the device); a segment whose
+    // count overflows it is redone, with room for that count, below.
+    const uint64_t need = std::min<uint64_t>(std::max<uint64_t>(fs_cand_capacity(st.p, len), cand_floor), 0xFFFFFE00ull);
     // a one-segment job that underfills the GPU runs on several lanes per
     // prime: lane k is a copy of the candidates at k * stride (a multiple of
     // the 256-thread block) covering n in [nb[k], nb[k + 1]), all lanes in one
-    // setup and one iterate launch
+    // setup and one iterate launch; the lane count follows the expected
+    // number of candidates, len / ln p
     const bool whole = (st.p == pmin && stop == st.pmax);
-    const uint32_t L = (whole && o.n_chunk == 0) ? split_lanes(count, nmin, nmax, path) : 1;
-    const uint32_t stride = L > 1 ? (count + 255u) & ~255u : count;
-    if (L > 1 && (uint64_t)(L - 1) * stride + count > d.c.cap) {  // room for the copies: grow, redo the segment
+    const double expect = (double)len / std::max(log((double)st.p + 0.5 * (double)len), 1.0);
+    const uint32_t L = (whole && o.n_chunk == 0) ? split_lanes((uint32_t)std::min(expect, 4.0e9), nmin, nmax, path) : 1;
+    const uint32_t stride = (uint32_t)((need + 255u) & ~(uint64_t)255u);
+    const uint64_t room = L > 1 ? (uint64_t)L * stride : need;
+    if (room > d.c.cap) {
       fs_cands_free(&d.c);
-      if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>((uint64_t)L * stride + count / 4 + 4096,
-                                                                  0xFFFFFF00ull))))
-        return rc;
-      continue;
+      if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>(room + room / 4, 0xFFFFFF00ull)))) return rc;
     }
+    // getsegprimes writes at most `need` candidates (lane 0's stride with
+    // lanes): the kernels' grids cover that many, and a larger count raises
+    // the overflow flag instead
+    d.c.grid_hint = (uint32_t)need;
+    fs_cands gen = d.c;
+    gen.cap = L > 1 ? stride : (uint32_t)need;
+    if ((rc = fs_clearn(d.ctr, d.s)) || (rc = fs_event_record(d.ev[0], d.s)) ||
+        (rc = fs_getsegprimes(st.p, stop, &gen, d.ctr, d.s)) || (rc = fs_event_record(d.ev[1], d.s)))
+      return rc;
     fs_lane_set lanes{};
     lanes.lanes = L;
-    lanes.count = count;
     lanes.stride = stride;
-    for (uint32_t k = 0; k <= L; ++k) lanes.nb[k] = nmin + (uint32_t)((uint64_t)(nmax - nmin) * k / L);
-    fs_cands first = d.c;  // lane 0, and the candidates the checks read
-    if (L > 1) first.cap = count;
+    lane_bounds(nmin, nmax, L, path, lanes.nb);
+    fs_cands first = gen;  // lane 0, and the candidates the checks read
     fs_cands top = first;  // the last lane: its residues are nmax! at the end
     top.p += (size_t)(L - 1) * stride;
     top.q += (size_t)(L - 1) * stride;
@@ -1515,55 +1530,54 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
         lanes.poff[k] = d.lpoff[k];
         lanes.terms[k] = d.lterms[k];
       }
-      if ((rc = fs_cands_replicate(&d.c, count, stride, L, d.s))) return rc;  // the lanes' p, q, r2
+      if ((rc = fs_cands_replicate(&d.c, d.ctr, stride, L, d.s))) return rc;  // the lanes' p, q, r2
     }
 
-    // setup: n! mod p at each lane's first n, in chunks of the program
-    if ((rc = fs_event_record(d.ev[2], d.s))) return rc;
+    // power-table / setup cross-check on the first candidate of the first
+    // non-empty segment (verify kernels, cl_sieve.cpp:868-907): the products
+    // run on a side stream (they read only p, q and r2, written by
+    // getsegprimes, which the side stream waits for), enqueued after the
+    // segment's setup; the comparison with the setup residue follows setup
+    const bool verify_now = !verified;
+    auto verify_products = [&]() -> int {
+      if (!verify_now) return FS_OK;
+      int e;
+      if ((e = d.init_side())) return e;
+      const fs_stream vs = (fs_stream)d.vs;
+      if (hipStreamWaitEvent(d.vs, (hipEvent_t)d.ev[1], 0) != hipSuccess) return FS_E_HIP;
+      if ((e = fs_verifyslow(&first, d.ctr, d.verify, nmin, vs)) ||
+          (e = fs_verifypow(&first, d.ctr, d.pt_bases, d.pt_exps, d.pt_count, d.verify, vs)) ||
+          (e = fs_verifyreduce(&first, d.ctr, d.verify, vs)))
+        return e;
+      if (hipEventRecord(d.vev, d.vs) != hipSuccess) return FS_E_HIP;
+      return hipStreamWaitEvent((hipStream_t)d.s, d.vev, 0) == hipSuccess ? FS_OK : FS_E_HIP;
+    };
+    const std::vector<uint32_t> cuts = loop_boundaries(path, st.p, stop, nmin, nmax);
     if (L > 1) {
-      uint32_t most = 0;
-      for (uint32_t k = 0; k < L; ++k) most = std::max(most, lanes.terms[k]);
-      for (uint32_t s0 = 0; s0 == 0 || s0 < most; s0 += prog_chunk)
-        if ((rc = fs_setup_lanes(&d.c, &lanes, s0, s0 + prog_chunk, path, d.s))) return rc;
+      // setup and iterate of every lane in one launch (no separate setup window)
+      if ((rc = fs_event_record(d.ev[2], d.s)) || (rc = fs_event_record(d.ev[3], d.s)) ||
+          (rc = fs_event_record(d.ev[4], d.s)))
+        return rc;
+      if ((rc = fs_lanes(&d.c, d.ctr, d.fac, d.fcap, &lanes, cuts.data(), (uint32_t)cuts.size(), d.verify, path,
+                         d.s)))
+        return rc;
+      ++r.iterate_launches;
+      if ((rc = fs_event_record(d.ev[5], d.s))) return rc;
+      if ((rc = verify_products())) return rc;
+      if (verify_now && (rc = fs_verifyresult_res0(&first, d.ctr, d.verify, d.s))) return rc;
     } else {
+      // setup: n! mod p, in chunks of the program
+      if ((rc = fs_event_record(d.ev[2], d.s))) return rc;
       if (d.prog_terms == 0 && (rc = fs_setup(&first, d.ctr, d.prog, 0, 0, 0, nmin, path, d.s))) return rc;
       for (uint32_t s0 = 0; s0 < d.prog_terms; s0 += prog_chunk) {
         const uint32_t s1 = std::min(d.prog_terms, s0 + prog_chunk);
         if ((rc = fs_setup(&first, d.ctr, d.prog, d.prog_terms, s0, s1, nmin, path, d.s))) return rc;
       }
-    }
-    // power-table / setup cross-check on the first candidate of the first
-    // non-empty segment (verify kernels, cl_sieve.cpp:868-907): the products
-    // run on a side stream beside setup, enqueued after it (they read only p,
-    // q and r2, which the wait for the candidate count has made final), and
-    // the comparison with the setup residue follows setup
-    const bool verify_now = !verified && count > 0;
-    if (verify_now) {
-      if ((rc = d.init_side())) return rc;
-      const fs_stream vs = (fs_stream)d.vs;
-      if ((rc = fs_verifyslow(&first, d.ctr, d.verify, nmin, vs)) ||
-          (rc = fs_verifypow(&first, d.ctr, d.pt_bases, d.pt_exps, d.pt_count, d.verify, vs)) ||
-          (rc = fs_verifyreduce(&first, d.ctr, d.verify, vs)))
-        return rc;
-      if (hipEventRecord(d.vev, d.vs) != hipSuccess) return FS_E_HIP;
-    }
-
-    if ((rc = fs_event_record(d.ev[3], d.s))) return rc;
-    if (verify_now) {
-      if (hipStreamWaitEvent((hipStream_t)d.s, d.vev, 0) != hipSuccess) return FS_E_HIP;
-      if ((rc = fs_verifyresult(&first, d.ctr, d.verify, d.s))) return rc;
-      verify_pending = true;  // its flag is read with the segment's counters, before any result is taken
-    }
-
-    // iterate over the n-range in chunks cut where the loops change, then fold
-    // the checksum of the last lane
-    if ((rc = fs_event_record(d.ev[4], d.s))) return rc;
-    const std::vector<uint32_t> cuts = loop_boundaries(path, st.p, stop, nmin, nmax);
-    if (L > 1) {
-      if ((rc = fs_iterate_lanes(&d.c, d.ctr, d.fac, d.fcap, &lanes, cuts.data(), (uint32_t)cuts.size(), path, d.s)))
-        return rc;
-      ++r.iterate_launches;
-    } else {
+      if ((rc = fs_event_record(d.ev[3], d.s))) return rc;
+      if ((rc = verify_products())) return rc;
+      if (verify_now && (rc = fs_verifyresult(&first, d.ctr, d.verify, d.s))) return rc;
+      // iterate over the n-range in chunks cut where the loops change
+      if ((rc = fs_event_record(d.ev[4], d.s))) return rc;
       size_t ci = 0;
       for (uint32_t n0 = nmin; n0 < nmax;) {
         uint32_t n1 = (nmax - n0 > nchunk_max) ? n0 + nchunk_max : nmax;
@@ -1573,10 +1587,31 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
         ++r.iterate_launches;
         n0 = n1;
       }
+      if ((rc = fs_event_record(d.ev[5], d.s))) return rc;
     }
-    if ((rc = fs_event_record(d.ev[5], d.s))) return rc;
+    verify_pending = verify_now;  // its flag is read with the segment's counters, before any result is taken
+    // fold the checksum of the last lane
     if ((rc = fs_check(&top, d.ctr, d.sum, nmax, d.s))) return rc;
     if ((rc = d.stage())) return rc;
+    bx.begin();  // the wait for the segment's kernels (waitOnEvent, cl_sieve.cpp:237-285)
+    rc = fs_stream_sync(d.s);
+    bx.end();
+    if (rc) return rc;
+    d.staged = true;
+    const uint32_t count = d.h_ctr[FS_CTR_COUNT];
+    if (d.h_ctr[FS_CTR_PRIMEOVF]) {
+      // more candidates than room: every kernel of the segment saw none, so
+      // nothing of it was taken; redo it with room for the count
+      if (++overflows > 3) {
+        fs_set_error_msg("error: gpu prime array overflow");
+        return FS_E_OVERFLOW;
+      }
+      cand_floor = std::max<uint64_t>(cand_floor, (uint64_t)count + count / 2 + 65536);
+      const uint32_t zero = 0;
+      if ((rc = fs_memcpy_htod(d.ctr + FS_CTR_PRIMEOVF, &zero, 4, d.s, 1))) return rc;
+      d.staged = false;
+      continue;
+    }
     for (uint32_t k = 1; k < L; ++k) r.setup_mulmods += (uint64_t)count * lanes.terms[k];
     if (L > 1) r.lanes = L;
     r.steps += (uint64_t)count * (nmax - nmin);
@@ -1586,21 +1621,16 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     const uint64_t nextp = st.p + len;
     const bool last = (nextp >= st.pmax || nextp < st.p);
     st.p = last ? st.pmax : nextp;
-    bx.begin();  // the wait for the segment's kernels (waitOnEvent, cl_sieve.cpp:237-285)
-    rc = fs_stream_sync(d.s);
-    bx.end();
-    if (rc) return rc;
-    d.staged = true;
-    if (verify_pending) {  // the power-table cross-check of the first segment (cl_sieve.cpp:903-907)
+    if (verify_pending && count > 0) {  // the power-table cross-check of the first segment (cl_sieve.cpp:903-907)
       if (d.h_ctr[FS_CTR_POWERTABLE]) {
         fs_set_error_msg("error: power table verification failed");
         return FS_E_POWERTABLE;
       }
-      verify_pending = false;
       verified = true;
       fprintf(stderr, "Verified power table (%llu bytes) starting sieve...\n", (unsigned long long)d.pt_count * 16);
       if (verbose) printf("Verified power table (%llu bytes) starting sieve...\n", (unsigned long long)d.pt_count * 16);
     }
+    verify_pending = false;  // an empty segment checked nothing: the next one checks again
     account(d, &r);
 
     const double tn = now_s();

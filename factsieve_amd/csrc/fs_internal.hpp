@@ -40,28 +40,30 @@ uint64_t fs_cand_capacity(uint64_t lo, uint64_t len);
 // The fault injected with fs_inject_fault for counter idx, taken once: 1 when
 // it was set (and it is then cleared), else 0.
 int fs_take_injected_fault(int idx);
-// Copies p, q and r2 of the first `count` candidates of c to positions
-// k * stride .. k * stride + count - 1 for k = 1 .. lanes - 1 (one launch on
-// s); c->cap must hold (lanes - 1) * stride + count.
-int fs_cands_replicate(const fs_cands* c, uint32_t count, uint32_t stride, uint32_t lanes, fs_stream s);
+// Copies p, q and r2 of the segment's candidates (their count read on the
+// device from d_ctr, at most stride) to positions k * stride + i for
+// k = 1 .. lanes - 1 (one launch on s); c->cap must hold lanes * stride.
+int fs_cands_replicate(const fs_cands* c, const uint32_t* d_ctr, uint32_t stride, uint32_t lanes, fs_stream s);
 
-// Lanes per prime in one launch: lane k is the copy of a segment's `count`
-// candidates at k * stride (stride a multiple of the 256-thread block) and
-// covers n in [nb[k], nb[k + 1]), starting from the setup program of nb[k]!
+// Lanes per prime in one launch: lane k is the copy of a segment's candidates
+// (their count read on the device, at most stride) at k * stride (stride a
+// multiple of the 256-thread block) and covers n in [nb[k], nb[k + 1]),
+// starting from the setup program of nb[k]!
 // (terms[k] terms, as fs_setup takes it, at word poff[k] of the device buffer
 // progs; offsets are multiples of 4 words).  FP64 and hybrid paths.
 #define FS_LANES 4
 struct fs_lane_set {
-  uint32_t lanes, count, stride;
+  uint32_t lanes, stride;
   uint32_t nb[FS_LANES + 1];
   const void* progs;
   uint32_t poff[FS_LANES];
   uint32_t terms[FS_LANES];
 };
-// Chunk [start, end) of every lane's program (multiples of 4 terms); the first
-// chunk starts at 0 and the chunks run in order up to the longest program.
-int fs_setup_lanes(const fs_cands* c, const fs_lane_set* ls, uint32_t start, uint32_t end, int path, fs_stream s);
-// Every lane's n-range in one launch, cut internally at the (at most 4) loop
-// boundaries `cuts`; appends factors as fs_iterate does.
-int fs_iterate_lanes(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t fcap, const fs_lane_set* ls,
-                     const uint32_t* cuts, uint32_t ncuts, int path, fs_stream s);
+// Every lane's setup (its whole program) and then its n-range in one launch,
+// cut internally at the (at most 4) loop boundaries `cuts`; appends factors as
+// fs_iterate does.  The first candidate's setup residue goes to a slot of the
+// verify buffer d_verify, which fs_verifyresult_res0 compares.
+int fs_lanes(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t fcap, const fs_lane_set* ls,
+             const uint32_t* cuts, uint32_t ncuts, void* d_verify, int path, fs_stream s);
+// fs_verifyresult against the setup residue fs_lanes kept (c.res[0] has moved on).
+int fs_verifyresult_res0(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_stream s);

@@ -148,6 +148,11 @@ def test_is_prime_and_verify_match_oracle(fs):
     rng = random.Random(2)
     vals = [rng.getrandbits(64) | 1 for _ in range(3000)] + ol.candidates(10 ** 12, 10 ** 12 + 5000)
     vals += json.load(open(os.path.join(GOLDEN, "spsp2_survivors_1e6.json")))["values"]
+    # around the three-base bound 4,759,123,141 (itself composite, 48781 * 97561) and
+    # strong pseudoprimes to several small bases (3215031751 = 151 * 751 * 28351)
+    vals += [rng.getrandbits(33) | 1 for _ in range(3000)] + list(range(4759123141 - 400, 4759123141 + 400, 2))
+    vals += [3215031751, 2152302898747, 3474749660383, 341550071728321]
+    assert not fs.is_prime(4759123141) and not fs.is_prime(3215031751)
     for v in vals:
         assert fs.is_prime(v) == bool(O.ora_is_prime(v))
     assert [v for v in range(0, 200) if fs.is_prime(v)] == [v for v in range(2, 200)
