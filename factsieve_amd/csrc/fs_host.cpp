@@ -1479,6 +1479,10 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     if (room > d.c.cap) {
       fs_cands_free(&d.c);
       if ((rc = fs_cands_alloc(&d.c, (uint32_t)std::min<uint64_t>(room + room / 4, 0xFFFFFF00ull)))) return rc;
+      if (room > d.c.cap) {
+        fs_set_error_msg("error: gpu prime array overflow");
+        return FS_E_OVERFLOW;
+      }
     }
     // getsegprimes writes at most `need` candidates (lane 0's stride with
     // lanes): the kernels' grids cover that many, and a larger count raises
