@@ -606,8 +606,12 @@ void fs_sort_factors(fs_factor* f, size_t n) { std::stable_sort(f, f + n, factor
 // a few large-n factors still use every thread.
 // Worker threads kept between calls (the role of the OpenMP thread team the
 // reference's verify loop runs on, cl_sieve.cpp:491-515): run(t, fn) executes
-// fn on the calling thread and t - 1 pool threads and returns when all are
-// done.  One parallel region at a time; the threads are joined at exit.
+// fn on the calling thread and on up to t - 1 pool threads.  fn must take its
+// work items from a shared counter until none are left, so any number of
+// callers finishes the region: run returns once the caller's own fn has
+// returned and every helper that entered fn has left it; a helper that wakes
+// after that finds the region closed and does not enter.  One parallel region
+// at a time; the threads are joined at exit.
 namespace {
 class WorkerPool {
  public:
@@ -622,13 +626,14 @@ class WorkerPool {
       }
       task_ = &fn;
       want_ = helpers;
-      pending_ = helpers;
+      open_ = true;
       ++gen_;
     }
     cv_.notify_all();
     fn();
     std::unique_lock<std::mutex> lk(mu_);
-    done_.wait(lk, [this] { return pending_ == 0; });
+    open_ = false;  // helpers not yet inside fn stay out
+    done_.wait(lk, [this] { return active_ == 0; });
     task_ = nullptr;
   }
   ~WorkerPool() {
@@ -648,12 +653,13 @@ class WorkerPool {
       cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
       if (stop_) return;
       seen = gen_;
-      if (idx >= want_) continue;
+      if (!open_ || idx >= want_) continue;
       const std::function<void()>* fn = task_;
+      ++active_;
       lk.unlock();
       (*fn)();
       lk.lock();
-      if (--pending_ == 0) done_.notify_all();
+      if (--active_ == 0) done_.notify_all();
     }
   }
   std::mutex region_mu_, mu_;
@@ -661,8 +667,8 @@ class WorkerPool {
   std::vector<std::thread> threads_;
   const std::function<void()>* task_ = nullptr;
   uint64_t gen_ = 0;
-  int want_ = 0, pending_ = 0;
-  bool stop_ = false;
+  int want_ = 0, active_ = 0;
+  bool open_ = false, stop_ = false;
 };
 
 WorkerPool& worker_pool() {
@@ -671,13 +677,18 @@ WorkerPool& worker_pool() {
 }
 }  // namespace
 
-int64_t fs_verify_all(const fs_factor* f, size_t n, int threads) {
+int64_t fs_verify_all(const fs_factor* f, size_t n, int threads, signed char* prime) {
   if (n == 0) return -1;
   struct Range {
     size_t i;
     uint64_t lo, hi;
   };
-  const uint64_t kRange = 1u << 15;
+  // ranges of at most 2^15 multipliers, smaller when the whole list is short
+  // (about eight ranges per thread)
+  uint64_t work = 0;
+  for (size_t i = 0; i < n; ++i) work += (uint64_t)(f[i].nc < 0 ? -(int64_t)f[i].nc : f[i].nc);
+  const uint64_t kRange = std::min<uint64_t>(
+      1u << 15, std::max<uint64_t>(1u << 11, work / (8 * (uint64_t)std::max(threads, 1)) + 1));
   std::vector<Range> rg;
   std::vector<size_t> first(n + 1);
   std::vector<char> direct(n, 0);  // decided without a product
@@ -692,19 +703,33 @@ int64_t fs_verify_all(const fs_factor* f, size_t n, int threads) {
     for (uint64_t lo = 2; lo <= fn; lo += kRange) rg.push_back({i, lo, std::min<uint64_t>(lo + kRange - 1, fn)});
   }
   first[n] = rg.size();
+  // the primality of each distinct p (the list is sorted by p) goes into the
+  // same parallel pass, for the composite filter
+  std::vector<size_t> distinct;
+  if (prime)
+    for (size_t i = 0; i < n; ++i) {
+      prime[i] = -1;
+      if (i == 0 || f[i].p != f[i - 1].p) distinct.push_back(i);
+    }
+  const size_t items = rg.size() + distinct.size();
   std::vector<uint64_t> part(rg.size());
   if (threads < 1) threads = 1;
-  if ((size_t)threads > rg.size()) threads = (int)std::max<size_t>(rg.size(), 1);
+  if ((size_t)threads > items) threads = (int)std::max<size_t>(items, 1);
   std::atomic<size_t> next(0);
-  const std::function<void()> work = [&]() {
+  const std::function<void()> job = [&]() {
     for (;;) {
       const size_t t = next.fetch_add(1);
-      if (t >= rg.size()) break;
-      part[t] = range_product_mod(f[rg[t].i].p, rg[t].lo, rg[t].hi);
+      if (t >= items) break;
+      if (t < rg.size()) {
+        part[t] = range_product_mod(f[rg[t].i].p, rg[t].lo, rg[t].hi);
+      } else {
+        const size_t i = distinct[t - rg.size()];
+        prime[i] = (signed char)(fs_is_prime(f[i].p) != 0);
+      }
     }
   };
-  if (threads > 1) worker_pool().run(std::min(threads, 256), work);
-  else work();
+  if (threads > 1) worker_pool().run(std::min(threads, 256), job);
+  else job();
   for (size_t i = 0; i < n; ++i) {
     const uint64_t p = f[i].p;
     const int32_t fc = f[i].nc < 0 ? -1 : 1;
@@ -728,7 +753,7 @@ extern "C" int64_t fs_verify_factors(const fs_factor* f, uint64_t n, int threads
     return FS_E_VERIFY;
   }
   try {
-    return fs_verify_all(f, (size_t)n, threads);
+    return fs_verify_all(f, (size_t)n, threads, nullptr);
   } catch (const std::exception&) {
     fs_set_error_msg("fs_verify_factors: host allocation or thread creation failed");
     return FS_E_NOMEM;
@@ -749,20 +774,23 @@ bool fs_verify_uses_gpu(const fs_factor* f, size_t n, int mode) {
   return work > ((uint64_t)1 << 24);
 }
 
-int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s) {
-  return fs_verify_uses_gpu(f, n, mode) ? fs_verify_factors_gpu(f, n, s) : fs_verify_all(f, n, threads);
+int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s, signed char* prime) {
+  if (!fs_verify_uses_gpu(f, n, mode)) return fs_verify_all(f, n, threads, prime);
+  if (prime)
+    for (size_t i = 0; i < n; ++i) prime[i] = -1;
+  return fs_verify_factors_gpu(f, n, s);
 }
 
 // Filter composites (2-PRP p) and fold kept factors into the checksum
 // (cl_sieve.cpp:531-550).  Compacts kept factors to the front; returns count.
-size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose) {
+size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose, const signed char* prime) {
   size_t kept = 0;
   uint64_t lastgood = 0;
   for (size_t i = 0; i < n; ++i) {
     const uint64_t fp = f[i].p;
     const uint32_t fn = (uint32_t)(f[i].nc < 0 ? -f[i].nc : f[i].nc);
     const int32_t fc = f[i].nc < 0 ? -1 : 1;
-    if (fp == lastgood || fs_is_prime(fp)) {
+    if (fp == lastgood || (prime && prime[i] >= 0 ? prime[i] != 0 : fs_is_prime(fp) != 0)) {
       lastgood = fp;
       *checksum += (uint32_t)(fn + (uint32_t)fc);
       f[kept++] = f[i];
@@ -963,7 +991,9 @@ struct Drain {
 
 // getResults (cl_sieve.cpp:439-554): read counters and sums, check the flags,
 // read + sort + verify + filter the factors, append them to factors.txt.
-static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs_result* res) {
+// The last drain of a call leaves the counters as they are (clear = false):
+// the next call on these buffers clears them before its first segment.
+static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs_result* res, bool clear = true) {
   uint32_t ctr[FS_CTR_WORDS];
   uint64_t sum[FS_SUM_WORDS];
   int rc;
@@ -1007,7 +1037,8 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
     res->raw_factors += nf;
     const char* where = fs_verify_uses_gpu(h.data(), nf, o.verify_device) ? "GPU" : "CPU";
     if (verbose) printf("processing %u factors on %s\n", nf, where);
-    const int64_t bad = fs_verify_select(h.data(), nf, o.verify_threads, o.verify_device, d.s);
+    std::vector<signed char> prime(nf);
+    const int64_t bad = fs_verify_select(h.data(), nf, o.verify_threads, o.verify_device, d.s, prime.data());
     if (bad < -1) return (int)bad;
     if (bad >= 0) {
       char buf[256];
@@ -1021,7 +1052,7 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
     fprintf(stderr, "Verified %u factors.\n", nf);
     if (verbose) printf("Verified %u factors.\n", nf);
     uint64_t ck = 0;
-    const size_t kept = fs_filter_factors(h.data(), nf, &ck, verbose);
+    const size_t kept = fs_filter_factors(h.data(), nf, &ck, verbose, prime.data());
     st.checksum += ck;
     st.factorcount += kept;
     if (o.write_files) {
@@ -1044,7 +1075,7 @@ static int drain(DevBufs& d, WorkStatus& st, const fs_options& o, Drain& out, fs
     out.kept.insert(out.kept.end(), h.begin(), h.begin() + kept);
   }
   d.staged = false;
-  return fs_clearresult(d.ctr, d.sum, d.s);
+  return clear ? fs_clearresult(d.ctr, d.sum, d.s) : FS_OK;
 }
 
 // finalizeResults (cl_sieve.cpp:721-771): line-count check, then the checksum trailer.
@@ -1672,7 +1703,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   bx.begin();  // the last drain and the results file (cl_sieve.cpp:1249-1256)
   bx.fraction(1.0);
   if (verbose) printf("Sieve Progress: %.1f%%\n", 100.0);
-  rc = drain(d, st, o, dr, &r);
+  rc = drain(d, st, o, dr, &r, false);
   if (rc == FS_OK && o.write_files) {
     if (!o.test_mode) {
       bx.trickle(st, now_s() - t0);

@@ -25,13 +25,19 @@ std::shared_ptr<const fs_program> fs_program_cached(uint32_t nmin, int path);
 // Sort by (p, |n|); verify on CPU threads (index of first failure or -1);
 // drop composite p and fold kept factors into *checksum (returns kept count).
 void fs_sort_factors(fs_factor* f, size_t n);
-int64_t fs_verify_all(const fs_factor* f, size_t n, int threads);
+// prime: null, or n flags that receive 1/0 for the primality of each
+// distinct p (first of a run of equal p; -1 elsewhere), computed in the same
+// parallel pass for the composite filter.
+int64_t fs_verify_all(const fs_factor* f, size_t n, int threads, signed char* prime = nullptr);
 // CPU or GPU verifier by mode (FS_VERIFY_*) and the amount of work (the GPU
 // above 2^24 multiplications in FS_VERIFY_AUTO); the contract of
 // fs_verify_factors.  fs_verify_uses_gpu tells which one fs_verify_select runs.
 bool fs_verify_uses_gpu(const fs_factor* f, size_t n, int mode);
-int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s);
-size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose);
+int64_t fs_verify_select(const fs_factor* f, size_t n, int threads, int mode, fs_stream s,
+                         signed char* prime = nullptr);
+// prime: null, or the flags fs_verify_all filled (-1 entries are computed here).
+size_t fs_filter_factors(fs_factor* f, size_t n, uint64_t* checksum, bool verbose,
+                         const signed char* prime = nullptr);
 uint32_t fs_auto_segment_candidates();
 // Range checks of setupSearch (cl_sieve.cpp:557-589); FS_OK or FS_E_ARG.
 int fs_validate_ranges(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax);
