@@ -609,9 +609,16 @@ void fs_sort_factors(fs_factor* f, size_t n) { std::stable_sort(f, f + n, factor
 // fn on the calling thread and on up to t - 1 pool threads.  fn must take its
 // work items from a shared counter until none are left, so any number of
 // callers finishes the region: run returns once the caller's own fn has
-// returned and every helper that entered fn has left it; a helper that wakes
-// after that finds the region closed and does not enter.  One parallel region
-// at a time; the threads are joined at exit.
+// returned and every helper that entered fn has left it; a helper that comes
+// later finds the region closed and does not enter (it may also enter the
+// next region's fn more than once, which then finds no items left).  Entry
+// and exit are atomics (an entering helper counts itself active before it
+// checks that the region is open; the caller closes the region before it
+// reads the active count).  Like an OpenMP team under its default wait
+// policy, a helper polls for the next region for a while (kSpin) before it
+// sleeps on a condition variable, so drains a few hundred microseconds apart
+// find their helpers awake.  One region at a time; the threads are joined at
+// exit.
 namespace {
 class WorkerPool {
  public:
@@ -624,51 +631,57 @@ class WorkerPool {
         const int idx = (int)threads_.size();
         threads_.emplace_back([this, idx] { loop(idx); });
       }
-      task_ = &fn;
-      want_ = helpers;
-      open_ = true;
-      ++gen_;
     }
-    cv_.notify_all();
+    task_.store(&fn);
+    want_.store(helpers);
+    open_.store(true);
+    gen_.fetch_add(1);
+    if (sleepers_.load() > 0) {  // a helper counted as sleeping has not yet seen this region
+      std::lock_guard<std::mutex> lk(mu_);
+      cv_.notify_all();
+    }
     fn();
-    std::unique_lock<std::mutex> lk(mu_);
-    open_ = false;  // helpers not yet inside fn stay out
-    done_.wait(lk, [this] { return active_ == 0; });
-    task_ = nullptr;
+    open_.store(false);
+    while (active_.load() != 0) std::this_thread::yield();
   }
   ~WorkerPool() {
     {
       std::lock_guard<std::mutex> lk(mu_);
-      stop_ = true;
+      stop_.store(true);
     }
     cv_.notify_all();
     for (auto& th : threads_) th.join();
   }
 
  private:
+  static constexpr std::chrono::microseconds kSpin{1000};
   void loop(int idx) {
     uint64_t seen = 0;
-    std::unique_lock<std::mutex> lk(mu_);
     for (;;) {
-      cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-      if (stop_) return;
-      seen = gen_;
-      if (!open_ || idx >= want_) continue;
-      const std::function<void()>* fn = task_;
-      ++active_;
-      lk.unlock();
-      (*fn)();
-      lk.lock();
-      if (--active_ == 0) done_.notify_all();
+      const auto t0 = std::chrono::steady_clock::now();
+      while (gen_.load() == seen && !stop_.load() && std::chrono::steady_clock::now() - t0 < kSpin)
+        std::this_thread::yield();
+      if (gen_.load() == seen && !stop_.load()) {
+        std::unique_lock<std::mutex> lk(mu_);
+        sleepers_.fetch_add(1);
+        cv_.wait(lk, [&] { return stop_.load() || gen_.load() != seen; });
+        sleepers_.fetch_sub(1);
+      }
+      if (stop_.load()) return;
+      seen = gen_.load();
+      if (idx >= want_.load()) continue;
+      active_.fetch_add(1);
+      if (open_.load()) (*task_.load())();
+      active_.fetch_sub(1);
     }
   }
   std::mutex region_mu_, mu_;
-  std::condition_variable cv_, done_;
+  std::condition_variable cv_;
   std::vector<std::thread> threads_;
-  const std::function<void()>* task_ = nullptr;
-  uint64_t gen_ = 0;
-  int want_ = 0, active_ = 0;
-  bool open_ = false, stop_ = false;
+  std::atomic<const std::function<void()>*> task_{nullptr};
+  std::atomic<uint64_t> gen_{0};
+  std::atomic<int> want_{0}, active_{0}, sleepers_{0};
+  std::atomic<bool> open_{false}, stop_{false};
 };
 
 WorkerPool& worker_pool() {
