@@ -80,9 +80,12 @@ __global__ void k_set_flag(uint32_t* ctr, uint32_t idx) {
 }
 
 // Segmented candidate generation over [low, high) (getsegprimes.cl:170-239);
-// the block routine is fs_primegen_block.
+// the block routine is fs_primegen_block.  Candidate idx (< c.cap) is stored
+// at idx + k * stride for every lane k < lanes (one copy per lane of a job
+// run as lanes per prime; lanes = 1 otherwise).
 __global__ __launch_bounds__(FS_BLOCK) void k_getsegprimes(uint64_t base, uint64_t high, uint32_t split,
-                                                           fs_cands c, uint32_t* ctr) {
+                                                           fs_cands c, uint32_t* ctr, uint32_t lanes,
+                                                           uint32_t stride) {
   __shared__ uint64_t s_tab[FS_NMASK];
   __shared__ uint64_t s_mask[FS_BLOCK];
   __shared__ uint32_t s_excl[FS_BLOCK];
@@ -93,11 +96,14 @@ __global__ __launch_bounds__(FS_BLOCK) void k_getsegprimes(uint64_t base, uint64
       [&](uint32_t n) { return atomicAdd(&ctr[FS_CTR_COUNT], n); },
       [&](uint32_t idx, uint64_t N, uint64_t q, uint64_t r2) {
         if (idx < c.cap) {
-          c.p[idx] = N;
-          c.q[idx] = q;
-          c.r2[idx] = r2;
-          c.res[idx] = 0;
-          c.nnext[idx] = 0;
+          for (uint32_t k = 0; k < lanes; ++k) {
+            const size_t g = (size_t)idx + (size_t)k * stride;
+            c.p[g] = N;
+            c.q[g] = q;
+            c.r2[g] = r2;
+            c.res[g] = 0;
+            c.nnext[g] = 0;
+          }
         } else {
           atomicOr(&ctr[FS_CTR_PRIMEOVF], 1u);
         }
@@ -182,21 +188,6 @@ __global__ void k_fill_ntab(double* t, double* pt, uint32_t half, uint32_t first
     double* e = pt + (size_t)(j & 1) * 2 * half + 2 * (size_t)(j >> 1);
     e[0] = (double)(a * (a + 1));  // exact while a < 2^26; only used for a < 2^22
     e[1] = (double)(a + 1);
-  }
-}
-
-// Lane copies of a segment's candidates (p, q, r2): entry i of lane k is
-// entry i, at k * stride + i; grid-stride over (lanes - 1) * count entries,
-// count = the segment's candidates (at most stride).
-__global__ void k_replicate(fs_cands c, const uint32_t* ctr, uint32_t stride, uint32_t lanes) {
-  const uint32_t count = fs_live(ctr, stride);
-  const uint64_t total = (uint64_t)(lanes - 1) * count;
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t i = (uint32_t)(t % count);
-    const uint64_t dst = (t / count + 1) * stride + i;
-    c.p[dst] = c.p[i];
-    c.q[dst] = c.q[i];
-    c.r2[dst] = c.r2[i];
   }
 }
 
@@ -525,17 +516,6 @@ extern "C" int fs_inject_fault(int idx) {
   return FS_OK;
 }
 
-int fs_cands_replicate(const fs_cands* c, const uint32_t* d_ctr, uint32_t stride, uint32_t lanes, fs_stream s) {
-  if (lanes < 2 || stride == 0) return FS_OK;
-  if ((uint64_t)lanes * stride > c->cap) {
-    g_last_error = "cands_replicate: capacity below the lanes' copies";
-    return FS_E_ARG;
-  }
-  const uint64_t total = (uint64_t)(lanes - 1) * stride;
-  k_replicate<<<(uint32_t)std::min<uint64_t>((total + 255) / 256, 4096), 256, 0, S(s)>>>(*c, d_ctr, stride, lanes);
-  HIPCHK(hipGetLastError(), "cands_replicate");
-  return FS_OK;
-}
 
 // The injected fault `idx`, once: 1 (and the injection cleared) when it was set.
 int fs_take_injected_fault(int idx) {
@@ -734,6 +714,11 @@ extern "C" int fs_clearresult(uint32_t* d_ctr, uint64_t* d_sum, fs_stream s) {
 
 extern "C" int fs_getsegprimes(uint64_t low, uint64_t high, const fs_cands* c, uint32_t* d_ctr,
                                fs_stream s) {
+  return fs_getsegprimes_lanes(low, high, c, d_ctr, 1, 0, s);
+}
+
+int fs_getsegprimes_lanes(uint64_t low, uint64_t high, const fs_cands* c, uint32_t* d_ctr, uint32_t lanes,
+                          uint32_t stride, fs_stream s) {
   int rc = check_cands(c, "getsegprimes");
   if (rc) return rc;
   if (low > high) {
@@ -752,7 +737,11 @@ extern "C" int fs_getsegprimes(uint64_t low, uint64_t high, const fs_cands* c, u
   // a range of fewer than 2,048 tiles shares each tile's tests among up to 16
   // blocks, so the chip holds at least ~2,048 workgroups
   const uint32_t split = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16, 2048 / blocks));
-  k_getsegprimes<<<(uint32_t)(blocks * split), FS_BLOCK, 0, S(s)>>>(base, high, split, *c, d_ctr);
+  if (lanes < 1 || (lanes > 1 && (stride < c->cap || (uint64_t)(lanes - 1) * stride + c->cap > 0xFFFFFFFFull))) {
+    g_last_error = "getsegprimes: bad lane layout";
+    return FS_E_ARG;
+  }
+  k_getsegprimes<<<(uint32_t)(blocks * split), FS_BLOCK, 0, S(s)>>>(base, high, split, *c, d_ctr, lanes, stride);
   LAUNCHCHK("getsegprimes");
   if (g_inject == FS_CTR_PRIMEOVF) {
     k_set_flag<<<1, 64, 0, S(s)>>>(d_ctr, FS_CTR_PRIMEOVF);

@@ -1535,7 +1535,8 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     fs_cands gen = d.c;
     gen.cap = L > 1 ? stride : (uint32_t)need;
     if ((rc = fs_clearn(d.ctr, d.s)) || (rc = fs_event_record(d.ev[0], d.s)) ||
-        (rc = fs_getsegprimes(st.p, stop, &gen, d.ctr, d.s)) || (rc = fs_event_record(d.ev[1], d.s)))
+        (rc = fs_getsegprimes_lanes(st.p, stop, &gen, d.ctr, L, stride, d.s)) ||
+        (rc = fs_event_record(d.ev[1], d.s)))
       return rc;
     fs_lane_set lanes{};
     lanes.lanes = L;
@@ -1578,7 +1579,6 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
         lanes.poff[k] = d.lpoff[k];
         lanes.terms[k] = d.lterms[k];
       }
-      if ((rc = fs_cands_replicate(&d.c, d.ctr, stride, L, d.s))) return rc;  // the lanes' p, q, r2
     }
 
     // power-table / setup cross-check on the first candidate of the first

@@ -46,10 +46,11 @@ uint64_t fs_cand_capacity(uint64_t lo, uint64_t len);
 // The fault injected with fs_inject_fault for counter idx, taken once: 1 when
 // it was set (and it is then cleared), else 0.
 int fs_take_injected_fault(int idx);
-// Copies p, q and r2 of the segment's candidates (their count read on the
-// device from d_ctr, at most stride) to positions k * stride + i for
-// k = 1 .. lanes - 1 (one launch on s); c->cap must hold lanes * stride.
-int fs_cands_replicate(const fs_cands* c, const uint32_t* d_ctr, uint32_t stride, uint32_t lanes, fs_stream s);
+// fs_getsegprimes that stores candidate idx (< c->cap) at idx + k * stride for
+// every lane k < lanes (stride >= c->cap; the lanes' room must hold
+// (lanes - 1) * stride + c->cap entries).  lanes = 1 is fs_getsegprimes.
+int fs_getsegprimes_lanes(uint64_t low, uint64_t high, const fs_cands* c, uint32_t* d_ctr, uint32_t lanes,
+                          uint32_t stride, fs_stream s);
 
 // Lanes per prime in one launch: lane k is the copy of a segment's candidates
 // (their count read on the device, at most stride) at k * stride (stride a
