@@ -148,6 +148,13 @@ FS_HD bool fs_fp_pair_exact_ok(uint64_t p, uint32_t n1) {
   return fs_fp_pair_ok(p, n1) && (unsigned __int128)p * ((uint64_t)n1 * n1) <= ((unsigned __int128)1 << 53);
 }
 
+// FP64 square: fs_fp_mulmod(s, s) for |s| < 0.8 p needs the multiplier s below
+// 2^50, which p < 2^50 gives.  The quotient estimate then misses X = s^2/p
+// (< 0.64 p < 0.64 * 2^50) by less than 3 * 2^-53 * 0.64 * 2^50 = 0.24, so the
+// result keeps |s'| < 0.74 p, and h - k p stays below 0.74 p + 2^47 < 2^53 (exact).
+#define FS_FP_SQUARE_PMAX ((uint64_t)1 << 50)
+FS_HD bool fs_fp_square_ok(uint64_t p) { return p < FS_FP_SQUARE_PMAX; }
+
 // Canonical [0, p) integer from a symmetric FP64 residue.
 FS_HD uint64_t fs_fp_to_canon(double s, uint64_t p) {
   const int64_t v = (int64_t)s;

@@ -229,6 +229,11 @@ __device__ __forceinline__ T fs_pick(const T* v, uint32_t lane) {
 // a short program starts iterating while longer ones still set up.  The first
 // candidate's setup residue is also written to *res0 for the power-table
 // cross-check (the iterate then moves the residue on).
+// The paired loops run 8 multipliers per block here (16 in k_iterate_*): their
+// two scalar register sets of pairs then take 32 SGPRs instead of 64, which
+// keeps the kernel at 80 (FP64) / 90 (hybrid) SGPRs and 8 waves per SIMD
+// rather than 106 with spills and 7.  Every live wave of a launch that fills
+// most of the GPU once is then resident from the start.
 template <int PATH>
 __global__ __launch_bounds__(FS_BLOCK) void k_lanes(fs_cands c, uint32_t* ctr, fs_factor* fac, uint32_t fcap, LaneArgs a,
                                                     const uint64_t* __restrict__ progs, const double* __restrict__ ntab,
@@ -251,9 +256,9 @@ __global__ __launch_bounds__(FS_BLOCK) void k_lanes(fs_cands c, uint32_t* ctr, f
     const double* nt = ntab + j;
     const double* pt = pairs + (size_t)(j & 1) * 2 * a.thalf + 2 * (size_t)(j >> 1);
     if (PATH == FS_PATH_FP64)
-      s = fs_cand_iter_fp64<FS_UNROLL_DEFAULT, 16>(p, s, lo, b, nt, pt, true, FS_REC(p));
+      s = fs_cand_iter_fp64<FS_UNROLL_DEFAULT, 8>(p, s, lo, b, nt, pt, true, FS_REC(p));
     else
-      s = fs_cand_iter_hybrid<FS_UNROLL_DEFAULT, 16>(p, s, lo, b, nt, pt, true, FS_REC(p));
+      s = fs_cand_iter_hybrid<FS_UNROLL_DEFAULT, 8>(p, s, lo, b, nt, pt, true, FS_REC(p));
     lo = b;
   }
   c.res[i] = s;

@@ -129,9 +129,13 @@ extern "C" int fs_hostemu_setup(int path, uint64_t p, uint32_t nmin, uint64_t* r
     const double pd = (double)p, pinv = 1.0 / pd;
     double s = 1.0;
     for (int64_t k = 0; k < T; ++k) {
-      if (w[k] & FS_PROG_SQUARE) {
-        const uint64_t a = fs_fp_to_canon(s, p);
-        s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+      if (w[k] & FS_PROG_SQUARE) {  // as fs_cand_setup_fp64 squares (one modulus: its own wave)
+        if (fs_fp_square_ok(p)) {
+          s = fs_fp_mulmod(s, s, pd, pinv);
+        } else {
+          const uint64_t a = fs_fp_to_canon(s, p);
+          s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+        }
       }
       double m;
       const uint64_t bits = w[k] & ~FS_PROG_SQUARE;

@@ -174,16 +174,24 @@ struct alignas(32) fs_prog_block {
 };
 
 // FP64 program (p < 2^53): multipliers < 2^50 as double bits.
+// A square is fs_fp_mulmod with the residue itself as the multiplier when every
+// modulus of the wave is below FS_FP_SQUARE_PMAX (|s| < 0.8 p < 2^50, see
+// fs_fp_square_ok); above it, two Montgomery products on the canonical value.
 __device__ __forceinline__ uint64_t fs_cand_setup_fp64(uint64_t p, uint64_t q, uint64_t r2, uint64_t s_in,
                                                        const double* __restrict__ progd, uint32_t start,
                                                        uint32_t end) {
   const fs_prog_block* __restrict__ prog = (const fs_prog_block*)progd;
   const double pd = (double)p, pinv = 1.0 / pd;
   double s = (start == 0) ? 1.0 : fs_canon_to_fp(s_in, p);
+  const bool fpsq = __all(fs_fp_square_ok(p));
   auto square = [&](const fs_prog_block& blk) {  // first use of a block: waits for its load
     if (blk.square()) {
-      const uint64_t a = fs_fp_to_canon(s, p);
-      s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+      if (fpsq) {
+        s = fs_fp_mulmod(s, s, pd, pinv);
+      } else {
+        const uint64_t a = fs_fp_to_canon(s, p);
+        s = fs_canon_to_fp(fs_mul_mod(a, a, p, q, r2), p);
+      }
     }
   };
   auto terms4 = [&](const fs_prog_block& blk) {
