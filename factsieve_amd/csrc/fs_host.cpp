@@ -1165,6 +1165,28 @@ uint32_t fs_auto_segment_candidates() {
   return cus * 4u * 8u * 64u * 8u;
 }
 
+// Candidates per segment of a job: 8 rounds of the GPU's wave slots
+// (fs_auto_segment_candidates) times min(8, 2^22 / n-range), so a segment's
+// p*n steps stay within 8 rounds x 2^22 (about two seconds of iterate) and the
+// segment loop still drains and may checkpoint every few seconds.  Each launch
+// ends with its last waves draining the SIMDs, so fewer, longer launches lose
+// less (n in [1e5, 2e5), p in [1e12, 1e12 + 1e9): 9 segments -> 2, iterate
+// 353 -> 347 ms).  The job's expected candidates (its p-range over ln p) are
+// split into segments of equal size.  The segment loop turns a target into a
+// p-length of target x (ln p - 1), which holds about (1 - 1/ln p) of the
+// target; the target is raised by that factor and 2 % more, so that no sliver
+// is left over.
+static uint32_t segment_target(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nmax) {
+  const uint64_t base = fs_auto_segment_candidates();
+  const uint64_t nr = std::max<uint64_t>(nmax > nmin ? nmax - nmin : 1, 1);
+  const uint64_t cap = base * std::min<uint64_t>(8, std::max<uint64_t>(1, ((uint64_t)1 << 22) / nr));
+  const double lnp = std::max(log(std::max(0.5 * ((double)pmin + (double)pmax), 3.0)), 2.0);
+  const double expect = ((double)pmax - (double)pmin) / lnp;
+  const double nseg = std::max(1.0, ceil(expect / (double)cap));
+  const double t = std::min(1.02 * expect / nseg * lnp / (lnp - 1.0), (double)cap);
+  return (uint32_t)std::max<double>(t, (double)base);
+}
+
 // n values where a cheaper iterate loop stops applying to every modulus of
 // [plo, phi) (the kernels choose their loop per wave by the launch's last n):
 // a launch that ends at such a point runs the n below it with that loop.
@@ -1487,7 +1509,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     if (verbose) printf("Compressed %zu power table terms to %u\n", fs_primes_upto(nmin).size(), d.pt_count);
   }
 
-  const uint32_t target = o.seg_candidates ? o.seg_candidates : fs_auto_segment_candidates();
+  const uint32_t target = o.seg_candidates ? o.seg_candidates : segment_target(pmin, pmax, nmin, nmax);
   const uint32_t nchunk_max = o.n_chunk ? o.n_chunk : (1u << 22);
   bool verified = false, verify_pending = false;
   uint64_t cand_floor = 0;  // room for candidates at least this large after an overflow

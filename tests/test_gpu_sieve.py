@@ -266,6 +266,7 @@ def test_paired_steps_chunked_match_oracle(gpu, n_chunk):
     (2 ** 63 - 3000, 2 ** 63 + 3000, 2 ** 24 - 40, 2 ** 24 + 40),  # high-word limit drops from p >> 34 to p >> 39
     (2 ** 64 - 2 ** 58 - 100000, 2 ** 64 - 2 ** 58 + 100000, 3000, 3500),  # top of the high-word quotient
     ((2 ** 64 - 1) // 3 - 100000, (2 ** 64 - 1) // 3 + 100000, 3000, 3500),  # setup: two-stage REDC / REDC64
+    (2 ** 50 - 100000, 2 ** 50 + 100000, 5000, 5200),   # FP64 setup: squares in FP64 below 2^50, Montgomery above
 ])
 def test_edge_ranges_match_oracle(gpu, args):
     print("edge", args, flush=True)

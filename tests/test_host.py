@@ -302,6 +302,20 @@ def test_residue_step_at_error_bound_edges(fs):
             assert raw % p == res and abs(raw) <= (0.5 + 2 ** -19) * p, (p, s, m, raw)
 
 
+def test_fp64_square_at_error_bound_edges(fs):
+    """The FP64 setup's square (fs_fp_mulmod with the residue as its own
+    multiplier, used while p < 2^50): |s| up to 0.8p at p just below 2^50 keeps
+    the multiplier below 2^50, the result exact and |s'| < 0.74p."""
+    rng = random.Random(12)
+    for p in [2 ** 50 - 1, 2 ** 50 - 27, 2 ** 49 + 9, 2 ** 40 + 15, 1000003]:
+        smax = int(0.8 * p) - 2
+        ss = [smax, -smax, smax - 1, 1, -1, 2] + [rng.randrange(-smax, smax + 1) for _ in range(300)]
+        for s in ss:
+            raw, res = fs.hostemu_mulmod(fs.PATH_FP64, p, s, abs(s))
+            assert res == (s * abs(s)) % p, (p, s)
+            assert raw % p == res and abs(raw) < 0.74 * p, (p, s, raw)
+
+
 def test_fp64_exact_step_at_error_bound_edges(fs):
     """The FP64 step without the product's error term (used when p * n1 <=
     2^53; the iterate loops keep |s| <= 0.51p, so |s*m| <= 0.51 * 2^53): exact,
