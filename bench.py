@@ -116,10 +116,10 @@ def hybrid_hi(pmin, pmax, nmax):
 INT32_MUL_PER_STEP = 11                         # 64-bit Montgomery step: 4 + 3 + 4 partial products
 INT32_MUL_PEAK = VALU_PEAK_TOPS * 1e12          # full-rate v_mad_u64_u32 / v_mul_*_u32 (measured)
 # HBM bytes per candidate per iterate launch, measured with rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE (k_iterate_fp64 on c2: 26.2 B read + 12.0 B written; k_iterate_hybrid
-# on c3: 20.4 + 12.0), each scaled by the counter's reading for a 1 GiB device copy
-# (x2.0 fetch, x1.0 write).
-TRAFFIC_BYTES_PER_CAND = {1: 38.2, 2: 32.4}
+# and WRITE_SIZE (k_iterate_fp64 on c2, launches of 18 M candidates: 23.6 B read + 12.0 B
+# written; k_iterate_hybrid on c3: 20.1 + 12.0), each scaled by the counter's reading for
+# a 1 GiB device copy (x2.0 fetch, x1.0 write).
+TRAFFIC_BYTES_PER_CAND = {1: 35.6, 2: 32.1}
 
 
 def cpu_baseline(pmin, nmin, nmax, target_s=15.0):
