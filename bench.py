@@ -188,7 +188,7 @@ def bench_batched(fs, njobs=64, seed=None):
             "batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
             "bit_exact_vs_plain": same, "autotuned": opts,
             "paths": {k: sum(1 for r in res if r.path == v) for k, v in (("fp64", 1), ("hybrid", 2), ("mont", 3))}}, \
-        {tuple(j): r for j, r in zip(jobs, res)}
+        {tuple(j): r for j, r in zip(jobs, res)}, {tuple(j): p.seconds for j, p in zip(jobs, plain)}
 
 
 def bench_graph_replay(fs, reps=300):
@@ -209,9 +209,11 @@ def bench_graph_replay(fs, reps=300):
     return out
 
 
-def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192, use_graph=1):
+def bench_serving(fs, reference, alone_s=None, rate_frac=0.7, nwu=64, n_chunk=8192, use_graph=1):
     """Poisson arrivals of mixed workunits into the continuous-batching server;
-    p50/p99 arrival-to-result latency and aggregate steps/s."""
+    p50/p99 arrival-to-result latency and aggregate steps/s.  With alone_s (each
+    workunit's wall time run alone, one job at a time on the whole GPU) also the
+    p50/p99 of each workunit's latency over that time (its slowdown)."""
     from factsieve_amd import workloads
     jobs = workloads.mixed_jobs(nwu)
     mean_steps = sum(r.steps for r in reference.values()) / max(len(reference), 1)
@@ -253,10 +255,16 @@ def bench_serving(fs, reference, rate_frac=0.7, nwu=64, n_chunk=8192, use_graph=
                (reference[tuple(jobs[k])].factorcount, reference[tuple(jobs[k])].primecount,
                 reference[tuple(jobs[k])].checksum, reference[tuple(jobs[k])].factors) for k in results)
     ls = sorted(lat.values())
-    return {"workunits": nwu, "arrival_rate_per_s": rate, "target_utilisation": rate_frac, "n_chunk": n_chunk,
+    out = {"workunits": nwu, "arrival_rate_per_s": rate, "target_utilisation": rate_frac, "n_chunk": n_chunk,
             "graph": bool(use_graph), "steps": steps_run, "mean_step_ms": step_s / max(steps_run, 1) * 1e3,
             "p50_latency_s": ls[len(ls) // 2], "p99_latency_s": ls[min(len(ls) - 1, int(0.99 * len(ls)))],
             "makespan_s": makespan, "steps_per_s": steps / makespan, "bit_exact_vs_batched_alone": same}
+    if alone_s:
+        sl = sorted(lat[k] / alone_s[tuple(jobs[k])] for k in lat)
+        out["p50_slowdown"] = sl[len(sl) // 2]
+        out["p99_slowdown"] = sl[min(len(sl) - 1, int(0.99 * len(sl)))]
+        out["largest_alone_s"] = max(alone_s[tuple(j)] for j in jobs)
+    return out
 
 
 def plain_leg(fs, workload, steps, warmup):
@@ -363,7 +371,7 @@ def main():
               flush=True)
         return
     if args.workload == "c4":
-        bat, _ = bench_batched(fs)
+        bat, _, _ = bench_batched(fs)
         print(json.dumps({"metric": "p*n residue steps/s (batched, 64 mixed jobs)",
                           "value": bat["batched_steps_per_s"], "unit": "p*n steps/s", "batched": bat}), flush=True)
         return
@@ -395,11 +403,11 @@ def main():
             if w != args.workload:
                 line["plain_" + w] = plain_leg(fs, w, steps, warm)[0]
     if not args.no_batch:
-        bat, ref_jobs = bench_batched(fs)
+        bat, ref_jobs, alone = bench_batched(fs)
         line["batched"] = bat
         bench_serving.capacity_steps_per_s = bat["batched_steps_per_s"]
-        line["serving"] = bench_serving(fs, ref_jobs)
-        line["serving_without_graph"] = bench_serving(fs, ref_jobs, use_graph=0)
+        line["serving"] = bench_serving(fs, ref_jobs, alone)
+        line["serving_without_graph"] = bench_serving(fs, ref_jobs, alone, use_graph=0)
         line["graph_replay"] = bench_graph_replay(fs)
     if not args.no_cpu_baseline:
         pmin, _, nmin, nmax, _ = WORKLOADS[args.workload]
