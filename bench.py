@@ -209,7 +209,7 @@ def bench_graph_replay(fs, reps=300):
     return out
 
 
-def bench_serving(fs, reference, alone_s=None, rate_frac=0.7, nwu=64, n_chunk=8192, use_graph=1):
+def bench_serving(fs, reference, alone_s=None, rate_frac=0.7, nwu=64, n_chunk=49152, use_graph=1):
     """Poisson arrivals of mixed workunits into the continuous-batching server;
     p50/p99 arrival-to-result latency and aggregate steps/s.  With alone_s (each
     workunit's wall time run alone, one job at a time on the whole GPU) also the

@@ -649,7 +649,7 @@ def batch_autotune(jobs, **opts):
 class Server:
     """Continuous-batching serving loop over the C ABI (fs_server_*)."""
 
-    def __init__(self, mem_budget=0, max_slots=256, n_chunk=8192, verify_threads=16, use_graph=1):
+    def __init__(self, mem_budget=0, max_slots=256, n_chunk=49152, verify_threads=16, use_graph=1):
         o = ServerOptions()
         lib().fs_default_server_options(ctypes.byref(o))
         o.mem_budget, o.max_slots, o.n_chunk, o.verify_threads = mem_budget, max_slots, n_chunk, verify_threads

@@ -426,7 +426,7 @@ typedef void* fs_server;
 typedef struct {
   uint64_t mem_budget;      /* device bytes the server may hold for admitted work (0 = 50% free) */
   uint32_t max_slots;       /* concurrently resident workunits */
-  uint32_t n_chunk;         /* n values per scheduling quantum */
+  uint32_t n_chunk;         /* n values per scheduling quantum (default 49,152) */
   int verify_threads;
   uint32_t use_graph;       /* 1 (default): each quantum's launches replay a captured hipGraph */
 } fs_server_options;

@@ -70,7 +70,7 @@ def test_c5_poisson_serving_full_size_equals_alone(gpu, c4_jobs, c4_plain):
     _, t_batch = gpu.batch(c4_jobs)
     rate = 0.7 * len(c4_jobs) / t_batch  # workunits per second at 70 % of batched capacity
     arrivals = workloads.poisson_arrivals(rate, len(c4_jobs))
-    srv = gpu.Server(n_chunk=8192)
+    srv = gpu.Server()  # the default quantum, as the benchmark serves
     ids, got = {}, {}
     try:
         t0 = time.perf_counter()
