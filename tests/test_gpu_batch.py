@@ -5,7 +5,7 @@ graph capture or admission schedule."""
 import pytest
 
 import oracle_lib as ol
-from jobs_lib import random_jobs
+from jobs_lib import random_jobs, wide_jobs
 from factsieve_amd import workloads
 
 pytestmark = pytest.mark.gpu
@@ -129,6 +129,23 @@ def test_batch_random_jobs_match_oracle(gpu, seed):
         assert (o.factorcount, o.primecount, o.checksum) == (want["factorcount"], want["primecount"],
                                                              want["checksum"]), job
         assert o.factors == want["factors"]
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_wide_random_jobs_plain_and_batch_match_oracle(gpu, seed):
+    """Seeded random jobs with setup programs of every length (nmin up to 2e5)
+    and moduli from 2^20 to 2^64, through the plain sieve and one batch, each
+    against the oracle."""
+    jobs = wide_jobs(seed, 12)
+    res, _ = gpu.batch(jobs)
+    for job, b in zip(jobs, res):
+        print("wide", job, flush=True)
+        o = ol.sieve(*job, threads=8)
+        want = (o["factorcount"], o["primecount"], o["checksum"], o["factors"])
+        r = gpu.sieve(*job)
+        assert b.status == 0, job
+        assert (b.factorcount, b.primecount, b.checksum, b.factors) == want, job
+        assert (r.factorcount, r.primecount, r.checksum, r.factors) == want, job
 
 
 def long_n_mix():
