@@ -74,6 +74,12 @@ __global__ void k_clearresult(uint32_t* ctr, uint64_t* sum) {
   if (t < FS_SUM_WORDS) sum[t] = 0;
   if (t >= 1 && t < FS_CTR_WORDS) ctr[t] = 0;
 }
+// clearn and clearresult in one launch (a job's first segment).
+__global__ void k_clearall(uint32_t* ctr, uint64_t* sum) {
+  const uint32_t t = threadIdx.x;
+  if (t < FS_SUM_WORDS) sum[t] = 0;
+  if (t < FS_CTR_WORDS) ctr[t] = 0;
+}
 
 __global__ void k_set_flag(uint32_t* ctr, uint32_t idx) {
   if (threadIdx.x == 0) atomicOr(&ctr[idx], 1u);
@@ -714,6 +720,13 @@ extern "C" int fs_clearresult(uint32_t* d_ctr, uint64_t* d_sum, fs_stream s) {
   if (!d_ctr || !d_sum) return FS_E_ARG;
   k_clearresult<<<1, 64, 0, S(s)>>>(d_ctr, d_sum);
   LAUNCHCHK("clearresult");
+  return FS_OK;
+}
+
+int fs_clearall(uint32_t* d_ctr, uint64_t* d_sum, fs_stream s) {
+  if (!d_ctr || !d_sum) return FS_E_ARG;
+  k_clearall<<<1, 64, 0, S(s)>>>(d_ctr, d_sum);
+  LAUNCHCHK("clearall");
   return FS_OK;
 }
 

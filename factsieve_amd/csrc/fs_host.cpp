@@ -1229,12 +1229,15 @@ static std::vector<uint32_t> loop_boundaries(int path, uint64_t plo, uint64_t ph
   return b;
 }
 
-// Time the events of the last segment's kernels into the result counters.
-static void account(DevBufs& d, fs_result* r) {
+// Time the events of the last segment's kernels into the result counters
+// (a segment run as lanes records no setup window, its setup being in the one
+// launch, and its iterate window starts at the end of candidate generation,
+// which the lanes launch follows directly on the stream).
+static void account(DevBufs& d, fs_result* r, bool lanes) {
   float ms = 0;
   if (fs_event_elapsed_ms(d.ev[0], d.ev[1], &ms) == FS_OK) r->kernel_ms_primegen += ms;
-  if (fs_event_elapsed_ms(d.ev[2], d.ev[3], &ms) == FS_OK) r->kernel_ms_setup += ms;
-  if (fs_event_elapsed_ms(d.ev[4], d.ev[5], &ms) == FS_OK) r->kernel_ms_iterate += ms;
+  if (!lanes && fs_event_elapsed_ms(d.ev[2], d.ev[3], &ms) == FS_OK) r->kernel_ms_setup += ms;
+  if (fs_event_elapsed_ms(d.ev[lanes ? 1 : 4], d.ev[5], &ms) == FS_OK) r->kernel_ms_iterate += ms;
 }
 
 // Lanes per prime for a job whose one segment holds fewer candidates than half
@@ -1460,7 +1463,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   // factor buffer: 1e6 records, 3e7 below 2^32 (main.cpp:193, cl_sieve.cpp:593-595)
   if ((rc = d.need_factors((pmin < 0xFFFFFFFFull) ? 30000000u : 1000000u))) return rc;
   d.staged = false;
-  if ((rc = fs_clearresult(d.ctr, d.sum, d.s))) return rc;  // the flags and sums; clearn zeroes the count
+  bool clear_all = true;  // the first segment's clear also zeroes the flags and sums (clearresult)
 
   // setup program and the reference power table (verifypow, and setup on MONT),
   // uploaded when they differ from what the buffers hold
@@ -1556,10 +1559,12 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     d.c.grid_hint = (uint32_t)need;
     fs_cands gen = d.c;
     gen.cap = L > 1 ? stride : (uint32_t)need;
-    if ((rc = fs_clearn(d.ctr, d.s)) || (rc = fs_event_record(d.ev[0], d.s)) ||
+    if ((rc = clear_all ? fs_clearall(d.ctr, d.sum, d.s) : fs_clearn(d.ctr, d.s)) ||
+        (rc = fs_event_record(d.ev[0], d.s)) ||
         (rc = fs_getsegprimes_lanes(st.p, stop, &gen, d.ctr, L, stride, d.s)) ||
         (rc = fs_event_record(d.ev[1], d.s)))
       return rc;
+    clear_all = false;
     fs_lane_set lanes{};
     lanes.lanes = L;
     lanes.stride = stride;
@@ -1624,10 +1629,9 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
     };
     const std::vector<uint32_t> cuts = loop_boundaries(path, st.p, stop, nmin, nmax);
     if (L > 1) {
-      // setup and iterate of every lane in one launch (no separate setup window)
-      if ((rc = fs_event_record(d.ev[2], d.s)) || (rc = fs_event_record(d.ev[3], d.s)) ||
-          (rc = fs_event_record(d.ev[4], d.s)))
-        return rc;
+      // setup and iterate of every lane in one launch, timed from the end of
+      // candidate generation (an event record between two kernels of a stream
+      // also delays the second on the GPU, by a few microseconds each)
       if ((rc = fs_lanes(&d.c, d.ctr, d.fac, d.fcap, &lanes, cuts.data(), (uint32_t)cuts.size(), d.verify, path,
                          d.s)))
         return rc;
@@ -1701,7 +1705,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
       if (verbose) printf("Verified power table (%llu bytes) starting sieve...\n", (unsigned long long)d.pt_count * 16);
     }
     verify_pending = false;  // an empty segment checked nothing: the next one checks again
-    account(d, &r);
+    account(d, &r, L > 1);
 
     const double tn = now_s();
     if (tn - last_prog > 2.0) {  // fraction done every 2 s (cl_sieve.cpp:1119-1124)
@@ -1735,6 +1739,7 @@ extern "C" int fs_sieve(uint64_t pmin, uint64_t pmax, uint32_t nmin, uint32_t nm
   }
 
   st.p = st.pmax;
+  if (clear_all && (rc = fs_clearresult(d.ctr, d.sum, d.s))) return rc;  // no segment ran
   bx.begin();  // the last drain and the results file (cl_sieve.cpp:1249-1256)
   bx.fraction(1.0);
   if (verbose) printf("Sieve Progress: %.1f%%\n", 100.0);

@@ -74,3 +74,5 @@ int fs_lanes(const fs_cands* c, uint32_t* d_ctr, fs_factor* d_factor, uint32_t f
              const uint32_t* cuts, uint32_t ncuts, void* d_verify, int path, fs_stream s);
 // fs_verifyresult against the setup residue fs_lanes kept (c.res[0] has moved on).
 int fs_verifyresult_res0(const fs_cands* c, uint32_t* d_ctr, void* d_verify, fs_stream s);
+// fs_clearn and fs_clearresult in one launch.
+int fs_clearall(uint32_t* d_ctr, uint64_t* d_sum, fs_stream s);
