@@ -69,8 +69,14 @@ OPS_PER_STEP_FP64_PAIR = 3.5                    # FP64 paired steps (64 nmax <= 
 OPS_PER_STEP_FP64_PAIR_EXACT = 2.5              # paired steps with an exact product (p nmax^2 <= 2^53): 8 x (4 + 1)
 # VALU lane-ops per setup program term (PMC of k_setup_fp64 / k_setup_hybrid):
 # counted in the iterate window of a job run as lanes per prime, whose one
-# launch (k_lanes) sets up and iterates
+# launch (k_lanes) sets up and iterates.  FP64 setups square in FP64 while every
+# modulus is below 2^50 (6.04 per term) and with Montgomery products above (6.3).
 SETUP_OPS_PER_TERM = {1: 6.3, 2: 18.1}
+SETUP_OPS_PER_TERM_FP64_FPSQ = 6.04
+
+
+def setup_ops_per_term(path, pmax):
+    return SETUP_OPS_PER_TERM_FP64_FPSQ if path == 1 and pmax - 1 < 2 ** 50 else SETUP_OPS_PER_TERM[path]
 
 
 def fp64_ops_at(pmin, pmax, n1):
@@ -180,6 +186,7 @@ def bench_batched(fs, njobs=64, seed=None):
     same = all((r.factorcount, r.primecount, r.checksum, r.factors) ==
                (p.factorcount, p.primecount, p.checksum, p.factors) and r.status == 0 for r, p in zip(res, plain))
     steps = sum(r.steps for r in res)
+    ops = batch_ops(fs, jobs, res)
     return {"workload": f"{njobs} mixed jobs (seed {workloads.DEFAULT_SEED if seed is None else seed}): n_start "
                         "U[1e3,5e5], n_len lognormal(median 2e4, sigma 1), p_start 2^U[32,63], p_len "
                         "lognormal(median 2^24, sigma 1)",
@@ -187,8 +194,31 @@ def bench_batched(fs, njobs=64, seed=None):
             "batched_steps_per_s": steps / t_batch, "plain_steps_per_s": steps / t_plain,
             "batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
             "bit_exact_vs_plain": same, "autotuned": opts,
+            "roofline_wall": {"bound": "valu", "kernel": "k_bsieve (whole batch: wall time, candidate generation "
+                                                        "and host work included)",
+                              "achieved": ops / t_batch / 1e12, "peak": VALU_PEAK_TOPS, "unit": "TOP/s (VALU lane-ops)",
+                              "frac": ops / t_batch / 1e12 / VALU_PEAK_TOPS, "lane_ops": ops},
             "paths": {k: sum(1 for r in res if r.path == v) for k, v in (("fp64", 1), ("hybrid", 2), ("mont", 3))}}, \
         {tuple(j): r for j, r in zip(jobs, res)}, {tuple(j): p.seconds for j, p in zip(jobs, plain)}
+
+
+def batch_ops(fs, jobs, res):
+    """VALU lane-operations of a batch by the same model as the plain legs: each
+    job's steps at the loop its waves take for the whole n-range (the batch runs
+    it unchunked) plus its candidates' setup program terms."""
+    total = 0.0
+    for (pmin, pmax, nmin, nmax), r in zip(jobs, res):
+        if r.path == 1:
+            per_step = fp64_ops_at(pmin, pmax, nmax)
+        elif r.path == 2:
+            per_step = (OPS_PER_STEP_HYBRID_PAIR if hybrid_pair(pmin, pmax, nmax) else
+                        OPS_PER_STEP_HYBRID_HI if hybrid_hi(pmin, pmax, nmax) else OPS_PER_STEP[2])
+        else:
+            per_step = OPS_PER_STEP[3]
+        total += r.steps * per_step
+        if r.path in SETUP_OPS_PER_TERM:
+            total += r.primecount * fs.setup_program_terms(nmin, r.path) * setup_ops_per_term(r.path, pmax)
+    return total
 
 
 def bench_graph_replay(fs, reps=300):
@@ -295,7 +325,7 @@ def plain_leg(fs, workload, steps, warmup):
         ops = fp64_ops(pmin, pmax, nmin, nmax)
     window_ops = steps_total * ops
     if ref.lanes > 1:  # the lanes' launch also runs every lane's setup
-        window_ops += sum(r.setup_mulmods for r in results) * SETUP_OPS_PER_TERM[ref.path]
+        window_ops += sum(r.setup_mulmods for r in results) * setup_ops_per_term(ref.path, pmax)
         ops = window_ops / steps_total
     achieved = window_ops / (it_ms * 1e-3) / 1e12
     traffic = None

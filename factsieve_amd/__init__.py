@@ -341,6 +341,11 @@ def setup_program(nmin, path):
     return n, list(words[:nw])
 
 
+def setup_program_terms(nmin, path):
+    """Number of setup program terms for nmin on a path (no words copied)."""
+    return check(lib().fs_setup_program(nmin, path, None))
+
+
 def hostemu_setup(path, p, nmin):
     r = u64()
     check(lib().fs_hostemu_setup(path, p, nmin, ctypes.byref(r)))
