@@ -47,6 +47,21 @@ def test_reference_self_test_case(gpu, k):
     assert (r.factorcount, r.primecount, f"{r.checksum:016X}") == (nf, pc, ck)
 
 
+@pytest.mark.parametrize("path", ["fp64", "hybrid", "mont"])
+@pytest.mark.parametrize("k", range(5))
+def test_reference_self_test_case_every_path(gpu, k, path):
+    """The reference's five self-test vectors with each arithmetic formulation
+    forced (all five ranges lie below 2^53, so every path covers them): the
+    FP64, hybrid and Montgomery residues give the reference's counts and
+    checksums alike."""
+    pmin, pmax, nmin, nmax, nf, pc, ck = KATS[k]
+    forced = {"fp64": gpu.PATH_FP64, "hybrid": gpu.PATH_HYBRID, "mont": gpu.PATH_MONT}[path]
+    print(f"KAT {k + 1} on the {path} path", flush=True)
+    r = gpu.sieve(pmin, pmax, nmin, nmax, verify_threads=16, path=forced)
+    assert r.path == forced
+    assert (r.factorcount, r.primecount, f"{r.checksum:016X}") == (nf, pc, ck)
+
+
 def test_c1_golden_factor_list(gpu):
     r = gpu.sieve(*C1["range"])
     assert r.factorcount == 25 and r.primecount == 108529 and f"{r.checksum:016X}" == C1["checksum"]
