@@ -485,6 +485,38 @@ def test_repeated_sieves_keep_device_memory_flat(gpu):
     assert drift < 64 * 2 ** 20
 
 
+def test_concurrent_sieves_from_host_threads(gpu):
+    """fs_sieve called from four host threads at once (each call leases its own
+    stream and buffers; verification threads are shared): every result equals
+    the same job run alone."""
+    import threading
+    jobs = [tuple(C1["range"]), (10 ** 8, 10 ** 8 + 300000, 1000, 3000), (2 ** 40, 2 ** 40 + 200000, 20000, 24000),
+            (2 ** 62, 2 ** 62 + 100000, 5000, 9000), (4 * 10 ** 6, 5 * 10 ** 6, 11, 997),
+            ((16385 << 39) - 100000, (16385 << 39) + 100000, 2000, 12000)]
+    want = {j: gpu.sieve(*j, verify_threads=4) for j in jobs}
+    got, errors = [], []
+
+    def worker(t):
+        try:
+            for k in range(6):
+                j = jobs[(t + k) % len(jobs)]
+                got.append((j, gpu.sieve(*j, verify_threads=4)))
+        except Exception as e:  # reported below, on the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    assert not errors, errors
+    assert len(got) == 24
+    for j, r in got:
+        w = want[j]
+        assert (r.factorcount, r.primecount, r.checksum, r.factors) == \
+            (w.factorcount, w.primecount, w.checksum, w.factors), j
+
+
 def reference_order(factors):
     """True when a factor list is in the reference's order: p ascending, then
     |n| ascending (factorcompare, cl_sieve.cpp:421-436), with no duplicates."""
