@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <vector>
 #include "fs_batch_internal.hpp"
@@ -259,9 +260,11 @@ fs_batch_bufs& fs_batch_engine() {
 }
 
 uint32_t fs_device_cus() {
-  static uint32_t cus = 0;
+  static std::atomic<uint32_t> cached{0};  // callers on several host threads may race to fill it
+  uint32_t cus = cached.load();
   if (!cus) {
     if (fs_device_info(nullptr, 0, &cus, nullptr, nullptr) != FS_OK || !cus) cus = 256;
+    cached.store(cus);
   }
   return cus;
 }
