@@ -107,15 +107,16 @@ int fs_malloc(void** dptr, size_t bytes);                 /* clCreateBuffer */
 int fs_free(void* dptr);                                  /* sclReleaseMemObject */
 int fs_memcpy_htod(void* dst, const void* src, size_t bytes, fs_stream s, int blocking); /* sclWrite[NB] */
 int fs_memcpy_dtoh(void* dst, const void* src, size_t bytes, fs_stream s, int blocking); /* sclRead[NB] */
-int fs_memset(void* dptr, int value, size_t bytes, fs_stream s);
+int fs_memset(void* dptr, int value, size_t bytes, fs_stream s); /* no OpenCL call: the reference clears
+                                                   device words with its clearn/clearresult kernels */
 int fs_memcpy_dtod(void* dst, const void* src, size_t bytes, fs_stream s); /* device copy (also the
                                                    HBM-counter calibration kernel of the benchmark) */
-int fs_cands_alloc(fs_cands* c, uint32_t cap);
-int fs_cands_free(fs_cands* c);
+int fs_cands_alloc(fs_cands* c, uint32_t cap);            /* clCreateBuffer(d_primes) (cl_sieve.cpp:1057) */
+int fs_cands_free(fs_cands* c);                           /* clReleaseMemObject (cl_sieve.cpp:79-96) */
 
 /* -------------------------------------------- streams, events, graphs */
-int fs_stream_create(fs_stream* s);
-int fs_stream_destroy(fs_stream s);
+int fs_stream_create(fs_stream* s);                       /* clCreateCommandQueue (main.cpp:266) */
+int fs_stream_destroy(fs_stream s);                       /* sclReleaseClHard (simpleCL.c) */
 int fs_stream_sync(fs_stream s);                          /* sclFinish / sleepCPU (cl_sieve.cpp:289-354) */
 int fs_event_create(fs_event* e);
 int fs_event_destroy(fs_event e);
@@ -123,6 +124,8 @@ int fs_event_record(fs_event e, fs_stream s);             /* clEnqueueMarker */
 int fs_event_query(fs_event e);                           /* 1 done, 0 pending (clGetEventInfo) */
 int fs_event_sync(fs_event e);                            /* waitOnEvent (cl_sieve.cpp:237-285) */
 int fs_event_elapsed_ms(fs_event start, fs_event stop, float* ms); /* ProfilesclEnqueueKernel */
+/* Graphs have no counterpart in the reference: the batched form captures its
+ * segment loop (k_bclear ... k_bsieve) once per shape and replays it. */
 int fs_graph_begin(fs_stream s);                          /* start capturing launches on s */
 int fs_graph_end(fs_stream s, fs_graph* g);               /* instantiate the captured graph */
 int fs_graph_launch(fs_graph g, fs_stream s);
