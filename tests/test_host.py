@@ -195,6 +195,35 @@ def test_verify_factor_list_matches_single(fs):
     assert fs.verify_factors([], threads=4) == -1
 
 
+def test_verify_factors_from_concurrent_callers(fs):
+    """The verifier's worker threads are shared: callers on six host threads at
+    once (each call one parallel region, entered one at a time) all get the
+    answers a lone caller gets, good lists and lists with a planted bad factor."""
+    import threading
+    good = [tuple(f) for f in json.load(open(os.path.join(GOLDEN, "c1.json")))["factors"]]
+    bad = list(good)
+    bad[7] = (bad[7][0], -bad[7][1])
+    lists = [(good, -1), (bad, 7), (good[:5], -1), (bad[3:12], 4)]
+    errors = []
+
+    def worker(t):
+        try:
+            for k in range(40):
+                lst, want = lists[(t + k) % len(lists)]
+                got = fs.verify_factors(lst, threads=1 + (t + k) % 8)
+                if got != want:
+                    errors.append((t, k, got, want))
+        except Exception as e:  # reported on the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(t,)) for t in range(6)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=300)
+    assert not errors, errors[:5]
+
+
 def test_verify_factors_null_list_is_an_error(fs):
     """A null list with n > 0 must not read as "all pass" (-1)."""
     assert fs.lib().fs_verify_factors(None, 3, 2) == fs.FS_E_VERIFY
