@@ -108,6 +108,44 @@ def build_sanitized(jobs=8):
     return lib
 
 
+# ThreadSanitizer on the host code only (again after -Xarch_host): the worker
+# threads of the factor verifier, the buffer pools and the caches are shared by
+# every host thread that calls the library.
+TSAN_DIR = os.path.join(HERE, "_build", "tsan")
+CLANGXX = os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(HIPCC))), "llvm", "bin", "clang++")
+
+
+def build_tsan(driver_src, jobs=8):
+    """The library with its host code under ThreadSanitizer, into
+    factsieve_amd/_build/tsan/, and a host-only driver program (driver_src, C++
+    that calls the C ABI) linked against it.  Returns the driver's path."""
+    os.makedirs(TSAN_DIR, exist_ok=True)
+    flags = [f for f in HIP_FLAGS if f != "-O3"] + ["-O1", "-g", "-Xarch_host", "-fsanitize=thread"]
+
+    def one(src):
+        obj = os.path.join(TSAN_DIR, os.path.splitext(src)[0] + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        _run([HIPCC] + flags + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(one, SOURCES))
+    rt = sorted(glob.glob(os.path.join(os.path.dirname(os.path.dirname(CLANGXX)), "lib", "clang", "*", "lib", "*",
+                                       "libclang_rt.tsan-x86_64.so")))
+    if not rt:
+        raise RuntimeError("no libclang_rt.tsan-x86_64.so beside " + CLANGXX)
+    rt_dir = os.path.dirname(rt[-1])
+    lib = os.path.join(TSAN_DIR, "libfactsieve.so")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-shared-libsan", "-o", lib + ".tmp"] + objs
+         + ["-Xarch_host", "-fsanitize=thread", "-Wl,-rpath," + rt_dir, "-lpthread"])
+    os.replace(lib + ".tmp", lib)
+    drv = os.path.join(TSAN_DIR, "driver")
+    _run([CLANGXX, "-std=c++17", "-O1", "-g", "-fsanitize=thread", "-shared-libsan", "-I", os.path.join(ROOT, "include"),
+          driver_src, "-L", TSAN_DIR, "-lfactsieve", "-Wl,-rpath," + TSAN_DIR, "-Wl,-rpath," + rt_dir, "-lpthread",
+          "-o", drv])
+    return drv
+
+
 def build_oracle():
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "all"])
