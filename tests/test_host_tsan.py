@@ -3,9 +3,10 @@
 factsieve_amd.build.build_tsan() compiles the library with -fsanitize=thread
 on the host side only (the gfx950 code is the ordinary build) and links the
 host-only driver tests/tsan_verify_driver.cpp against it; the driver calls the
-factor verifier from six threads at once.  The verifier's worker threads
-enter and leave each parallel region through atomics, so a data race there
-would be reported here."""
+factor verifier from six threads at once, and builds setup programs and power
+tables on two more.  The verifier's worker threads enter and leave each
+parallel region through atomics, and the prime list behind the programs is a
+shared cache, so a data race in either would be reported here."""
 import json
 import os
 import subprocess

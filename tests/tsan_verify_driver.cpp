@@ -2,6 +2,8 @@
 // fs_verify_factors at once, 200 times each, on a list of real factors (read
 // from stdin as "p nc" lines) and on the same list with one sign flipped; every
 // call must return what a lone caller gets (-1, or the flipped factor's index).
+// Beside them, two threads build setup programs and power tables for changing
+// nmin (the shared prime-list cache), checked against a lone caller's counts.
 #include <stdint.h>
 #include <stdio.h>
 #include <atomic>
@@ -28,6 +30,24 @@ int main() {
         const bool b = ((t + k) & 1) != 0;
         const std::vector<fs_factor>& l = b ? bad : good;
         if (fs_verify_factors(l.data(), l.size(), 1 + (t + k) % 8) != (b ? 7 : -1)) wrong++;
+      }
+    });
+  const uint32_t nmins[] = {1000, 5000, 20000, 1000, 77777, 5000};
+  std::vector<int64_t> want;
+  for (uint32_t n : nmins) {
+    want.push_back(fs_setup_program(n, FS_PATH_FP64, nullptr));
+    want.push_back(fs_setup_program(n, FS_PATH_HYBRID, nullptr));
+    want.push_back(fs_power_table(n, nullptr, nullptr, nullptr));
+  }
+  for (int t = 0; t < 2; ++t)
+    threads.emplace_back([&, t] {
+      for (int k = 0; k < 30; ++k) {
+        const size_t i = (size_t)(t + k) % 6;
+        const uint32_t n = nmins[i];
+        if (fs_setup_program(n, FS_PATH_FP64, nullptr) != want[3 * i] ||
+            fs_setup_program(n, FS_PATH_HYBRID, nullptr) != want[3 * i + 1] ||
+            fs_power_table(n, nullptr, nullptr, nullptr) != want[3 * i + 2])
+          wrong++;
       }
     });
   for (auto& th : threads) th.join();
